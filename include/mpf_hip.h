@@ -1,0 +1,231 @@
+/*
+ * mpf_hip.h -- the C ABI of the MI355X (gfx950) particle-grid library, libmpfhip.so.
+ *
+ * Plain pointers, sizes and scalars; no torch or HIP types in the signatures (the stream
+ * is an opaque pointer, a hipStream_t, 0 for the null stream).  Every pointer to marker,
+ * field or edge data is a DEVICE pointer; geometry arrays of the generic entry points are
+ * HOST arrays of ndim entries.  Each call is asynchronous on the given stream and returns
+ * MPF_OK or an error code, with the message in mpf_last_error().
+ *
+ * The entry points replace the reference's kernel catalogue (max-models/mpifield,
+ * src/mpifield/kernels/catalog.py:77-114) behind its kernel convention
+ * (src/mpifield/weighting.py:22-38, src/mpifield/edges.py:23-31):
+ *
+ *   scatter(positions, values, field, *geometry[, degree])      adds into field
+ *   gather(positions, field, result, *geometry[, degree])        overwrites result
+ *   deposit_paths(start, end, charges, edges, *geometry[, degree])  adds into edges
+ *
+ * with geometry = (*lower, *cell_size, *num_nodes, *code), code 0 bounded (stencils fold
+ * onto the wall node), 1 periodic, 2 clamped spline knots (_types.py:27-44).
+ *
+ * Layouts (weighting.py:31-35): positions (ndim, N) and values/results (ncomp, N) are
+ * component-major rows of `pos_stride` / `val_stride` / `result_stride` ELEMENTS apart
+ * (rows may have gaps, storage[:, :n]); fields are C-contiguous num_nodes + (ncomp,)
+ * float64; edge fields num_nodes (+ degree - 1 on a clamped axis) + (ndim,) float64.
+ * Marker arrays are float32 or float64 (`dtype`, MPF_F32 / MPF_F64); cell selection is
+ * always done in float64 exactly as the reference does it.
+ *
+ * Deposit modes:
+ *   MPF_MODE_ATOMIC  float64 atomic adds (the plain GPU form of the reference's CUDA
+ *                    kernels, e.g. scatter_quadratic_3d_cuda.cu:30-59); order-dependent
+ *                    in the last bits.
+ *   MPF_MODE_FIXED   each contribution rounded to an int64 fixed-point value of scale
+ *                    2^k and summed in integers: bit-reproducible under any marker order.
+ *                    k is chosen on the device so that N * max|value| * 2^k < 2^61
+ *                    (MPF_SCALE_AUTO) or fixed by the caller.
+ *
+ * A marker whose values are all zero deposits nothing and its position is not read
+ * (weighting.py:34-36); gathers read every position (weighting.py:220-222).
+ */
+#ifndef MPF_HIP_H
+#define MPF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mpf_context mpf_context; /* scratch owned by the library (accumulators, bins) */
+typedef void *mpf_stream;               /* a hipStream_t */
+
+#define MPF_OK 0
+#define MPF_ERR_ARGUMENT 1
+#define MPF_ERR_HIP 2
+#define MPF_ERR_UNSUPPORTED 3
+
+#define MPF_F32 0
+#define MPF_F64 1
+
+#define MPF_MODE_ATOMIC 0
+#define MPF_MODE_FIXED 1
+#define MPF_SCALE_AUTO (-100000)
+
+#define MPF_KIND_LINEAR 0
+#define MPF_KIND_QUADRATIC 1
+#define MPF_KIND_SPLINE 2
+
+/* ---- library --------------------------------------------------------------------- */
+
+/* The message of the last failed call on this thread ("" if none). */
+const char *mpf_last_error(void);
+/* ABI version, bumped on any signature change. */
+int mpf_abi_version(void);
+/* A context on `device` (its scratch grows on demand); NULL on failure.  A NULL context
+ * argument anywhere below means the library's default context of the current device. */
+mpf_context *mpf_context_create(int device);
+void mpf_context_destroy(mpf_context *ctx);
+
+/* Kernel timing for benchmarks: while enabled, the library records HIP events on the
+ * launch stream around its main kernels ("scatter_direct", "gather_direct",
+ * "scatter_tiled", "gather_tiled"); mpf_profile_query waits for them and returns the
+ * summed milliseconds and the number of timed launches of one name. */
+int mpf_profile_enable(int on);
+int mpf_profile_reset(void);
+int mpf_profile_query(const char *name, double *total_ms, int64_t *launches);
+
+/* ---- generic node kernels -------------------------------------------------------- */
+
+/* kind MPF_KIND_*, degree 0..3 (spline only), ndim 1..3; lower/cell_size/num_nodes/code
+ * are host arrays of ndim entries.  Replaces every scatter_* kernel of catalog.py. */
+int mpf_scatter(int kind, int degree, int ndim, const void *positions, int64_t pos_stride, const void *values,
+                int64_t val_stride, int dtype, int64_t n, int64_t ncomp, double *field, const double *lower,
+                const double *cell_size, const int64_t *num_nodes, const int64_t *code, int mode, int scale_exp,
+                mpf_context *ctx, mpf_stream stream);
+
+/* Replaces every gather_* kernel of catalog.py; result has the marker dtype. */
+int mpf_gather(int kind, int degree, int ndim, const void *positions, int64_t pos_stride, int dtype, int64_t n,
+               const double *field, int64_t ncomp, void *result, int64_t result_stride, const double *lower,
+               const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
+               mpf_stream stream);
+
+/* ---- the catalogue's node kernels, argument for argument ------------------------- */
+
+#define MPF_SCATTER_IN \
+  const void *positions, int64_t pos_stride, const void *values, int64_t val_stride, int dtype, int64_t n, \
+      int64_t ncomp, double *field
+#define MPF_GATHER_IN \
+  const void *positions, int64_t pos_stride, int dtype, int64_t n, const double *field, int64_t ncomp, void *result, \
+      int64_t result_stride
+#define MPF_GEOM_1D double lower_x, double cell_size_x, int64_t num_x, int64_t periodic_x
+#define MPF_GEOM_2D                                                                                               \
+  double lower_x, double lower_y, double cell_size_x, double cell_size_y, int64_t num_x, int64_t num_y, \
+      int64_t periodic_x, int64_t periodic_y
+#define MPF_GEOM_3D                                                                                          \
+  double lower_x, double lower_y, double lower_z, double cell_size_x, double cell_size_y, double cell_size_z, \
+      int64_t num_x, int64_t num_y, int64_t num_z, int64_t periodic_x, int64_t periodic_y, int64_t periodic_z
+#define MPF_TAIL mpf_context *ctx, mpf_stream stream
+
+/* kernels/scatter_linear_{1,2,3}d (weighting.py:259-336: the 1D kernel one component at a
+ * time, the 3D ones on rows (nx*ny*nz, ncomp) -- the same memory as num_nodes + (ncomp,)) */
+int mpf_scatter_linear_1d(MPF_SCATTER_IN, MPF_GEOM_1D, int mode, MPF_TAIL);
+int mpf_scatter_linear_2d(MPF_SCATTER_IN, MPF_GEOM_2D, int mode, MPF_TAIL);
+int mpf_scatter_linear_3d(MPF_SCATTER_IN, MPF_GEOM_3D, int mode, MPF_TAIL);
+/* kernels/gather_linear_{1,2,3}d */
+int mpf_gather_linear_1d(MPF_GATHER_IN, MPF_GEOM_1D, MPF_TAIL);
+int mpf_gather_linear_2d(MPF_GATHER_IN, MPF_GEOM_2D, MPF_TAIL);
+int mpf_gather_linear_3d(MPF_GATHER_IN, MPF_GEOM_3D, MPF_TAIL);
+/* kernels/scatter_quadratic_{1,2,3}d (scatter_quadratic_3d_pyccel.py:130-218) */
+int mpf_scatter_quadratic_1d(MPF_SCATTER_IN, MPF_GEOM_1D, int mode, MPF_TAIL);
+int mpf_scatter_quadratic_2d(MPF_SCATTER_IN, MPF_GEOM_2D, int mode, MPF_TAIL);
+int mpf_scatter_quadratic_3d(MPF_SCATTER_IN, MPF_GEOM_3D, int mode, MPF_TAIL);
+/* kernels/gather_quadratic_{1,2,3}d (gather_quadratic_3d_pyccel.py:117-186) */
+int mpf_gather_quadratic_1d(MPF_GATHER_IN, MPF_GEOM_1D, MPF_TAIL);
+int mpf_gather_quadratic_2d(MPF_GATHER_IN, MPF_GEOM_2D, MPF_TAIL);
+int mpf_gather_quadratic_3d(MPF_GATHER_IN, MPF_GEOM_3D, MPF_TAIL);
+/* kernels/scatter_spline_{1,2,3}d, degree 0..3 last (scatter_spline_3d_pyccel.py:107-123) */
+int mpf_scatter_spline_1d(MPF_SCATTER_IN, MPF_GEOM_1D, int degree, int mode, MPF_TAIL);
+int mpf_scatter_spline_2d(MPF_SCATTER_IN, MPF_GEOM_2D, int degree, int mode, MPF_TAIL);
+int mpf_scatter_spline_3d(MPF_SCATTER_IN, MPF_GEOM_3D, int degree, int mode, MPF_TAIL);
+/* kernels/gather_spline_{1,2,3}d */
+int mpf_gather_spline_1d(MPF_GATHER_IN, MPF_GEOM_1D, int degree, MPF_TAIL);
+int mpf_gather_spline_2d(MPF_GATHER_IN, MPF_GEOM_2D, int degree, MPF_TAIL);
+int mpf_gather_spline_3d(MPF_GATHER_IN, MPF_GEOM_3D, int degree, MPF_TAIL);
+
+/* ---- path deposits (edges.py:187-215) -------------------------------------------- */
+
+#define MPF_PATHS_IN                                                                                           \
+  const void *start, int64_t start_stride, const void *end, int64_t end_stride, const void *charges, int dtype, \
+      int64_t n, double *edges
+
+/* WHITNEY: kernels/deposit_paths_{1,2,3}d (deposit_paths_2d_pyccel.py:141-183) */
+int mpf_deposit_paths_1d(MPF_PATHS_IN, MPF_GEOM_1D, int mode, MPF_TAIL);
+int mpf_deposit_paths_2d(MPF_PATHS_IN, MPF_GEOM_2D, int mode, MPF_TAIL);
+int mpf_deposit_paths_3d(MPF_PATHS_IN, MPF_GEOM_3D, int mode, MPF_TAIL);
+/* QUADRATIC_EDGES / CUBIC_EDGES: kernels/deposit_paths_spline_{1,2,3}d, degree 1..3
+ * (deposit_paths_spline_2d_pyccel.py:19-142) */
+int mpf_deposit_paths_spline_1d(MPF_PATHS_IN, MPF_GEOM_1D, int degree, int mode, MPF_TAIL);
+int mpf_deposit_paths_spline_2d(MPF_PATHS_IN, MPF_GEOM_2D, int degree, int mode, MPF_TAIL);
+int mpf_deposit_paths_spline_3d(MPF_PATHS_IN, MPF_GEOM_3D, int degree, int mode, MPF_TAIL);
+
+/* Generic form: kind 0 Whitney, 1 spline edges of `degree`. */
+int mpf_deposit_paths(int kind, int degree, int ndim, const void *start, int64_t start_stride, const void *end,
+                      int64_t end_stride, const void *charges, int dtype, int64_t n, double *edges,
+                      const double *lower, const double *cell_size, const int64_t *num_nodes, const int64_t *code,
+                      int mode, int scale_exp, mpf_context *ctx, mpf_stream stream);
+
+/* ---- halos, gradient, divergence -------------------------------------------------- */
+
+/* Halo fill (Field.fill_halos -> mpiarray update_halos, field.py:601-613) and fold
+ * (Field.exchange_halos -> accumulate_halos, field.py:615-630) of ONE block holding the
+ * whole periodic axis: storage_shape (ndim entries) includes `halo` nodes on every side,
+ * data is storage_shape + (ncomp,) float64.  Axis by axis over the full storage, so the
+ * corners reach the diagonal in two steps.  Fill: periodic halos take the opposite
+ * interior layers, bounded ones are left unchanged.  Fold: periodic halos are added into
+ * the opposite interior layers; every halo layer is then cleared. */
+int mpf_halo_fill(int ndim, const int64_t *storage_shape, int64_t ncomp, int64_t halo, const int64_t *periodic,
+                  double *data, mpf_stream stream);
+int mpf_halo_fold(int ndim, const int64_t *storage_shape, int64_t ncomp, int64_t halo, const int64_t *periodic,
+                  double *data, mpf_stream stream);
+
+/* EdgeWeighting.divergence_block (edges.py:240-259): out[num_nodes] = G^T edges. */
+int mpf_divergence(int ndim, const int64_t *shape, const int64_t *periodic, const double *edges, double *out,
+                   mpf_stream stream);
+/* EdgeWeighting.gradient_block (edges.py:218-237): out[num_nodes + (ndim,)] = G nodes. */
+int mpf_gradient(int ndim, const int64_t *shape, const int64_t *periodic, const double *nodes, double *out,
+                 mpf_stream stream);
+
+/* ---- the binned (tiled) path ------------------------------------------------------- */
+
+/* The number of stencil cells of a grid (product of num_nodes); the offsets array of
+ * mpf_bin_markers holds cells + 2 entries. */
+int mpf_bin_sizes(int ndim, const int64_t *num_nodes, int64_t *cells);
+
+/* On-device counting sort of markers by stencil cell (the cell whose nodes a marker's
+ * stencil covers, found with the reference's float64 arithmetic), row-major cell order;
+ * markers whose values are all zero go to a trailing bin.  values may be NULL with
+ * ncomp 0 (a gather's markers: all live).  Writes, into caller-owned device memory:
+ * sorted_positions (ndim, n) and sorted_values (ncomp, n) with contiguous rows, order[n]
+ * (int64: sorted index -> input index), start[cells + 2] (uint32 offsets; start[cells] is
+ * the first dead marker) and scalars[3] = {2^k, 2^-k, markers per register pass}, the
+ * fixed-point scale with n * max|value| * 2^k < 2^61 and max|value| * 2^k < 2^49
+ * (scale_exp != MPF_SCALE_AUTO fixes k).  No counterpart in the reference: its banded CPU
+ * buffers assume markers sorted by cell (scatter_linear_3d_pyccel.py:11-16) but leave the
+ * sort to the caller. */
+int mpf_bin_markers(int kind, int degree, int ndim, const void *positions, int64_t pos_stride, const void *values,
+                    int64_t val_stride, int dtype, int64_t n, int64_t ncomp, const double *lower,
+                    const double *cell_size, const int64_t *num_nodes, const int64_t *code, void *sorted_positions,
+                    void *sorted_values, int64_t *order, uint32_t *start, double *scalars, int scale_exp,
+                    mpf_context *ctx, mpf_stream stream);
+
+/* Deposit binned markers (scatter_quadratic_3d etc., the tiled form): one workgroup per
+ * tile of cells accumulates int64 fixed point in LDS and adds the tile into field once;
+ * bit-reproducible.  3D, codes 0/1, (linear | quadratic) x (1 | 3 components) and cubic x 1
+ * component; MPF_ERR_UNSUPPORTED otherwise (use mpf_scatter). */
+int mpf_scatter_binned(int kind, int degree, int ndim, const void *sorted_positions, const void *sorted_values,
+                       int dtype, int64_t n, int64_t ncomp, const uint32_t *start, const double *scalars,
+                       double *field, const double *lower, const double *cell_size, const int64_t *num_nodes,
+                       const int64_t *code, mpf_context *ctx, mpf_stream stream);
+
+/* Gather at binned markers (the tiled form of gather_quadratic_3d etc.); result (ncomp, n)
+ * in the sorted order, contiguous rows.  Same cases as mpf_scatter_binned. */
+int mpf_gather_binned(int kind, int degree, int ndim, const void *sorted_positions, int dtype, int64_t n,
+                      const uint32_t *start, const double *field, int64_t ncomp, void *result, const double *lower,
+                      const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
+                      mpf_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPF_HIP_H */

@@ -1,0 +1,110 @@
+"""ctypes binding of libmpfhip.so (include/mpf_hip.h).
+
+The library is the only compute path: importing a kernel wrapper without the built
+library, or calling one without a ROCm GPU, raises -- there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("MPF_HIP_LIBRARY", _HERE / "libmpfhip.so"))
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_INT = ctypes.c_int
+_DBL = ctypes.c_double
+
+MPF_OK = 0
+F32, F64 = 0, 1
+MODE_ATOMIC, MODE_FIXED = 0, 1
+SCALE_AUTO = -100000
+KIND_LINEAR, KIND_QUADRATIC, KIND_SPLINE = 0, 1, 2
+
+
+class MpfError(RuntimeError):
+    """A failed call into libmpfhip.so, with the library's message."""
+
+
+def _geom(ndim: int) -> list:
+    return [_DBL] * (2 * ndim) + [_I64] * (2 * ndim)
+
+
+_SCATTER_IN = [_P, _I64, _P, _I64, _INT, _I64, _I64, _P]
+_GATHER_IN = [_P, _I64, _INT, _I64, _P, _I64, _P, _I64]
+_PATHS_IN = [_P, _I64, _P, _I64, _P, _INT, _I64, _P]
+_TAIL = [_P, _P]
+
+#: every exported entry point of include/mpf_hip.h with its argument types
+SIGNATURES: dict[str, list] = {
+    "mpf_abi_version": [],
+    "mpf_last_error": [],
+    "mpf_context_create": [_INT],
+    "mpf_context_destroy": [_P],
+    "mpf_profile_enable": [_INT],
+    "mpf_profile_reset": [],
+    "mpf_profile_query": [ctypes.c_char_p, _P, _P],
+    "mpf_scatter": [_INT, _INT, _INT, _P, _I64, _P, _I64, _INT, _I64, _I64, _P, _P, _P, _P, _P, _INT, _INT, _P, _P],
+    "mpf_gather": [_INT, _INT, _INT, _P, _I64, _INT, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P],
+    "mpf_deposit_paths": [_INT, _INT, _INT, _P, _I64, _P, _I64, _P, _INT, _I64, _P, _P, _P, _P, _P, _INT, _INT, _P,
+                          _P],
+    "mpf_halo_fill": [_INT, _P, _I64, _I64, _P, _P, _P],
+    "mpf_halo_fold": [_INT, _P, _I64, _I64, _P, _P, _P],
+    "mpf_divergence": [_INT, _P, _P, _P, _P, _P],
+    "mpf_gradient": [_INT, _P, _P, _P, _P, _P],
+    "mpf_bin_sizes": [_INT, _P, _P],
+    "mpf_bin_markers": [_INT, _INT, _INT, _P, _I64, _P, _I64, _INT, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                        _INT, _P, _P],
+    "mpf_scatter_binned": [_INT, _INT, _INT, _P, _P, _INT, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "mpf_gather_binned": [_INT, _INT, _INT, _P, _INT, _I64, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P],
+}
+for _d in (1, 2, 3):
+    for _kind in ("linear", "quadratic"):
+        SIGNATURES[f"mpf_scatter_{_kind}_{_d}d"] = _SCATTER_IN + _geom(_d) + [_INT] + _TAIL
+        SIGNATURES[f"mpf_gather_{_kind}_{_d}d"] = _GATHER_IN + _geom(_d) + _TAIL
+    SIGNATURES[f"mpf_scatter_spline_{_d}d"] = _SCATTER_IN + _geom(_d) + [_INT, _INT] + _TAIL
+    SIGNATURES[f"mpf_gather_spline_{_d}d"] = _GATHER_IN + _geom(_d) + [_INT] + _TAIL
+    SIGNATURES[f"mpf_deposit_paths_{_d}d"] = _PATHS_IN + _geom(_d) + [_INT] + _TAIL
+    SIGNATURES[f"mpf_deposit_paths_spline_{_d}d"] = _PATHS_IN + _geom(_d) + [_INT, _INT] + _TAIL
+
+_RESTYPES = {"mpf_last_error": ctypes.c_char_p, "mpf_context_create": _P, "mpf_context_destroy": None}
+
+_LIB: ctypes.CDLL | None = None
+
+
+def load(path: Path | str | None = None) -> ctypes.CDLL:
+    """Load libmpfhip.so and declare its signatures; raises if it is not built."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    where = Path(path) if path is not None else LIB_PATH
+    if not where.exists():
+        raise MpfError(
+            f"{where} is not built: run `python -m mpifield_amd.csrc.build` (or __graft_entry__.build()); "
+            "mpifield_amd has no CPU fallback"
+        )
+    lib = ctypes.CDLL(str(where))
+    for name, argtypes in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = _RESTYPES.get(name, _INT)
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    """Raise MpfError with the library's message for a non-zero status."""
+    if rc != MPF_OK:
+        message = load().mpf_last_error().decode(errors="replace")
+        kind = {1: ValueError, 3: NotImplementedError}.get(rc)
+        if kind is not None:
+            raise kind(message)
+        raise MpfError(f"libmpfhip.so: {message} (status {rc})")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args))

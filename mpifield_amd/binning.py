@@ -1,0 +1,143 @@
+"""Markers binned by stencil cell: the on-device counting sort and the tiled deposit and
+gather that read the bins (mpf_bin_markers, mpf_scatter_binned, mpf_gather_binned).
+
+A particle code that keeps its markers sorted by cell between steps (what the reference's
+banded CPU buffers assume, scatter_linear_3d_pyccel.py:11-16) bins once and deposits and
+gathers many times::
+
+    bins = bin_markers(positions, values, QUADRATIC, grid)
+    scatter_binned(bins, field)              # adds into field, bit-reproducible
+    gather_binned(bins, field, result)       # result (ncomp, N) in bins order
+
+``bins.order[i]`` is the input index of binned marker ``i``. `Weighting.scatter_block` /
+`gather_block` take this path by themselves for large batches (binning inside the call).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from mpifield_amd import _lib
+from mpifield_amd import kernels as K
+
+#: (kind, degree) of the built-in node weightings, by name and degree
+_KINDS = {("linear", 1): (0, 1), ("quadratic", 2): (1, 2), ("cubic", 3): (2, 3)}
+
+
+def kind_of(weighting) -> tuple[int, int]:
+    """The kernel kind and degree of a built-in weighting (ValueError for others)."""
+    key = (weighting.name, weighting.degree)
+    if key in _KINDS:
+        return _KINDS[key]
+    if weighting.degree is not None and weighting.clamps:
+        return (2, int(weighting.degree))
+    raise ValueError(f"the {weighting.name} weighting has no binned kernels")
+
+
+@dataclass
+class MarkerBins:
+    """Markers sorted by stencil cell on the GPU, and what the tiled kernels need."""
+
+    kind: int
+    degree: int
+    geometry: tuple
+    positions: torch.Tensor  # (ndim, N) sorted, contiguous rows
+    values: torch.Tensor | None  # (ncomp, N) sorted, or None for gather-only bins
+    order: torch.Tensor  # (N,) int64: sorted index -> input index
+    start: torch.Tensor  # (cells + 2,) int32 holding uint32 offsets
+    scalars: torch.Tensor  # (3,) float64: [2^k, 2^-k, markers per register pass]
+
+    @property
+    def n(self) -> int:
+        return int(self.positions.shape[1])
+
+    @property
+    def ncomp(self) -> int:
+        return 0 if self.values is None else int(self.values.shape[0])
+
+    def unsort(self, sorted_rows: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Rows (k, N) in bins order back to the input order."""
+        if out is None:
+            out = torch.empty_like(sorted_rows)
+        out.index_copy_(1, self.order, sorted_rows)
+        return out
+
+
+def _geometry_arrays(geometry):
+    ndim, lower, cell, num, code, _ = K._split(geometry)
+    return ndim, lower, cell, num, code
+
+
+def bin_markers(positions, values, kind_degree: tuple[int, int], geometry, scale_exp: int = _lib.SCALE_AUTO
+                ) -> MarkerBins:
+    """Sort markers by the stencil cell of the (kind, degree) weighting on ``geometry``."""
+    kind, degree = kind_degree
+    ndim, lower, cell, num, code = _geometry_arrays(geometry)
+    pos, ps = K.marker_rows(positions, ndim)
+    n = pos.shape[1]
+    dev = pos.device
+    if values is not None:
+        val, vs = K.marker_rows(values, None, pos.dtype)
+        if val.shape[1] != n:
+            raise ValueError(f"{n} positions for {val.shape[1]} values")
+        ncomp = val.shape[0]
+        sval = torch.empty((ncomp, n), dtype=pos.dtype, device=dev)
+        val_ptr, sval_ptr = val.data_ptr(), sval.data_ptr()
+    else:
+        vs, ncomp, sval, val_ptr, sval_ptr = 0, 0, None, None, None
+    cells = ctypes_int64()
+    _lib.call("mpf_bin_sizes", ndim, num.ctypes.data, cells.ptr)
+    spos = torch.empty((ndim, n), dtype=pos.dtype, device=dev)
+    order = torch.empty(n, dtype=torch.int64, device=dev)
+    start = torch.empty(int(cells.value) + 2, dtype=torch.int32, device=dev)
+    scalars = torch.empty(3, dtype=torch.float64, device=dev)
+    with torch.cuda.device(dev):
+        _lib.call("mpf_bin_markers", kind, degree, ndim, pos.data_ptr(), ps, val_ptr, vs, K._dtype_code(pos), n,
+                  ncomp, lower.ctypes.data, cell.ctypes.data, num.ctypes.data, code.ctypes.data, spos.data_ptr(),
+                  sval_ptr, order.data_ptr(), start.data_ptr(), scalars.data_ptr(), int(scale_exp), None,
+                  K.stream_handle(dev))
+    return MarkerBins(kind, degree, tuple(geometry), spos, sval, order, start, scalars)
+
+
+class ctypes_int64:
+    """A host int64 slot for an out-parameter."""
+
+    def __init__(self):
+        self.array = np.zeros(1, dtype=np.int64)
+        self.ptr = self.array.ctypes.data
+
+    @property
+    def value(self) -> int:
+        return int(self.array[0])
+
+
+def scatter_binned(bins: MarkerBins, field: torch.Tensor) -> None:
+    """Add the binned markers' values into ``field`` (num_nodes + (ncomp,)) with the tiled kernels."""
+    if bins.values is None:
+        raise ValueError("these bins hold no values (binned for a gather)")
+    field = K._field(field)
+    ndim, lower, cell, num, code = _geometry_arrays(bins.geometry)
+    if tuple(field.shape) != (*num, bins.ncomp):
+        raise ValueError(f"a field on this grid is {(*num, bins.ncomp)}, got {tuple(field.shape)}")
+    with torch.cuda.device(field.device):
+        _lib.call("mpf_scatter_binned", bins.kind, bins.degree, ndim, bins.positions.data_ptr(),
+                  bins.values.data_ptr(), K._dtype_code(bins.positions), bins.n, bins.ncomp, bins.start.data_ptr(),
+                  bins.scalars.data_ptr(), field.data_ptr(), lower.ctypes.data, cell.ctypes.data, num.ctypes.data,
+                  code.ctypes.data, None, K.stream_handle(field.device))
+
+
+def gather_binned(bins: MarkerBins, field: torch.Tensor, result: torch.Tensor) -> None:
+    """Interpolate ``field`` at the binned markers into ``result`` (ncomp, N) in bins order."""
+    field = K._field(field)
+    ndim, lower, cell, num, code = _geometry_arrays(bins.geometry)
+    ncomp = field.shape[-1]
+    if tuple(result.shape) != (ncomp, bins.n) or not result.is_contiguous() or result.dtype != bins.positions.dtype:
+        raise ValueError(f"result must be a contiguous ({ncomp}, {bins.n}) {bins.positions.dtype} tensor")
+    with torch.cuda.device(field.device):
+        _lib.call("mpf_gather_binned", bins.kind, bins.degree, ndim, bins.positions.data_ptr(),
+                  K._dtype_code(bins.positions), bins.n, bins.start.data_ptr(), field.data_ptr(), ncomp,
+                  result.data_ptr(), lower.ctypes.data, cell.ctypes.data, num.ctypes.data, code.ctypes.data, None,
+                  K.stream_handle(field.device))

@@ -1,0 +1,89 @@
+// mpf_internal.h -- host-side plumbing shared by the library's translation units:
+// the context (scratch owned by the library), error reporting and launch helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/mpf_hip.h"
+#include "mpf_device.h"
+
+namespace mpf {
+
+// Scratch that outlives a call: the fixed-point accumulator (kept zero between calls),
+// the scale slot, the binning buffers of the tiled path.  Grown on demand, never shrunk.
+struct Buffer {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace mpf
+
+struct mpf_context {
+  int device = 0;
+  mpf::Buffer accum;     // int64 fixed-point accumulator, zero between calls
+  mpf::Buffer scalars;   // [0] max |v| bits, [1] scale, [2] 1/scale, [3..] spare
+  mpf::Buffer sort_keys;
+  mpf::Buffer sort_tmp;
+  mpf::Buffer offsets;
+  mpf::Buffer perm;
+};
+
+namespace mpf {
+
+void set_error(const char *fmt, ...);
+int fail_hip(hipError_t e, const char *what);
+
+// Clear device memory with a kernel (graph-capturable, unlike hipMemsetAsync).
+int zero_async(void *p, size_t bytes, hipStream_t stream);
+
+// Grow a buffer to at least `bytes`; `zero` clears new memory.
+int ensure(Buffer &b, size_t bytes, bool zero, hipStream_t stream);
+
+mpf_context *default_context();
+
+inline int blocks_for(int64_t n, int threads, int64_t cap = 1 << 20) {
+  int64_t b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (int)b;
+}
+
+// Build the kernel's Grid from the reference's geometry arrays; extent adds degree - 1 on
+// clamped axes (edges.py:145-167, scatter_spline_3d_pyccel.py:129).
+int make_grid(Grid &g, int ndim, int kind, int degree, const double *lower, const double *cell,
+              const int64_t *num, const int64_t *code);
+
+// The fixed-point scale: S = 2^k with k chosen so that n * max|value| * S < 2^61, unless
+// the caller fixes k.  Computed on the device into ctx->scalars (no host sync).
+int prepare_scale(mpf_context *ctx, const void *values, int64_t vstride, int dtype, int64_t rows, int64_t n,
+                  double bound_factor, int scale_exp, hipStream_t stream);
+double *scale_slot(mpf_context *ctx);  // device pointer: [0] scale, [1] 1/scale
+
+// field[i] += accum[i] / scale; accum[i] = 0
+int finalize_fixed(mpf_context *ctx, double *field, int64_t count, hipStream_t stream);
+
+// Kernel timing: when enabled (mpf_profile_enable), the library records a pair of HIP
+// events on the launch stream around the launches it names, for mpf_profile_query.
+struct ProfileScope {
+  const char *name;
+  hipStream_t stream;
+  int slot;
+  ProfileScope(const char *name, hipStream_t stream);
+  ~ProfileScope();
+};
+
+}  // namespace mpf
+
+#define MPF_CHECK(call)                                           \
+  do {                                                            \
+    hipError_t _e = (call);                                       \
+    if (_e != hipSuccess) return mpf::fail_hip(_e, #call);        \
+  } while (0)
+
+#define MPF_LAUNCH_CHECK()                                        \
+  do {                                                            \
+    hipError_t _e = hipGetLastError();                            \
+    if (_e != hipSuccess) return mpf::fail_hip(_e, "kernel launch"); \
+  } while (0)

@@ -1,0 +1,113 @@
+"""CPU checks of the drop-in boundary: the library builds, loads and exports every entry
+point include/mpf_hip.h declares, with the argument lists the ctypes binding uses; and the
+host-side plugin API keeps the reference's names, checks and messages."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "mpf_hip.h"
+
+
+def declared() -> set[str]:
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(mpf_\w+)\s*\(", text, flags=re.M))
+
+
+def test_library_exports_every_declared_entry_point():
+    from mpifield_amd import _lib
+
+    names = declared()
+    assert len(names) >= 35
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the binding declares exactly the header's entry points
+    assert set(_lib.SIGNATURES) == names
+
+
+def test_abi_version_and_error_slot_work_without_a_gpu():
+    from mpifield_amd import _lib
+
+    lib = _lib.load()
+    assert lib.mpf_abi_version() == 1
+    assert lib.mpf_last_error() == b""
+
+
+def test_argument_errors_need_no_gpu():
+    """Argument checks run on the host before any device work."""
+    from mpifield_amd import _lib
+
+    lib = _lib.load()
+    lower = np.zeros(1)
+    cell = np.ones(1)
+    num = np.array([5], dtype=np.int64)
+    code = np.array([0], dtype=np.int64)
+    rc = lib.mpf_scatter(2, 9, 1, None, 0, None, 0, 1, 0, 1, None, lower.ctypes.data, cell.ctypes.data,
+                         num.ctypes.data, code.ctypes.data, 0, _lib.SCALE_AUTO, None, None)
+    assert rc == 1 and b"spline degree" in lib.mpf_last_error()
+    code2 = np.array([2], dtype=np.int64)
+    rc = lib.mpf_gather(1, 0, 1, None, 0, 1, 0, None, 1, None, 0, lower.ctypes.data, cell.ctypes.data,
+                        num.ctypes.data, code2.ctypes.data, None, None)
+    assert rc == 1 and b"clamped" in lib.mpf_last_error()
+    rc = lib.mpf_deposit_paths(1, 5, 1, None, 0, None, 0, None, 1, 0, None, lower.ctypes.data, cell.ctypes.data,
+                               num.ctypes.data, code.ctypes.data, 0, _lib.SCALE_AUTO, None, None)
+    assert rc == 1 and b"spline edge degrees" in lib.mpf_last_error()
+
+
+def test_plugin_api_names_and_messages():
+    import mpifield_amd as m
+
+    assert repr(m.QUADRATIC) == "Weighting('quadratic', ndim=(1, 2, 3), halo=2, degree=2)"
+    assert repr(m.CUBIC) == "Weighting('cubic', ndim=(1, 2, 3), halo=2, degree=3)"
+    assert repr(m.QUADRATIC_EDGES) == "EdgeWeighting('quadratic', ndim=(1, 2, 3), halo=2, degree=2)"
+    assert (m.WHITNEY.degree, m.QUADRATIC_EDGES.degree, m.CUBIC_EDGES.degree) == (1, 2, 3)
+    with pytest.raises(ValueError, match="1, 2 or 3 dimensions"):
+        m.Weighting("bad", scatter={4: print}, gather={})
+    with pytest.raises(ValueError, match="has no 2D scatter kernel"):
+        m.Weighting("mine", scatter={1: print}, gather={}).scatter_kernel(2)
+    with pytest.raises(TypeError):
+        m.LINEAR.scatter[2] = print  # read-only mapping
+    with pytest.raises(ValueError, match="does not take clamped axes"):
+        m.QUADRATIC.scatter_block(np.zeros((1, 1)), np.ones((1, 1)), np.zeros((4, 1)),
+                                  m.BlockGrid.from_geometry([0.0], [1.0], [4], clamped=[True]))
+    assert m.splines(2).edges is m.QUADRATIC_EDGES and m.splines(3, clamped=True).nodes is m.CUBIC
+    with pytest.raises(ValueError, match="no built-in splines of degree 5"):
+        m.splines(5)
+
+
+def test_block_grid_from_geometry():
+    """test_weighting.py:190-197."""
+    from mpifield_amd import BlockGrid
+
+    grid = BlockGrid.from_geometry((0.0, 0.0), (0.5, 0.5), (3, 3), (True, False))
+    assert grid.geometry == (0.0, 0.0, 0.5, 0.5, 3, 3, 1, 0)
+    with pytest.raises(ValueError, match="differ in length"):
+        BlockGrid.from_geometry((0.0,), (0.5, 0.5), (3, 3))
+    with pytest.raises(ValueError, match="either periodic or clamped"):
+        BlockGrid.from_geometry((0.0,), (0.5,), (3,), (True,), (True,))
+
+
+def test_product_path_fails_loudly_without_a_gpu(monkeypatch):
+    import torch
+
+    from mpifield_amd import _lib
+    from mpifield_amd import kernels as K
+
+    monkeypatch.setattr(torch.cuda, "is_available", lambda: False)
+    with pytest.raises(_lib.MpfError, match="needs a ROCm GPU"):
+        K.device()
+
+
+def test_missing_library_is_an_error(tmp_path):
+    from mpifield_amd import _lib
+
+    with pytest.raises(_lib.MpfError, match="no CPU fallback"):
+        _lib.load(tmp_path / "absent.so")

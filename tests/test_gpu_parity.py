@@ -1,0 +1,235 @@
+"""GPU parity of libmpfhip.so against the CPU oracle, through the C ABI (mpifield_amd.kernels).
+
+Bars: float64 markers in MODE_ATOMIC and MODE_FIXED agree with the oracle within 1e-12
+relative to the field's largest value (scatter) or the gathered values (gather); float32
+markers run the same float64 cell selection, so the oracle fed the same float32 values
+agrees as tightly, and the float32 gather result within float32 rounding (1e-6).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+if torch.cuda.is_available():
+    from mpifield_amd import _lib
+    from mpifield_amd import kernels as K
+
+KINDS = [("linear", 0, 1), ("quadratic", 1, 2), ("spline", 2, 0), ("spline", 2, 1), ("spline", 2, 2),
+         ("spline", 2, 3)]
+CASES = [(1, (0,)), (1, (1,)), (2, (0, 1)), (2, (1, 1)), (3, (0, 0, 0)), (3, (1, 0, 1)), (3, (1, 1, 1))]
+MODES = ["atomic", "fixed"]
+
+
+def geom(lower, cell, num, code):
+    return (*map(float, lower), *map(float, cell), *map(int, num), *map(int, code))
+
+
+def case(ndim, periodic, seed=0, n=2000, ncomp=2, clamped=False, degree=1):
+    rng = np.random.default_rng(seed)
+    shape = (7, 6, 5)[:ndim]
+    lower = np.array([0.0, -1.0, 2.0])[:ndim]
+    step = np.array([0.25, 0.5, 0.1])[:ndim]
+    upper = lower + step * np.array(shape)
+    pts = rng.uniform(lower - 0.3, upper + 0.3, (n, ndim)).T.copy()
+    vals = rng.normal(size=(ncomp, n))
+    code = [2] * ndim if clamped else list(periodic)
+    extent = tuple(s + (degree - 1 if clamped else 0) for s in shape)
+    field = rng.normal(size=(*extent, ncomp))
+    return geom(lower, step, shape, code), pts, vals, field
+
+
+def cuda(a):
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+
+
+def rel_err(got, want):
+    scale = max(np.abs(want).max(), 1e-300)
+    return np.abs(got - want).max() / scale
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("ndim,periodic", CASES)
+@pytest.mark.parametrize("name,kind,degree", KINDS)
+def test_scatter_matches_oracle(name, kind, degree, ndim, periodic, mode):
+    print(f"scatter {name} p={degree} {ndim}D periodic={periodic} mode={mode}", flush=True)
+    g, pts, vals, field = case(ndim, periodic, seed=kind * 10 + degree)
+    vals[:, ::9] = 0.0
+    pts[:, ::9] = np.nan  # dead markers may sit anywhere
+    want = field.copy()
+    orc.scatter(name, pts, vals, want, *g, degree=degree)
+    got = cuda(field)
+    with K.deposit_mode(mode):
+        K.scatter(kind, degree, cuda(pts), cuda(vals), got, g)
+    assert rel_err(got.cpu().numpy(), want) < 1e-12
+
+
+@pytest.mark.parametrize("ndim,periodic", CASES)
+@pytest.mark.parametrize("name,kind,degree", KINDS)
+def test_gather_matches_oracle(name, kind, degree, ndim, periodic):
+    print(f"gather {name} p={degree} {ndim}D periodic={periodic}", flush=True)
+    g, pts, _, field = case(ndim, periodic, seed=kind * 10 + degree + 1)
+    want = np.zeros((field.shape[-1], pts.shape[1]))
+    orc.gather(name, pts, field, want, *g, degree=degree)
+    got = torch.zeros((field.shape[-1], pts.shape[1]), dtype=torch.float64, device="cuda")
+    K.gather(kind, degree, cuda(pts), cuda(field), got, g)
+    assert rel_err(got.cpu().numpy(), want) < 1e-12
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+@pytest.mark.parametrize("degree", [1, 2, 3])
+def test_clamped_axes_match_oracle(degree, ndim, mode):
+    print(f"clamped p={degree} {ndim}D mode={mode}", flush=True)
+    g, pts, vals, field = case(ndim, (0,) * ndim, seed=degree, clamped=True, degree=degree)
+    want = field.copy()
+    orc.scatter("spline", pts, vals, want, *g, degree=degree)
+    got = cuda(field)
+    with K.deposit_mode(mode):
+        K.scatter(2, degree, cuda(pts), cuda(vals), got, g)
+    assert rel_err(got.cpu().numpy(), want) < 1e-12
+    want_g = np.zeros((field.shape[-1], pts.shape[1]))
+    orc.gather("spline", pts, field, want_g, *g, degree=degree)
+    got_g = torch.zeros_like(cuda(want_g))
+    K.gather(2, degree, cuda(pts), cuda(field), got_g, g)
+    assert rel_err(got_g.cpu().numpy(), want_g) < 1e-12
+
+
+def test_strided_rows_and_float32_markers():
+    """Rows with gaps (storage[:, :n], weighting.py:33-34) and float32 markers."""
+    g, pts, vals, field = case(3, (1, 0, 1), seed=3, n=5000, ncomp=3)
+    storage_p = torch.zeros((3, 8000), dtype=torch.float32, device="cuda")
+    storage_v = torch.zeros((3, 8000), dtype=torch.float32, device="cuda")
+    storage_p[:, :5000] = cuda(pts.astype(np.float32))
+    storage_v[:, :5000] = cuda(vals.astype(np.float32))
+    p32 = pts.astype(np.float32).astype(np.float64)
+    v32 = vals.astype(np.float32).astype(np.float64)
+    want = field.copy()
+    orc.scatter("quadratic", p32, v32, want, *g)
+    for mode in MODES:
+        got = cuda(field)
+        with K.deposit_mode(mode):
+            K.scatter(1, 0, storage_p[:, :5000], storage_v[:, :5000], got, g)
+        assert rel_err(got.cpu().numpy(), want) < 1e-12, mode
+    want_g = np.zeros((3, 5000))
+    orc.gather("quadratic", p32, field, want_g, *g)
+    got_g = torch.zeros((3, 5000), dtype=torch.float32, device="cuda")
+    K.gather(1, 0, storage_p[:, :5000], cuda(field), got_g, g)
+    np.testing.assert_allclose(got_g.cpu().numpy(), want_g, rtol=1e-6, atol=1e-6 * np.abs(want_g).max())
+
+
+def test_fixed_point_is_bit_reproducible_under_any_marker_order():
+    g, pts, vals, field = case(3, (1, 1, 1), seed=8, n=200000, ncomp=3)
+    first = torch.zeros(field.shape, dtype=torch.float64, device="cuda")
+    with K.deposit_mode("fixed"):
+        K.scatter(1, 0, cuda(pts), cuda(vals), first, g)
+        for seed in range(3):
+            order = np.random.default_rng(seed).permutation(pts.shape[1])
+            again = torch.zeros_like(first)
+            K.scatter(1, 0, cuda(pts[:, order]), cuda(vals[:, order]), again, g)
+            assert torch.equal(again, first)
+
+
+# ---- path deposits ---------------------------------------------------------------------
+
+
+def paths(ndim, periodic, seed, n=400, reach=3.0):
+    rng = np.random.default_rng(seed)
+    shape = (9, 8, 7)[:ndim]
+    lower = np.array([0.0, -1.0, 2.0])[:ndim]
+    step = np.array([0.25, 0.5, 0.1])[:ndim]
+    upper = lower + step * (np.array(shape) - 1)
+    start = rng.uniform(lower - 3 * step, upper + 3 * step, (n, ndim)).T.copy()
+    end = start + rng.normal(scale=reach * step[:, None], size=start.shape)
+    q = rng.normal(size=n)
+    q[::11] = 0.0
+    end[:, ::13] = np.nan
+    return geom(lower, step, shape, [periodic] * ndim), shape, start, end, q
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("periodic", [0, 1])
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+@pytest.mark.parametrize("degree", [0, 1, 2, 3])
+def test_path_deposits_match_oracle(degree, ndim, periodic, mode):
+    """degree 0 here means WHITNEY (deposit_paths_*d); 1..3 the spline edges."""
+    print(f"paths p={degree} {ndim}D periodic={periodic} mode={mode}", flush=True)
+    g, shape, start, end, q = paths(ndim, periodic, seed=degree)
+    want = np.zeros((*shape, ndim))
+    if degree == 0:
+        orc.deposit_whitney(start, end, q, want, *g)
+    else:
+        orc.deposit_spline(start, end, q, want, *g, degree=degree)
+    got = torch.zeros((*shape, ndim), dtype=torch.float64, device="cuda")
+    with K.deposit_mode(mode):
+        K.deposit_paths(0 if degree == 0 else 1, max(degree, 1), cuda(start), cuda(end), cuda(q), got, g)
+    assert rel_err(got.cpu().numpy(), want) < 1e-12
+
+
+@pytest.mark.parametrize("periodic", [0, 1])
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("degree", [2, 3])
+def test_path_deposit_continuity_on_the_device(degree, ndim, periodic):
+    """G^T j = rho(end) - rho(start) with the node weighting of the same degree, all on the GPU."""
+    g, shape, start, end, q = paths(ndim, periodic, seed=10 + degree, n=3000)
+    ok = np.isfinite(end).all(axis=0)
+    start, end, q = start[:, ok], end[:, ok], q[ok]
+    with K.deposit_mode("fixed"):
+        j = torch.zeros((*shape, ndim), dtype=torch.float64, device="cuda")
+        K.deposit_paths(1, degree, cuda(start), cuda(end), cuda(q), j, g)
+        rho0 = torch.zeros((*shape, 1), dtype=torch.float64, device="cuda")
+        rho1 = torch.zeros_like(rho0)
+        K.scatter(2, degree, cuda(start), cuda(q[None]), rho0, g)
+        K.scatter(2, degree, cuda(end), cuda(q[None]), rho1, g)
+    div = K.divergence(j, [periodic] * ndim)
+    np.testing.assert_allclose(div.cpu().numpy(), (rho1 - rho0)[..., 0].cpu().numpy(), atol=1e-11)
+
+
+# ---- halos, gradient, divergence --------------------------------------------------------
+
+
+@pytest.mark.parametrize("halo", [1, 2])
+@pytest.mark.parametrize("periodic", [(1, 0, 1), (1, 1, 1), (0, 1, 0)])
+def test_halo_fold_and_fill_match_oracle(periodic, halo):
+    rng = np.random.default_rng(halo)
+    data = rng.normal(size=(9, 8, 7, 3))
+    for op, fn in (("fold", orc.halo_fold), ("fill", orc.halo_fill)):
+        want = data.copy()
+        fn(want, halo, periodic)
+        got = cuda(data)
+        K.halo(op, got, halo, periodic)
+        np.testing.assert_array_equal(got.cpu().numpy(), want)
+
+
+def test_gradient_and_divergence_match_oracle():
+    rng = np.random.default_rng(0)
+    for periodic in ([0, 1, 1], [1, 0, 0], [1, 1]):
+        shape = (6, 5, 4)[: len(periodic)]
+        phi = rng.normal(size=shape)
+        j = rng.normal(size=(*shape, len(periodic)))
+        np.testing.assert_allclose(K.gradient(cuda(phi), periodic).cpu().numpy(), orc.gradient(phi, periodic),
+                                   atol=1e-14)
+        np.testing.assert_allclose(K.divergence(cuda(j), periodic).cpu().numpy(), orc.divergence(j, periodic),
+                                   atol=1e-14)
+
+
+# ---- errors come back through the ABI ---------------------------------------------------
+
+
+def test_errors_are_raised_with_the_library_message():
+    f = torch.zeros((5, 1), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="spline degree"):
+        K.scatter(2, 7, cuda(np.zeros((1, 3))), cuda(np.ones((1, 3))), f, (0.0, 1.0, 5, 0))
+    with pytest.raises(ValueError, match="clamped"):
+        K.scatter(1, 0, cuda(np.zeros((1, 3))), cuda(np.ones((1, 3))), f, (0.0, 1.0, 5, 2))
+    with pytest.raises(ValueError, match="a field on this grid"):
+        K.scatter(1, 0, cuda(np.zeros((1, 3))), cuda(np.ones((1, 3))), f, (0.0, 1.0, 6, 0))
+    with pytest.raises(ValueError, match="spline edge degrees"):
+        K.deposit_paths(1, 4, cuda(np.zeros((1, 1))), cuda(np.ones((1, 1))), cuda(np.ones(1)),
+                        torch.zeros((4, 1), dtype=torch.float64, device="cuda"), (0.0, 1.0, 4, 0))
+    assert _lib.load().mpf_abi_version() == 1
