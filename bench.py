@@ -49,6 +49,8 @@ def parse() -> argparse.Namespace:
     p.add_argument("--cpu-sample", type=int, default=1 << 21, help="markers in the CPU oracle's timed sample")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--only-headline", action="store_true", help="time the headline step only (for rocprofv3)")
+    p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                   help="only copy MIB MiB device-to-device (the counter calibration for rocprofv3 --pmc)")
     return p.parse_args()
 
 
@@ -131,6 +133,15 @@ def main() -> None:
               "DESIGN.md describes how it would shard)", file=sys.stderr)
         sys.exit(2)
     import torch
+
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": src.numel() * 8, "copies": 3}), flush=True)
+        return
 
     from mpifield_amd import _lib, binning
     from mpifield_amd import kernels as K

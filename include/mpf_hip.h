@@ -99,6 +99,17 @@ int mpf_gather(int kind, int degree, int ndim, const void *positions, int64_t po
                const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
                mpf_stream stream);
 
+/* The derivative-spline gather for differentiation: out[(d * ncomp + c) * out_stride + m]
+ * = d/dx_d of the interpolant of component c at marker m (the weights' derivatives along
+ * axis d times the others', over the cell size; zero where a bounded axis clamps the
+ * position).  No counterpart in the reference, which has no differentiation; the adjoint
+ * identity it tests (test_catalog_and_pyccel.py:160-182) is what the backward of the
+ * gather and scatter rest on.  Codes 0/1 only. */
+int mpf_gather_derivative(int kind, int degree, int ndim, const void *positions, int64_t pos_stride, int dtype,
+                          int64_t n, const double *field, int64_t ncomp, void *out, int64_t out_stride,
+                          const double *lower, const double *cell_size, const int64_t *num_nodes, const int64_t *code,
+                          mpf_context *ctx, mpf_stream stream);
+
 /* ---- the catalogue's node kernels, argument for argument ------------------------- */
 
 #define MPF_SCATTER_IN \
@@ -223,6 +234,46 @@ int mpf_gather_binned(int kind, int degree, int ndim, const void *sorted_positio
                       const uint32_t *start, const double *field, int64_t ncomp, void *result, const double *lower,
                       const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
                       mpf_stream stream);
+
+/* ---- many patches in one launch ---------------------------------------------------- */
+
+/* One refined-mesh patch: the arguments of one Weighting.scatter_block / gather_block call
+ * on BlockGrid.from_geometry(lower, cell_size, num_nodes, periodic) (_types.py:46-96;
+ * weighting.py:195-250), with its own weighting (kind, degree) and its slices of the
+ * batch's flat marker and field arrays.  Codes 0/1. */
+typedef struct mpf_patch {
+  double lower[3];
+  double cell_size[3];
+  int64_t num_nodes[3];
+  int64_t code[3];
+  int32_t kind;   /* MPF_KIND_* */
+  int32_t degree; /* 0..3 for MPF_KIND_SPLINE */
+  int64_t first_marker; /* markers [first_marker, first_marker + num_markers) of the batch */
+  int64_t num_markers;
+  int64_t field_offset; /* the patch field num_nodes + (ncomp,) starts at fields + field_offset */
+} mpf_patch;
+
+typedef struct mpf_patch_plan mpf_patch_plan;
+
+/* Upload the patch table and the work list (items of at most `chunk` markers of one
+ * patch).  Synchronous; NULL on error.  No counterpart in the reference, where the caller
+ * loops over the patches (edges.py:21). */
+mpf_patch_plan *mpf_patch_plan_create(int ndim, int64_t n_patches, const mpf_patch *patches, int64_t chunk,
+                                      int64_t ncomp);
+void mpf_patch_plan_destroy(mpf_patch_plan *plan);
+int64_t mpf_patch_plan_items(const mpf_patch_plan *plan);
+
+/* Deposit every patch's markers into its field in one persistent launch of `workgroups`
+ * workgroups of `block` (256 or 512) work-items: per item an LDS copy of the patch in
+ * int64 fixed point, added into an int64 accumulator, then one finalize pass into the
+ * float64 fields.  Bit-reproducible; no host synchronisation. */
+int mpf_patches_scatter(const mpf_patch_plan *plan, const void *positions, int64_t pos_stride, const void *values,
+                        int64_t val_stride, int dtype, int64_t ncomp, double *fields, int block, int workgroups,
+                        mpf_context *ctx, mpf_stream stream);
+/* Gather every patch's field at its markers (result (ncomp, N) rows of result_stride). */
+int mpf_patches_gather(const mpf_patch_plan *plan, const void *positions, int64_t pos_stride, int dtype,
+                       const double *fields, int64_t ncomp, void *result, int64_t result_stride, int workgroups,
+                       mpf_context *ctx, mpf_stream stream);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,7 @@ SIGNATURES: dict[str, list] = {
     "mpf_profile_query": [ctypes.c_char_p, _P, _P],
     "mpf_scatter": [_INT, _INT, _INT, _P, _I64, _P, _I64, _INT, _I64, _I64, _P, _P, _P, _P, _P, _INT, _INT, _P, _P],
     "mpf_gather": [_INT, _INT, _INT, _P, _I64, _INT, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P],
+    "mpf_gather_derivative": [_INT, _INT, _INT, _P, _I64, _INT, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P],
     "mpf_deposit_paths": [_INT, _INT, _INT, _P, _I64, _P, _I64, _P, _INT, _I64, _P, _P, _P, _P, _P, _INT, _INT, _P,
                           _P],
     "mpf_halo_fill": [_INT, _P, _I64, _I64, _P, _P, _P],
@@ -56,6 +57,11 @@ SIGNATURES: dict[str, list] = {
     "mpf_divergence": [_INT, _P, _P, _P, _P, _P],
     "mpf_gradient": [_INT, _P, _P, _P, _P, _P],
     "mpf_bin_sizes": [_INT, _P, _P],
+    "mpf_patch_plan_create": [_INT, _I64, _P, _I64, _I64],
+    "mpf_patch_plan_destroy": [_P],
+    "mpf_patch_plan_items": [_P],
+    "mpf_patches_scatter": [_P, _P, _I64, _P, _I64, _INT, _I64, _P, _INT, _INT, _P, _P],
+    "mpf_patches_gather": [_P, _P, _I64, _INT, _P, _I64, _P, _I64, _INT, _P, _P],
     "mpf_bin_markers": [_INT, _INT, _INT, _P, _I64, _P, _I64, _INT, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                         _INT, _P, _P],
     "mpf_scatter_binned": [_INT, _INT, _INT, _P, _P, _INT, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P],
@@ -70,7 +76,24 @@ for _d in (1, 2, 3):
     SIGNATURES[f"mpf_deposit_paths_{_d}d"] = _PATHS_IN + _geom(_d) + [_INT] + _TAIL
     SIGNATURES[f"mpf_deposit_paths_spline_{_d}d"] = _PATHS_IN + _geom(_d) + [_INT, _INT] + _TAIL
 
-_RESTYPES = {"mpf_last_error": ctypes.c_char_p, "mpf_context_create": _P, "mpf_context_destroy": None}
+_RESTYPES = {"mpf_last_error": ctypes.c_char_p, "mpf_context_create": _P, "mpf_context_destroy": None,
+             "mpf_patch_plan_create": _P, "mpf_patch_plan_destroy": None, "mpf_patch_plan_items": _I64}
+
+
+class Patch(ctypes.Structure):
+    """The C struct mpf_patch of include/mpf_hip.h."""
+
+    _fields_ = [
+        ("lower", ctypes.c_double * 3),
+        ("cell_size", ctypes.c_double * 3),
+        ("num_nodes", ctypes.c_int64 * 3),
+        ("code", ctypes.c_int64 * 3),
+        ("kind", ctypes.c_int32),
+        ("degree", ctypes.c_int32),
+        ("first_marker", ctypes.c_int64),
+        ("num_markers", ctypes.c_int64),
+        ("field_offset", ctypes.c_int64),
+    ]
 
 _LIB: ctypes.CDLL | None = None
 

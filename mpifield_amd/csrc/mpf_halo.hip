@@ -11,31 +11,35 @@
 namespace mpf {
 
 // One axis: `outer` = product of the storage extents before the axis, `inner` = product
-// after it times ncomp.  One work-item per (outer, layer, inner) triple.
+// after it times ncomp.  One work-item per (outer, inner) column runs over the h layers in
+// order, so a short interior (fewer than 2h nodes, where the two seams' layers overlap)
+// is folded exactly as layer after layer on one rank.
 __global__ void halo_axis_kernel(double *__restrict__ data, int64_t outer, int64_t len, int64_t inner, int64_t h,
                                  int mode, int periodic) {
   const int64_t n = len - 2 * h;
-  const int64_t total = outer * h * inner;
+  const int64_t total = outer * inner;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = t % inner;
-    const int64_t l = (t / inner) % h;
-    const int64_t o = t / (inner * h);
+    const int64_t o = t / inner;
     double *base = data + o * len * inner + i;
-    double *lo_halo = base + l * inner, *hi_halo = base + (n + h + l) * inner;
-    double *lo_in = base + (h + l) * inner, *hi_in = base + (n + l) * inner;
-    if (mode == 0) {  // fill
-      if (periodic) {
-        *lo_halo = *hi_in;
-        *hi_halo = *lo_in;
-      }
-    } else {  // fold: add across the seam, then clear the halos
-      if (periodic) {
+    for (int64_t l = 0; l < h; ++l) {
+      double *lo_halo = base + l * inner, *hi_halo = base + (n + h + l) * inner;
+      double *lo_in = base + (h + l) * inner, *hi_in = base + (n + l) * inner;
+      if (mode == 0) {  // fill
+        if (periodic) {
+          *lo_halo = *hi_in;
+          *hi_halo = *lo_in;
+        }
+      } else if (periodic) {  // fold: add across the seam
         *lo_in += *hi_halo;
         *hi_in += *lo_halo;
       }
-      *lo_halo = 0.0;
-      *hi_halo = 0.0;
     }
+    if (mode == 1)  // then clear the halos
+      for (int64_t l = 0; l < h; ++l) {
+        base[l * inner] = 0.0;
+        base[(n + h + l) * inner] = 0.0;
+      }
   }
 }
 
@@ -47,8 +51,8 @@ static int halo(int mode, int ndim, const int64_t *shape, int64_t ncomp, int64_t
   }
   if (h == 0) return MPF_OK;
   for (int d = 0; d < ndim; ++d) {
-    if (shape[d] - 2 * h < 2 * h) {
-      set_error("halo: axis %d stores %lld nodes, fewer than 2 * halo interior nodes plus its halos", d,
+    if (shape[d] - 2 * h < h) {
+      set_error("halo: axis %d stores %lld nodes, fewer than halo interior nodes plus its halos", d,
                 (long long)shape[d]);
       return MPF_ERR_ARGUMENT;
     }
@@ -57,7 +61,7 @@ static int halo(int mode, int ndim, const int64_t *shape, int64_t ncomp, int64_t
     int64_t outer = 1, inner = ncomp;
     for (int d = 0; d < axis; ++d) outer *= shape[d];
     for (int d = axis + 1; d < ndim; ++d) inner *= shape[d];
-    const int64_t total = outer * h * inner;
+    const int64_t total = outer * inner;
     hipLaunchKernelGGL(halo_axis_kernel, dim3(blocks_for(total, 256, 1 << 16)), dim3(256), 0, stream, data, outer,
                        shape[axis], inner, h, mode, (int)(periodic[axis] != 0));
     MPF_LAUNCH_CHECK();

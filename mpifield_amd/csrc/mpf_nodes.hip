@@ -87,6 +87,77 @@ __global__ void __launch_bounds__(256) gather_direct(const T *__restrict__ pos, 
   }
 }
 
+// d/dx of the stencil weights along one axis, in cells (times the inverse spacing by the
+// caller).  Zero where the position is clamped to a bounded wall or for degree 0.
+template <int KIND>
+__device__ inline int axis_derivative(double rel, int64_t num, int code, int degree, int64_t *node, double *w,
+                                      double *dw) {
+  const int size = axis_stencil<KIND>(rel, num, code, degree, node, w);
+  const bool clamped = code == 0 && (rel < 0.0 || rel > (double)(num - 1));
+  const int p = KIND == LINEAR ? 1 : (KIND == QUADRATIC ? 2 : degree);
+  if (clamped || p == 0) {
+    for (int a = 0; a < size; ++a) dw[a] = 0.0;
+    return size;
+  }
+  if (p == 1) {
+    // linear and spline p=1: w = (1 - f, f), f = rel - left within the cell
+    dw[0] = -1.0;
+    dw[1] = 1.0;
+    if (KIND == LINEAR && code == 0 && rel > (double)(num - 1)) dw[0] = dw[1] = 0.0;
+    return size;
+  }
+  // the offset from the middle node, recomputed as the stencil did
+  double r = rel;
+  if (code == 1 && !(r >= 0.0 && r < (double)num)) r = r - (double)num * floor(r / (double)num);
+  const double o = p == 2 ? r - (double)(int64_t)(r + 0.5) : r - (double)(int64_t)r;
+  if (p == 2) {
+    dw[0] = -(0.5 - o);
+    dw[1] = -2.0 * o;
+    dw[2] = 0.5 + o;
+  } else {
+    dw[0] = -0.5 * (1.0 - o) * (1.0 - o);
+    dw[1] = 1.5 * o * o - 2.0 * o;
+    dw[2] = -1.5 * o * o + o + 0.5;
+    dw[3] = 0.5 * o * o;
+  }
+  return size;
+}
+
+// out[(d * ncomp + c) * os + m] = d/dx_d of the interpolant of component c at marker m.
+template <class T, int KIND, int NDIM>
+__global__ void __launch_bounds__(256) gather_derivative_direct(const T *__restrict__ pos, int64_t ps, int64_t n,
+                                                                int ncomp, Grid g, int degree,
+                                                                const double *__restrict__ field, T *__restrict__ out,
+                                                                int64_t os) {
+  for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x) {
+    int64_t node[3][4];
+    double w[3][4], dw[3][4];
+    int size[3] = {1, 1, 1};
+    node[1][0] = node[2][0] = 0;
+    w[1][0] = w[2][0] = 1.0;
+    dw[1][0] = dw[2][0] = 0.0;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+      const double rel = ((double)pos[d * ps + m] - g.lower[d]) * g.inverse[d];
+      size[d] = axis_derivative<KIND>(rel, g.num[d], g.code[d], degree, node[d], w[d], dw[d]);
+    }
+    for (int c = 0; c < ncomp; ++c) {
+      double total[3] = {0.0, 0.0, 0.0};
+      for (int a = 0; a < size[0]; ++a)
+        for (int b = 0; b < size[1]; ++b)
+          for (int e = 0; e < size[2]; ++e) {
+            const double f =
+                field[(node[0][a] * g.stride[0] + node[1][b] * g.stride[1] + node[2][e] * g.stride[2]) * ncomp + c];
+            total[0] += dw[0][a] * w[1][b] * w[2][e] * f;
+            if (NDIM >= 2) total[1] += w[0][a] * dw[1][b] * w[2][e] * f;
+            if (NDIM == 3) total[2] += w[0][a] * w[1][b] * dw[2][e] * f;
+          }
+#pragma unroll
+      for (int d = 0; d < NDIM; ++d) out[(d * ncomp + c) * os + m] = (T)(total[d] * g.inverse[d]);
+    }
+  }
+}
+
 template <class T, int KIND, int NDIM>
 static void launch_scatter(int mode, const void *pos, int64_t ps, const void *val, int64_t vs, int64_t n, int ncomp,
                            const Grid &g, int degree, double *field, unsigned long long *accum, const double *slot,
@@ -236,6 +307,60 @@ extern "C" int mpf_gather(int kind, int degree, int ndim, const void *positions,
   else
     dispatch_gather<double>(kind, ndim, positions, pos_stride, n, (int)ncomp, g, kind_degree(kind, degree), field,
                             result, result_stride, stream);
+  MPF_LAUNCH_CHECK();
+  return MPF_OK;
+}
+
+template <class T, int KIND>
+static void dispatch_derivative_ndim(int ndim, const void *pos, int64_t ps, int64_t n, int ncomp, const Grid &g,
+                                     int degree, const double *field, void *out, int64_t os, hipStream_t stream) {
+  const int threads = 256, blocks = blocks_for(n, threads, 1 << 16);
+  if (ndim == 1)
+    hipLaunchKernelGGL((gather_derivative_direct<T, KIND, 1>), dim3(blocks), dim3(threads), 0, stream, (const T *)pos,
+                       ps, n, ncomp, g, degree, field, (T *)out, os);
+  if (ndim == 2)
+    hipLaunchKernelGGL((gather_derivative_direct<T, KIND, 2>), dim3(blocks), dim3(threads), 0, stream, (const T *)pos,
+                       ps, n, ncomp, g, degree, field, (T *)out, os);
+  if (ndim == 3)
+    hipLaunchKernelGGL((gather_derivative_direct<T, KIND, 3>), dim3(blocks), dim3(threads), 0, stream, (const T *)pos,
+                       ps, n, ncomp, g, degree, field, (T *)out, os);
+}
+
+extern "C" int mpf_gather_derivative(int kind, int degree, int ndim, const void *positions, int64_t pos_stride,
+                                     int dtype, int64_t n, const double *field, int64_t ncomp, void *out,
+                                     int64_t out_stride, const double *lower, const double *cell_size,
+                                     const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
+                                     mpf_stream stream_) {
+  (void)ctx;
+  int rc = check_common(kind, degree, dtype, n, ncomp);
+  if (rc) return rc;
+  for (int d = 0; d < ndim && d < 3; ++d)
+    if (code[d] == 2) {
+      set_error("the derivative gather does not take clamped axes");
+      return MPF_ERR_UNSUPPORTED;
+    }
+  Grid g;
+  rc = make_grid(g, ndim, kind, kind_degree(kind, degree), lower, cell_size, num_nodes, code);
+  if (rc) return rc;
+  if (n == 0) return MPF_OK;
+  hipStream_t stream = (hipStream_t)stream_;
+  const int deg = kind_degree(kind, degree);
+#define DER(T)                                                                                                    \
+  if (kind == LINEAR)                                                                                             \
+    dispatch_derivative_ndim<T, LINEAR>(ndim, positions, pos_stride, n, (int)ncomp, g, deg, field, out, out_stride, \
+                                        stream);                                                                  \
+  else if (kind == QUADRATIC)                                                                                     \
+    dispatch_derivative_ndim<T, QUADRATIC>(ndim, positions, pos_stride, n, (int)ncomp, g, deg, field, out,        \
+                                           out_stride, stream);                                                   \
+  else                                                                                                            \
+    dispatch_derivative_ndim<T, SPLINE>(ndim, positions, pos_stride, n, (int)ncomp, g, deg, field, out, out_stride, \
+                                        stream);
+  if (dtype == F32) {
+    DER(float)
+  } else {
+    DER(double)
+  }
+#undef DER
   MPF_LAUNCH_CHECK();
   return MPF_OK;
 }

@@ -127,7 +127,9 @@ def _split(geometry, ndim_extra: int = 0):
     return ndim, lower, cell, num, code, extra
 
 
-def _expected_extent(num, code, degree):
+def _expected_extent(num, code, degree, kind: int = _lib.KIND_SPLINE):
+    if kind != _lib.KIND_SPLINE and any(int(c) == 2 for c in code):
+        raise ValueError("a clamped axis (code 2) needs the spline kernels")
     return tuple(int(n) + (degree - 1 if c == 2 else 0) for n, c in zip(num, code))
 
 
@@ -164,7 +166,7 @@ def scatter(kind: int, degree: int, positions, values, field: torch.Tensor, geom
         raise ValueError(f"{pos.shape[1]} positions for {val.shape[1]} values")
     ncomp = val.shape[0]
     deg = {0: 1, 1: 2}.get(kind, degree)
-    extent = _expected_extent(num, code, deg)
+    extent = _expected_extent(num, code, deg, kind)
     if tuple(field.shape) != (*extent, ncomp):
         raise ValueError(f"a field on this grid is {(*extent, ncomp)}, got {tuple(field.shape)}")
     mode_code = _mode_code() if mode is None else mode
@@ -190,7 +192,7 @@ def gather(kind: int, degree: int, positions, field: torch.Tensor, result: torch
     pos, ps = marker_rows(positions, ndim, result.dtype if result.dtype in (torch.float32, torch.float64) else None)
     ncomp = field.shape[-1]
     deg = {0: 1, 1: 2}.get(kind, degree)
-    extent = _expected_extent(num, code, deg)
+    extent = _expected_extent(num, code, deg, kind)
     if tuple(field.shape) != (*extent, ncomp):
         raise ValueError(f"a field on this grid is {(*extent, ncomp)}, got {tuple(field.shape)}")
     if result.dim() == 1:
@@ -230,7 +232,7 @@ def deposit_paths(kind: int, degree: int, start, end, charges, edges: torch.Tens
     if e.shape != s.shape or q.numel() != s.shape[1]:
         raise ValueError(f"paths from {tuple(s.shape)} to {tuple(e.shape)} for {q.numel()} charges")
     deg = 1 if kind == 0 else degree
-    extent = _expected_extent(num, code, deg)
+    extent = _expected_extent(num, code, deg, _lib.KIND_LINEAR if kind == 0 else _lib.KIND_SPLINE)
     if tuple(edges.shape) != (*extent, ndim):
         raise ValueError(f"an edge field on this grid is {(*extent, ndim)}, got {tuple(edges.shape)}")
     with torch.cuda.device(edges.device):

@@ -1,7 +1,9 @@
 """GPU parity of libmpfhip.so against the CPU oracle, through the C ABI (mpifield_amd.kernels).
 
-Bars: float64 markers in MODE_ATOMIC and MODE_FIXED agree with the oracle within 1e-12
-relative to the field's largest value (scatter) or the gathered values (gather); float32
+Bars: float64 markers agree with the oracle within 1e-12 relative to the field's largest
+value (scatter, MODE_ATOMIC) or the gathered values (gather); MODE_FIXED deposits round every
+contribution to a quantum of 2^-k (k from the batch's size and largest value) and are held
+to 1e-10, far inside the 1e-6 that the fixed-point form promises; float32
 markers run the same float64 cell selection, so the oracle fed the same float32 values
 agrees as tightly, and the float32 gather result within float32 rounding (1e-6).
 """
@@ -24,6 +26,7 @@ KINDS = [("linear", 0, 1), ("quadratic", 1, 2), ("spline", 2, 0), ("spline", 2, 
          ("spline", 2, 3)]
 CASES = [(1, (0,)), (1, (1,)), (2, (0, 1)), (2, (1, 1)), (3, (0, 0, 0)), (3, (1, 0, 1)), (3, (1, 1, 1))]
 MODES = ["atomic", "fixed"]
+BAR = {"atomic": 1e-12, "fixed": 1e-10}
 
 
 def geom(lower, cell, num, code):
@@ -66,7 +69,7 @@ def test_scatter_matches_oracle(name, kind, degree, ndim, periodic, mode):
     got = cuda(field)
     with K.deposit_mode(mode):
         K.scatter(kind, degree, cuda(pts), cuda(vals), got, g)
-    assert rel_err(got.cpu().numpy(), want) < 1e-12
+    assert rel_err(got.cpu().numpy(), want) < BAR[mode]
 
 
 @pytest.mark.parametrize("ndim,periodic", CASES)
@@ -92,7 +95,7 @@ def test_clamped_axes_match_oracle(degree, ndim, mode):
     got = cuda(field)
     with K.deposit_mode(mode):
         K.scatter(2, degree, cuda(pts), cuda(vals), got, g)
-    assert rel_err(got.cpu().numpy(), want) < 1e-12
+    assert rel_err(got.cpu().numpy(), want) < BAR[mode]
     want_g = np.zeros((field.shape[-1], pts.shape[1]))
     orc.gather("spline", pts, field, want_g, *g, degree=degree)
     got_g = torch.zeros_like(cuda(want_g))
@@ -115,7 +118,7 @@ def test_strided_rows_and_float32_markers():
         got = cuda(field)
         with K.deposit_mode(mode):
             K.scatter(1, 0, storage_p[:, :5000], storage_v[:, :5000], got, g)
-        assert rel_err(got.cpu().numpy(), want) < 1e-12, mode
+        assert rel_err(got.cpu().numpy(), want) < BAR[mode], mode
     want_g = np.zeros((3, 5000))
     orc.gather("quadratic", p32, field, want_g, *g)
     got_g = torch.zeros((3, 5000), dtype=torch.float32, device="cuda")
@@ -168,7 +171,7 @@ def test_path_deposits_match_oracle(degree, ndim, periodic, mode):
     got = torch.zeros((*shape, ndim), dtype=torch.float64, device="cuda")
     with K.deposit_mode(mode):
         K.deposit_paths(0 if degree == 0 else 1, max(degree, 1), cuda(start), cuda(end), cuda(q), got, g)
-    assert rel_err(got.cpu().numpy(), want) < 1e-12
+    assert rel_err(got.cpu().numpy(), want) < BAR[mode]
 
 
 @pytest.mark.parametrize("periodic", [0, 1])
