@@ -37,16 +37,17 @@ namespace mpf {
 // a = 0..P, OFF = P / 2 (linear: 0), clamped to [0, num - 1] on a bounded axis and taken
 // modulo num on a periodic one.  Same arithmetic as mpf_device.h's stencils.
 template <int KIND, int P>
-__device__ __forceinline__ int64_t anchor_axis(double rel, int64_t num, int code, double *w) {
+__device__ __forceinline__ int anchor_axis(double rel, int num, int code, double *w) {
+  const double nd = (double)num;
   double r = rel;
   if (KIND == LINEAR) {
-    int64_t l;
+    int l;
     if (code == 1) {
-      r = periodic_position(r, num);
-      const int64_t i = (int64_t)r;
+      if (!(r >= 0.0 && r < nd)) r = r - nd * floor(r / nd);
+      const int i = (int)fmin(fmax(r, 0.0), nd);  // in range for any input (NaN gives 0)
       l = i >= num ? num - 1 : i;
     } else {
-      const int64_t i = r >= 0.0 ? (int64_t)r : 0;
+      const int i = (int)fmin(fmax(r, 0.0), nd);
       l = i > num - 2 ? num - 2 : i;
     }
     const double d = r - (double)l;
@@ -56,12 +57,13 @@ __device__ __forceinline__ int64_t anchor_axis(double rel, int64_t num, int code
     return l;
   }
   if (code == 1) {
-    if (!(r >= 0.0 && r < (double)num)) r = r - (double)num * floor(r / (double)num);
+    if (!(r >= 0.0 && r < nd)) r = r - nd * floor(r / nd);
   } else {
     if (r < 0.0) r = 0.0;
-    if (r > (double)(num - 1)) r = (double)num - 1.0;
+    if (r > nd - 1.0) r = nd - 1.0;
   }
-  const int64_t middle = (P % 2 == 0) ? (int64_t)(r + 0.5) : (int64_t)r;
+  const double rc = fmin(fmax(r, 0.0), nd);  // r itself for any finite input
+  const int middle = (P % 2 == 0) ? (int)(rc + 0.5) : (int)rc;
   const double o = r - (double)middle;
   if (P == 0) {
     w[0] = 1.0;
@@ -116,8 +118,10 @@ __global__ void __launch_bounds__(256) bin_count_kernel(const T *__restrict__ po
       for (int d = 0; d < NDIM; ++d) {
         double w[4];
         const double rel = ((double)pos[d * ps + m] - g.lower[d]) * g.inverse[d];
-        int64_t a = anchor_axis<KIND, P>(rel, g.num[d], g.code[d], w);
-        if (g.code[d] == 1 && a >= g.num[d]) a -= g.num[d];
+        const int num = (int)g.num[d];
+        int a = anchor_axis<KIND, P>(rel, num, g.code[d], w);
+        if (g.code[d] == 1 && a >= num) a -= num;
+        a = a < 0 ? 0 : (a > num - 1 ? num - 1 : a);  // a key in range whatever the position
         k = k * g.num[d] + a;
       }
     }
@@ -189,192 +193,430 @@ __device__ __forceinline__ int tile_of_color(int k, int nt, int colors, int colo
   return 2 * k + color;
 }
 
-template <class T, int KIND, int P, int C, int TX, int TY, int TZ, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) scatter_tiled(const T *__restrict__ pos, const T *__restrict__ val,
+// The deposit of one tile, in parts of BLOCK cells (x-slabs of BLOCK / (TY * TZ) planes).
+// A part's markers are staged in LDS with coalesced loads (up to CAP of them; the rest are
+// read from HBM where they lie).  Each cell's markers are cut into items of about K_MIN
+// markers (more when the part holds many markers, so that at most ITEMS items exist), the
+// items are ordered by size, and each work-item runs over items: it sums an item's
+// contributions in exact float64 registers and adds them into the tile's int64 LDS
+// accumulator, so the lanes of a wavefront carry equal work whatever the cell counts.
+// The accumulator spans the tile's footprint before any wrap or clamp: stencil node
+// (i, j, k) of cell (lx, ly, lz) sits at ((lx + i) * LY + ly + j) * LZ + lz + k, a constant
+// offset from the cell.  Nodes beyond a bounded wall are folded onto the wall node after
+// the last part, and the tile is then added into the field once.
+// Copy markers [0, staged) of a part into LDS, stage[e * CAP + i] for e = x, y, z, values.
+// float32 goes straight from memory to LDS (global_load_lds_dword: the data takes no
+// registers, so every load of the work-item is in flight at once); float64 through
+// registers.  Ends with the work-item's loads complete; the caller synchronises.
+template <class T, int C, int BLOCK, int CAP, int ROWS>
+__device__ __forceinline__ void stage_markers(const T *__restrict__ pos, const T *__restrict__ val, int64_t n,
+                                              uint32_t staged, const uint8_t *row_of, const uint32_t *row_first,
+                                              const uint32_t *row_offset, T *stage) {
+  constexpr int SPT = (CAP + BLOCK - 1) / BLOCK;
+  const int t = threadIdx.x;
+  if constexpr (sizeof(T) == 4) {
+    const int wave_base = t & ~63;
+#pragma unroll
+    for (int u = 0; u < SPT; ++u) {
+      const uint32_t i = t + u * BLOCK;
+      if (i < staged) {
+        const int row = row_of[i];
+        const int64_t m = (int64_t)row_first[row] + (i - row_offset[row]);
+        const int lds = u * BLOCK + wave_base;  // this wavefront's 64 consecutive slots
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          __builtin_amdgcn_global_load_lds((const void *)(pos + d * n + m),
+                                           (__attribute__((address_space(3))) void *)(stage + d * CAP + lds), 4, 0, 0);
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+          __builtin_amdgcn_global_load_lds((const void *)(val + c * n + m),
+                                           (__attribute__((address_space(3))) void *)(stage + (3 + c) * CAP + lds), 4,
+                                           0, 0);
+      }
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  } else {
+    for (uint32_t i = t; i < staged; i += BLOCK) {
+      const int row = row_of[i];
+      const int64_t m = (int64_t)row_first[row] + (i - row_offset[row]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) stage[d * CAP + i] = pos[d * n + m];
+#pragma unroll
+      for (int c = 0; c < C; ++c) stage[(3 + c) * CAP + i] = val[c * n + m];
+    }
+  }
+}
+
+template <int KIND, int P, int C, int S, bool FIRST>
+__device__ __forceinline__ void deposit_marker(double (&a)[C][S][S][S], double x, double y, double z,
+                                               const double *v, const Grid &g, int num0, int num1, int num2,
+                                               double scale) {
+  double wx[4], wy[4], wz[4];
+  anchor_axis<KIND, P>((x - g.lower[0]) * g.inverse[0], num0, g.code[0], wx);
+  anchor_axis<KIND, P>((y - g.lower[1]) * g.inverse[1], num1, g.code[1], wy);
+  anchor_axis<KIND, P>((z - g.lower[2]) * g.inverse[2], num2, g.code[2], wz);
+  double wyz[S][S];
+#pragma unroll
+  for (int j = 0; j < S; ++j)
+#pragma unroll
+    for (int k = 0; k < S; ++k) wyz[j][k] = wy[j] * wz[k];
+#pragma unroll
+  for (int q = 0; q < C; ++q) {
+    const double vs = v[q] * scale;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const double u = odd_significand(wx[i] * vs);
+#pragma unroll
+      for (int j = 0; j < S; ++j)
+#pragma unroll
+        for (int k = 0; k < S; ++k) a[q][i][j][k] = __builtin_fma(u, wyz[j][k], FIRST ? BIAS : a[q][i][j][k]);
+    }
+  }
+}
+
+template <class T, int KIND, int P, int C, int TX, int TY, int TZ, int BLOCK, int CAP, int ITEMS, int K_MIN>
+__global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ pos, const T *__restrict__ val,
                                                        int64_t n, const uint32_t *__restrict__ start, Grid g,
                                                        Tiling tl, double *__restrict__ field,
                                                        const double *__restrict__ slot) {
   using Sh = StencilShape<KIND, P>;
   constexpr int S = Sh::S, OFF = Sh::OFF;
   constexpr int LX = TX + S - 1, LY = TY + S - 1, LZ = TZ + S - 1, L = LX * LY * LZ;
+  constexpr int NC = TX * TY * TZ, PC = BLOCK, PARTS = NC / PC, ROWS = PC / TZ;
+  constexpr int BUCKETS = 17;  // items by size: 16 or more, 15, ..., 1 (and an empty class)
+  static_assert(NC % PC == 0 && PC % TZ == 0 && ROWS <= 64 && PC <= 256, "tile parts");
+  static_assert(ITEMS > PC && ITEMS < 4096, "item table");
   __shared__ unsigned long long acc[C * L];
+  __shared__ T stage[(3 + C) * CAP];
+  __shared__ uint32_t row_first[ROWS], row_offset[ROWS + 1];
+  __shared__ uint32_t cell_first[PC], cell_count[PC];
+  __shared__ uint32_t bucket[BUCKETS];
+  __shared__ uint32_t item[ITEMS];  // cell | index within the cell << 8 | items of the cell << 20
+  __shared__ uint8_t row_of[CAP];
 
+  const int num0 = (int)g.num[0], num1 = (int)g.num[1], num2 = (int)g.num[2];
   const int kx = blockIdx.x % tl.tiles_x;
   const int ky = (blockIdx.x / tl.tiles_x) % tl.tiles_y;
   const int kz = blockIdx.x / (tl.tiles_x * tl.tiles_y);
-  const int tx = tile_of_color(kx, tl.ntx, tl.cx, tl.color_x);
-  const int ty = tile_of_color(ky, tl.nty, tl.cy, tl.color_y);
-  const int tz = tile_of_color(kz, tl.ntz, tl.cz, tl.color_z);
-  const int64_t x0 = (int64_t)tx * TX, y0 = (int64_t)ty * TY, z0 = (int64_t)tz * TZ;
+  const int x0 = tile_of_color(kx, tl.ntx, tl.cx, tl.color_x) * TX;
+  const int y0 = tile_of_color(ky, tl.nty, tl.cy, tl.color_y) * TY;
+  const int z0 = tile_of_color(kz, tl.ntz, tl.cz, tl.color_z) * TZ;
 
   for (int i = threadIdx.x; i < C * L; i += BLOCK) acc[i] = 0ull;
-  __syncthreads();
 
   const double scale = slot[0], inv = slot[1];
-  const int64_t chunk = (int64_t)slot[2];
-  for (int cl = threadIdx.x; cl < TX * TY * TZ; cl += BLOCK) {
-    const int lz = cl % TZ, ly = (cl / TZ) % TY, lx = cl / (TZ * TY);
-    const int64_t ax = x0 + lx, ay = y0 + ly, az = z0 + lz;
-    const bool inside = ax < g.num[0] && ay < g.num[1] && az < g.num[2];
-    const int64_t cell = (ax * g.num[1] + ay) * g.num[2] + az;
-    const int64_t first = inside ? (int64_t)start[cell] : 0;
-    const int64_t last = inside ? (int64_t)start[cell + 1] : 0;
-    double a[C][S][S][S];
-    int64_t done = first;
-    while (done < last) {
-      const int64_t stop = last - done > chunk ? done + chunk : last;
-#pragma unroll
-      for (int c = 0; c < C; ++c)
-#pragma unroll
-        for (int i = 0; i < S; ++i)
-#pragma unroll
-          for (int j = 0; j < S; ++j)
-#pragma unroll
-            for (int k = 0; k < S; ++k) a[c][i][j][k] = BIAS;
-      for (int64_t m = done; m < stop; ++m) {
-        double wx[4], wy[4], wz[4];
-        anchor_axis<KIND, P>(((double)pos[m] - g.lower[0]) * g.inverse[0], g.num[0], g.code[0], wx);
-        anchor_axis<KIND, P>(((double)pos[n + m] - g.lower[1]) * g.inverse[1], g.num[1], g.code[1], wy);
-        anchor_axis<KIND, P>(((double)pos[2 * n + m] - g.lower[2]) * g.inverse[2], g.num[2], g.code[2], wz);
-        double wyz[S][S];
-#pragma unroll
-        for (int j = 0; j < S; ++j)
-#pragma unroll
-          for (int k = 0; k < S; ++k) wyz[j][k] = wy[j] * wz[k];
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          const double vs = (double)val[c * n + m] * scale;
-#pragma unroll
-          for (int i = 0; i < S; ++i) {
-            const double u = odd_significand(wx[i] * vs);
-#pragma unroll
-            for (int j = 0; j < S; ++j)
-#pragma unroll
-              for (int k = 0; k < S; ++k) a[c][i][j][k] = __builtin_fma(u, wyz[j][k], a[c][i][j][k]);
-          }
-        }
+  const uint32_t chunk = slot[2] < 4.0e9 ? (uint32_t)slot[2] : 4000000000u;
+  const int t = threadIdx.x;
+  for (int part = 0; part < PARTS; ++part) {
+    // the part's cells (z-rows of TZ cells): their offsets in the binned arrays
+    {
+      const int cl = part * PC + t;
+      const int ax = x0 + cl / (TY * TZ), ay = y0 + (cl / TZ) % TY, az = z0 + cl % TZ;
+      uint32_t first = 0, count = 0;
+      if (ax < num0 && ay < num1 && az < num2) {
+        const int64_t cell = ((int64_t)ax * num1 + ay) * num2 + az;
+        first = start[cell];
+        count = start[cell + 1] - first;
       }
-      // add the cell's integers into the tile: node (ax - OFF + i, ...) sits at LDS
-      // (ax - OFF + i) - (x0 - OFF) after the bounded-axis clamp
-#pragma unroll
-      for (int i = 0; i < S; ++i) {
-        const int64_t nx = g.code[0] == 1 ? ax - OFF + i : wrap_index(ax - OFF + i, g.num[0], 0);
-        const int ix = (int)(nx - (x0 - OFF));
-#pragma unroll
-        for (int j = 0; j < S; ++j) {
-          const int64_t ny = g.code[1] == 1 ? ay - OFF + j : wrap_index(ay - OFF + j, g.num[1], 0);
-          const int iy = (int)(ny - (y0 - OFF));
-#pragma unroll
-          for (int k = 0; k < S; ++k) {
-            const int64_t nz = g.code[2] == 1 ? az - OFF + k : wrap_index(az - OFF + k, g.num[2], 0);
-            const int iz = (int)(nz - (z0 - OFF));
-            const int at = (ix * LY + iy) * LZ + iz;
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-              const long long q = biased_to_int(a[c][i][j][k]);
-              if (q != 0) atomicAdd(&acc[c * L + at], (unsigned long long)q);
-            }
-          }
-        }
-      }
-      done = stop;
+      cell_first[t] = first;
+      cell_count[t] = count;
     }
+    if (t < BUCKETS) bucket[t] = 0u;
+    __syncthreads();
+    if (t < 64) {  // row lengths and their prefix, one wavefront
+      uint32_t len = 0;
+      if (t < ROWS) {
+        row_first[t] = cell_first[t * TZ];
+        for (int k = 0; k < TZ; ++k) len += cell_count[t * TZ + k];
+      }
+      uint32_t incl = len;
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t up = __shfl_up(incl, off, 64);
+        if (t >= off) incl += up;
+      }
+      if (t < ROWS) row_offset[t + 1] = incl;
+      if (t == 0) row_offset[0] = 0;
+    }
+    __syncthreads();
+    const uint32_t total = row_offset[ROWS];
+    const uint32_t staged = total < (uint32_t)CAP ? total : (uint32_t)CAP;
+    // item size: K_MIN, or more when the part holds many markers (at most ITEMS items)
+    const uint32_t per = total / (ITEMS - PC) + 1;
+    const uint32_t k_eff = per > (uint32_t)K_MIN ? per : (uint32_t)K_MIN;
+    const uint32_t my_count = cell_count[t];
+    const uint32_t my_items = (my_count + k_eff - 1) / k_eff;
+    const uint32_t my_base = my_items ? my_count / my_items : 0, my_rem = my_items ? my_count % my_items : 0;
+    // sizes my_base + 1 (my_rem of them) and my_base (the rest)
+    auto size_class = [](uint32_t s) { return BUCKETS - 1 - (s < BUCKETS - 1 ? s : BUCKETS - 1); };
+    if (my_rem) atomicAdd(&bucket[size_class(my_base + 1)], my_rem);
+    if (my_items > my_rem) atomicAdd(&bucket[size_class(my_base)], my_items - my_rem);
+    {  // each staged marker's row
+      const int wave = t / 64, lane = t % 64;
+      for (int r = wave; r < ROWS; r += BLOCK / 64) {
+        const uint32_t lo = row_offset[r], hi = row_offset[r + 1] < staged ? row_offset[r + 1] : staged;
+        for (uint32_t i = lo + lane; i < hi; i += 64) row_of[i] = (uint8_t)r;
+      }
+    }
+    __syncthreads();
+    if (t == 0) {  // exclusive scan of the size classes
+      uint32_t run = 0;
+      for (int b = 0; b < BUCKETS; ++b) {
+        const uint32_t k = bucket[b];
+        bucket[b] = run;
+        run += k;
+      }
+    }
+    // stage the part's markers: all of a work-item's loads in flight together
+    stage_markers<T, C, BLOCK, CAP, ROWS>(pos, val, n, staged, row_of, row_first, row_offset, stage);
+    __syncthreads();
+    const uint32_t nitems = bucket[BUCKETS - 1];  // items of size 1 and more come first
+    {
+      const uint32_t b1 = size_class(my_base + 1), b0 = size_class(my_base);
+      for (uint32_t j = 0; j < my_items; ++j) {
+        const uint32_t at = atomicAdd(&bucket[j < my_rem ? b1 : b0], 1u);
+        item[at] = (uint32_t)t | (j << 8) | (my_items << 20);
+      }
+    }
+    __syncthreads();
+
+    // the items, largest first; consecutive lanes take consecutive items
+    for (uint32_t it = t; it < nitems; it += BLOCK) {
+      const uint32_t d = item[it];
+      const int c = d & 0xFF;
+      const uint32_t j = (d >> 8) & 0xFFF, items = d >> 20;
+      const uint32_t count = cell_count[c];
+      const uint32_t base_n = count / items, rem = count % items;
+      const uint32_t begin = j * base_n + (j < rem ? j : rem), size = base_n + (j < rem ? 1u : 0u);
+      const int row = c / TZ;
+      const int cl = part * PC + c;
+      const int base = ((cl / (TY * TZ)) * LY + (cl / TZ) % TY) * LZ + cl % TZ;
+      const int64_t first = (int64_t)cell_first[c] + begin;
+      const uint32_t j0 = row_offset[row] + (uint32_t)(cell_first[c] - row_first[row]) + begin;
+      auto fetch = [&](uint32_t r, double &x, double &y, double &z, double *v) {
+        const uint32_t jl = j0 + r;
+        if (jl < staged) {
+          x = (double)stage[jl];
+          y = (double)stage[CAP + jl];
+          z = (double)stage[2 * CAP + jl];
+#pragma unroll
+          for (int q = 0; q < C; ++q) v[q] = (double)stage[(3 + q) * CAP + jl];
+        } else {
+          const int64_t m = first + r;
+          x = (double)pos[m];
+          y = (double)pos[n + m];
+          z = (double)pos[2 * n + m];
+#pragma unroll
+          for (int q = 0; q < C; ++q) v[q] = (double)val[q * n + m];
+        }
+      };
+      double a[C][S][S][S];
+      for (uint32_t done = 0; done < size;) {
+        const uint32_t stop = size - done > chunk ? done + chunk : size;
+        {
+          double x, y, z, v[C];
+          fetch(done, x, y, z, v);
+          deposit_marker<KIND, P, C, S, true>(a, x, y, z, v, g, num0, num1, num2, scale);
+        }
+        for (uint32_t r = done + 1; r < stop; ++r) {
+          double x, y, z, v[C];
+          fetch(r, x, y, z, v);
+          deposit_marker<KIND, P, C, S, false>(a, x, y, z, v, g, num0, num1, num2, scale);
+        }
+#pragma unroll
+        for (int q = 0; q < C; ++q)
+#pragma unroll
+          for (int i = 0; i < S; ++i)
+#pragma unroll
+            for (int jj = 0; jj < S; ++jj)
+#pragma unroll
+              for (int k = 0; k < S; ++k)
+                atomicAdd(&acc[q * L + base + (i * LY + jj) * LZ + k],
+                          (unsigned long long)biased_to_int(a[q][i][jj][k]));
+        done = stop;
+      }
+    }
+    __syncthreads();  // the staging and the tables are reused by the next part
   }
-  __syncthreads();
+
+  // bounded axes: fold the footprint beyond a wall onto the wall node, axis by axis, so
+  // that a corner reaches the wall nodes in two steps (the reference folds the stencil)
+  const int t0[3] = {x0, y0, z0};
+  const int ext[3] = {LX, LY, LZ};
+  const int stride[3] = {LY * LZ, LZ, 1};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (g.code[d] == 1) continue;
+    const int num = (int)g.num[d];
+    if (t0[d] - OFF >= 0 && t0[d] - OFF + ext[d] - 1 <= num - 1) continue;  // no wall in reach
+    for (int i = t; i < L; i += BLOCK) {
+      const int id = (i / stride[d]) % ext[d];
+      const int gd = t0[d] - OFF + id;
+      if (gd >= 0 && gd <= num - 1) continue;
+      const int to = gd < 0 ? 0 : num - 1;
+      const int j = i + (to - gd) * stride[d];
+#pragma unroll
+      for (int q = 0; q < C; ++q) atomicAdd(&acc[q * L + j], acc[q * L + i]);
+    }
+    __syncthreads();
+  }
 
   // the tile into the field: plain read-modify-write, the colouring keeps tiles apart
-  for (int i = threadIdx.x; i < L; i += BLOCK) {
+  for (int i = t; i < L; i += BLOCK) {
     const int iz = i % LZ, iy = (i / LZ) % LY, ix = i / (LZ * LY);
-    int64_t gx = x0 - OFF + ix, gy = y0 - OFF + iy, gz = z0 - OFF + iz;
-    if (g.code[0] == 1) gx = wrap_index(gx, g.num[0], 1);
-    if (g.code[1] == 1) gy = wrap_index(gy, g.num[1], 1);
-    if (g.code[2] == 1) gz = wrap_index(gz, g.num[2], 1);
-    if (gx < 0 || gx >= g.num[0] || gy < 0 || gy >= g.num[1] || gz < 0 || gz >= g.num[2]) continue;
-    const int64_t at = ((gx * g.num[1] + gy) * g.num[2] + gz) * C;
+    int gx = x0 - OFF + ix, gy = y0 - OFF + iy, gz = z0 - OFF + iz;
+    if (g.code[0] == 1) gx = gx < 0 ? gx + num0 : (gx >= num0 ? gx - num0 : gx);
+    if (g.code[1] == 1) gy = gy < 0 ? gy + num1 : (gy >= num1 ? gy - num1 : gy);
+    if (g.code[2] == 1) gz = gz < 0 ? gz + num2 : (gz >= num2 ? gz - num2 : gz);
+    if (gx < 0 || gx >= num0 || gy < 0 || gy >= num1 || gz < 0 || gz >= num2) continue;
+    const int64_t at = (((int64_t)gx * num1 + gy) * num2 + gz) * C;
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-      const long long q = (long long)acc[c * L + i];
-      if (q != 0) field[at + c] += (double)q * inv;
+    for (int q = 0; q < C; ++q) {
+      const long long v = (long long)acc[q * L + i];
+      if (v != 0) field[at + q] += (double)v * inv;
     }
   }
 }
 
 // ---- tiled gather ---------------------------------------------------------------------
-
-template <class T, int KIND, int P, int C, int TX, int TY, int TZ, int BLOCK>
+//
+// One workgroup per tile of TX x TY x TZ stencil cells.  The tile's node footprint is
+// staged in LDS as float64 (node-major, the C components of a node together), each
+// footprint position holding its node after the periodic wrap or the wall clamp, so that
+// stencil node (a, b, e) of a marker is a constant offset from its first node.  The tile's
+// markers are the TX*TY z-rows of TZ cells, each a contiguous run of the binned arrays; the
+// workgroup runs over their concatenation, one marker per work-item (a byte table in LDS
+// names each marker's row), so every lane has work and consecutive lanes read consecutive
+// markers.  Tiles are numbered so that each XCD (workgroups go round-robin over the eight)
+// takes a contiguous range of them and neighbouring footprints meet in its L2.
+template <class T, int KIND, int P, int C, int TX, int TY, int TZ, int BLOCK, int CAPM>
 __global__ void __launch_bounds__(BLOCK) gather_tiled(const T *__restrict__ pos, int64_t n,
                                                       const uint32_t *__restrict__ start, Grid g, int ntx, int nty,
                                                       const double *__restrict__ field, T *__restrict__ res) {
   using Sh = StencilShape<KIND, P>;
   constexpr int S = Sh::S, OFF = Sh::OFF;
   constexpr int LX = TX + S - 1, LY = TY + S - 1, LZ = TZ + S - 1, L = LX * LY * LZ;
-  __shared__ double tile[C * L];
+  constexpr int ROWS = TX * TY, WAVES = BLOCK / 64;
+  static_assert(ROWS <= 256, "row ids are bytes");
+  __shared__ double tile[L * C];
+  __shared__ uint32_t row_first[ROWS];
+  __shared__ uint32_t row_offset[ROWS + 1];
+  __shared__ uint8_t row_of[CAPM];
 
-  const int tx = blockIdx.x % ntx, ty = (blockIdx.x / ntx) % nty, tz = blockIdx.x / (ntx * nty);
-  const int64_t x0 = (int64_t)tx * TX, y0 = (int64_t)ty * TY, z0 = (int64_t)tz * TZ;
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int xcd = b % 8, per = nb / 8, extra = nb % 8;
+  const int id = xcd * per + (xcd < extra ? xcd : extra) + b / 8;
+  const int num0 = (int)g.num[0], num1 = (int)g.num[1], num2 = (int)g.num[2];
+  const int x0 = (id % ntx) * TX, y0 = ((id / ntx) % nty) * TY, z0 = (id / (ntx * nty)) * TZ;
+#pragma unroll
   for (int i = threadIdx.x; i < L; i += BLOCK) {
     const int iz = i % LZ, iy = (i / LZ) % LY, ix = i / (LZ * LY);
-    // the footprint node, wrapped on a periodic axis and clamped (only reached by clamped
-    // stencils) on a bounded one
-    const int64_t gx = wrap_index(x0 - OFF + ix, g.num[0], g.code[0]);
-    const int64_t gy = wrap_index(y0 - OFF + iy, g.num[1], g.code[1]);
-    const int64_t gz = wrap_index(z0 - OFF + iz, g.num[2], g.code[2]);
-    const int64_t at = ((gx * g.num[1] + gy) * g.num[2] + gz) * C;
+    const int64_t gx = wrap_index(x0 - OFF + ix, num0, g.code[0]);
+    const int64_t gy = wrap_index(y0 - OFF + iy, num1, g.code[1]);
+    const int64_t gz = wrap_index(z0 - OFF + iz, num2, g.code[2]);
+    const int64_t at = ((gx * num1 + gy) * num2 + gz) * C;
 #pragma unroll
-    for (int c = 0; c < C; ++c) tile[c * L + i] = field[at + c];
+    for (int c = 0; c < C; ++c) tile[i * C + c] = field[at + c];
+  }
+  // the tile's z-rows: their first binned marker and their lengths, then the prefix
+  if (threadIdx.x < 64) {
+    uint32_t carry = 0;
+    for (int base = 0; base < ROWS; base += 64) {
+      const int r = base + threadIdx.x;
+      uint32_t len = 0, first = 0;
+      if (r < ROWS) {
+        const int ax = x0 + r / TY, ay = y0 + r % TY;
+        if (ax < num0 && ay < num1) {
+          const int zend = z0 + TZ < num2 ? z0 + TZ : num2;
+          const int64_t c0 = ((int64_t)ax * num1 + ay) * num2 + z0;
+          first = start[c0];
+          len = start[c0 + (zend - z0)] - first;
+        }
+        row_first[r] = first;
+      }
+      uint32_t incl = len;  // inclusive wave scan
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t up = __shfl_up(incl, off, 64);
+        if ((int)threadIdx.x >= off) incl += up;
+      }
+      if (r < ROWS) row_offset[r + 1] = carry + incl;
+      carry += __shfl(incl, 63, 64);
+    }
+    if (threadIdx.x == 0) row_offset[0] = 0;
+  }
+  __syncthreads();
+  {  // each marker's row, for the first CAPM markers
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    for (int r = wave; r < ROWS; r += WAVES) {
+      const uint32_t lo = row_offset[r], hi = row_offset[r + 1] < (uint32_t)CAPM ? row_offset[r + 1] : CAPM;
+      for (uint32_t i = lo + lane; i < hi; i += 64) row_of[i] = (uint8_t)r;
+    }
   }
   __syncthreads();
 
-  for (int cl = threadIdx.x; cl < TX * TY * TZ; cl += BLOCK) {
-    const int lz = cl % TZ, ly = (cl / TZ) % TY, lx = cl / (TZ * TY);
-    const int64_t ax = x0 + lx, ay = y0 + ly, az = z0 + lz;
-    if (!(ax < g.num[0] && ay < g.num[1] && az < g.num[2])) continue;
-    const int64_t cell = (ax * g.num[1] + ay) * g.num[2] + az;
-    const int64_t first = start[cell], last = start[cell + 1];
-    if (first == last) continue;
-    double f[C][S][S][S];
+  const uint32_t total = row_offset[ROWS];
+  // U markers per work-item at a time: their position loads are in flight together
+  constexpr int U = 8;
+  for (uint32_t i0 = threadIdx.x; i0 < total; i0 += U * BLOCK) {
+    int64_t mm[U];
+    T px[U], py[U], pz[U];
 #pragma unroll
-    for (int i = 0; i < S; ++i) {
-      const int64_t nx = g.code[0] == 1 ? ax - OFF + i : wrap_index(ax - OFF + i, g.num[0], 0);
-      const int ix = (int)(nx - (x0 - OFF));
-#pragma unroll
-      for (int j = 0; j < S; ++j) {
-        const int64_t ny = g.code[1] == 1 ? ay - OFF + j : wrap_index(ay - OFF + j, g.num[1], 0);
-        const int iy = (int)(ny - (y0 - OFF));
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-          const int64_t nz = g.code[2] == 1 ? az - OFF + k : wrap_index(az - OFF + k, g.num[2], 0);
-          const int iz = (int)(nz - (z0 - OFF));
-#pragma unroll
-          for (int c = 0; c < C; ++c) f[c][i][j][k] = tile[c * L + (ix * LY + iy) * LZ + iz];
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = i0 + u * BLOCK;
+      mm[u] = -1;
+      if (i < total) {
+        int row;
+        if (i < (uint32_t)CAPM) {
+          row = row_of[i];
+        } else {  // the row holding marker i: row_offset[row] <= i < row_offset[row + 1]
+          int lo = 0, hi = ROWS - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (row_offset[mid] <= i) lo = mid;
+            else hi = mid - 1;
+          }
+          row = lo;
         }
+        const int64_t m = (int64_t)row_first[row] + (i - row_offset[row]);
+        mm[u] = m;
+        px[u] = pos[m];
+        py[u] = pos[n + m];
+        pz[u] = pos[2 * n + m];
       }
     }
-    for (int64_t m = first; m < last; ++m) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (mm[u] < 0) continue;
+      const int64_t m = mm[u];
       double wx[4], wy[4], wz[4];
-      anchor_axis<KIND, P>(((double)pos[m] - g.lower[0]) * g.inverse[0], g.num[0], g.code[0], wx);
-      anchor_axis<KIND, P>(((double)pos[n + m] - g.lower[1]) * g.inverse[1], g.num[1], g.code[1], wy);
-      anchor_axis<KIND, P>(((double)pos[2 * n + m] - g.lower[2]) * g.inverse[2], g.num[2], g.code[2], wz);
-      double w[S][S][S];
+      int bx = anchor_axis<KIND, P>(((double)px[u] - g.lower[0]) * g.inverse[0], num0, g.code[0], wx);
+      int by = anchor_axis<KIND, P>(((double)py[u] - g.lower[1]) * g.inverse[1], num1, g.code[1], wy);
+      int bz = anchor_axis<KIND, P>(((double)pz[u] - g.lower[2]) * g.inverse[2], num2, g.code[2], wz);
+      // the anchor cell within the tile (binning put it there; the clamp keeps LDS reads inside)
+      bx = (g.code[0] == 1 && bx >= num0 ? bx - num0 : bx) - x0;
+      by = (g.code[1] == 1 && by >= num1 ? by - num1 : by) - y0;
+      bz = (g.code[2] == 1 && bz >= num2 ? bz - num2 : bz) - z0;
+      bx = bx < 0 ? 0 : (bx > TX - 1 ? TX - 1 : bx);
+      by = by < 0 ? 0 : (by > TY - 1 ? TY - 1 : by);
+      bz = bz < 0 ? 0 : (bz > TZ - 1 ? TZ - 1 : bz);
+      const double *node = tile + ((bx * LY + by) * LZ + bz) * C;
+      double total_c[C];
 #pragma unroll
-      for (int i = 0; i < S; ++i)
+      for (int c = 0; c < C; ++c) total_c[c] = 0.0;
 #pragma unroll
-        for (int j = 0; j < S; ++j) {
-          const double wab = wx[i] * wy[j];
+      for (int a = 0; a < S; ++a)
 #pragma unroll
-          for (int k = 0; k < S; ++k) w[i][j][k] = wab * wz[k];
+        for (int b2 = 0; b2 < S; ++b2) {
+          const double wab = wx[a] * wy[b2];
+#pragma unroll
+          for (int e = 0; e < S; ++e) {
+            const double weight = wab * wz[e];
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+              total_c[c] = __builtin_fma(weight, node[((a * LY + b2) * LZ + e) * C + c], total_c[c]);
+          }
         }
 #pragma unroll
-      for (int c = 0; c < C; ++c) {
-        double total = 0.0;
-#pragma unroll
-        for (int i = 0; i < S; ++i)
-#pragma unroll
-          for (int j = 0; j < S; ++j)
-#pragma unroll
-            for (int k = 0; k < S; ++k) total += w[i][j][k] * f[c][i][j][k];
-        res[c * n + m] = (T)total;
-      }
+      for (int c = 0; c < C; ++c) res[c * n + m] = (T)total_c[c];
     }
   }
 }
@@ -414,12 +656,14 @@ struct TileShape {
   int tx, ty, tz, block;
 };
 
-// Tile extents per (kind, degree, ncomp); LDS holds C x (T + S - 1)^3 int64 / float64.
+// Tile extents in stencil cells (the deposit's LDS holds C x (T + S - 1)^3 int64).
+constexpr int TILE_X = 16, TILE_Y = 8, TILE_Z = 8;
+
 static TileShape tile_shape(int kind, int degree, int ncomp) {
   (void)kind;
   (void)degree;
   (void)ncomp;
-  return TileShape{8, 8, 16, 256};
+  return TileShape{TILE_X, TILE_Y, TILE_Z, 256};
 }
 
 static int colors_for(int nt, int code) {
@@ -584,7 +828,7 @@ extern "C" int mpf_bin_markers(int kind, int degree, int ndim, const void *posit
 template <class T, int KIND, int P, int C>
 static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const uint32_t *start, const Grid &g,
                              double *field, const double *scalars, hipStream_t stream) {
-  constexpr int TX = 8, TY = 8, TZ = 16, BLOCK = 256;
+  constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, BLOCK = 256, CAP = 1152, ITEMS = 768, K_MIN = 3;
   Tiling tl;
   tl.tx = TX;
   tl.ty = TY;
@@ -612,7 +856,8 @@ static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const 
         const int blocks = tl.tiles_x * tl.tiles_y * tl.tiles_z;
         if (blocks == 0) continue;
         ProfileScope prof("scatter_tiled", stream);
-        hipLaunchKernelGGL((scatter_tiled<T, KIND, P, C, TX, TY, TZ, BLOCK>), dim3(blocks), dim3(BLOCK), 0, stream,
+        hipLaunchKernelGGL((scatter_tiled<T, KIND, P, C, TX, TY, TZ, BLOCK, CAP, ITEMS, K_MIN>), dim3(blocks),
+                           dim3(BLOCK), 0, stream,
                            (const T *)pos, (const T *)val, n, start, g, tl, field, scalars);
         MPF_LAUNCH_CHECK();
       }
@@ -622,12 +867,12 @@ static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const 
 template <class T, int KIND, int P, int C>
 static int run_gather_tiled(const void *pos, int64_t n, const uint32_t *start, const Grid &g, const double *field,
                             void *res, hipStream_t stream) {
-  constexpr int TX = 8, TY = 8, TZ = 16, BLOCK = 256;
+  constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, BLOCK = 256, CAPM = 8192;
   const int ntx = (int)((g.num[0] + TX - 1) / TX), nty = (int)((g.num[1] + TY - 1) / TY),
             ntz = (int)((g.num[2] + TZ - 1) / TZ);
   {
     ProfileScope prof("gather_tiled", stream);
-    hipLaunchKernelGGL((gather_tiled<T, KIND, P, C, TX, TY, TZ, BLOCK>), dim3(ntx * nty * ntz), dim3(BLOCK), 0,
+    hipLaunchKernelGGL((gather_tiled<T, KIND, P, C, TX, TY, TZ, BLOCK, CAPM>), dim3(ntx * nty * ntz), dim3(BLOCK), 0,
                        stream, (const T *)pos, n, start, g, ntx, nty, field, (T *)res);
   }
   MPF_LAUNCH_CHECK();

@@ -20,7 +20,7 @@ if torch.cuda.is_available():
 NAMES = {(0, 1): "linear", (1, 2): "quadratic", (2, 3): "spline"}
 TILED = [(0, 1, 1), (0, 1, 3), (1, 2, 1), (1, 2, 3), (2, 3, 1)]  # (kind, degree, ncomp)
 GRIDS = [
-    ((16, 16, 32), (1, 1, 1)),
+    ((32, 16, 32), (1, 1, 1)),
     ((24, 16, 32), (0, 1, 0)),   # bounded x and z: folds at the walls
     ((21, 13, 37), (0, 0, 0)),   # bounded, partial tiles on every axis
 ]
@@ -96,7 +96,7 @@ def test_binned_deposit_is_bit_reproducible_under_any_marker_order():
 
 def test_weighting_routes_large_batches_through_the_bins():
     """Weighting/kernels calls above BINNED_MIN_MARKERS bin by themselves; same answers."""
-    num, code = (16, 16, 32), (1, 0, 1)
+    num, code = (32, 16, 32), (1, 0, 1)
     g = geom(num, code)
     n = K.BINNED_MIN_MARKERS + 1000
     pos, val = markers(num, code, n, 3, seed=9, dtype=np.float32)
