@@ -198,16 +198,19 @@ int mpf_gradient(int ndim, const int64_t *shape, const int64_t *periodic, const 
 
 /* ---- the binned (tiled) path ------------------------------------------------------- */
 
-/* The number of stencil cells of a grid (product of num_nodes); the offsets array of
- * mpf_bin_markers holds cells + 2 entries. */
+/* The number of bins of a grid: its stencil cells (product of num_nodes) in 1D and 2D; in
+ * 3D the cells padded to whole tiles of 8 x 8 x 16.  The offsets array of mpf_bin_markers
+ * holds bins + 2 entries. */
 int mpf_bin_sizes(int ndim, const int64_t *num_nodes, int64_t *cells);
 
 /* On-device counting sort of markers by stencil cell (the cell whose nodes a marker's
- * stencil covers, found with the reference's float64 arithmetic), row-major cell order;
- * markers whose values are all zero go to a trailing bin.  values may be NULL with
+ * stencil covers, found with the reference's float64 arithmetic): in 3D the cells of each
+ * tile of 8 x 8 x 16 cells are consecutive bins (x-major within the tile, tiles x-major),
+ * so a tile's markers are one contiguous run; in 1D and 2D, row-major cell order.  Markers
+ * whose values are all zero go to a trailing bin.  values may be NULL with
  * ncomp 0 (a gather's markers: all live).  Writes, into caller-owned device memory:
  * sorted_positions (ndim, n) and sorted_values (ncomp, n) with contiguous rows, order[n]
- * (int64: sorted index -> input index), start[cells + 2] (uint32 offsets; start[cells] is
+ * (int64: sorted index -> input index), start[bins + 2] (uint32 offsets; start[bins] is
  * the first dead marker) and scalars[3] = {2^k, 2^-k, markers per register pass}, the
  * fixed-point scale with n * max|value| * 2^k < 2^61 and max|value| * 2^k < 2^49
  * (scale_exp != MPF_SCALE_AUTO fixes k).  No counterpart in the reference: its banded CPU

@@ -47,7 +47,7 @@ class MarkerBins:
     positions: torch.Tensor  # (ndim, N) sorted, contiguous rows
     values: torch.Tensor | None  # (ncomp, N) sorted, or None for gather-only bins
     order: torch.Tensor  # (N,) int64: sorted index -> input index
-    start: torch.Tensor  # (cells + 2,) int32 holding uint32 offsets
+    start: torch.Tensor  # (bins + 2,) int32 holding uint32 offsets (3D bins in tile order)
     scalars: torch.Tensor  # (3,) float64: [2^k, 2^-k, markers per register pass]
 
     @property

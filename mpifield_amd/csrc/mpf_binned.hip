@@ -4,27 +4,27 @@
 // Binning.  A marker's stencil cell ("anchor") is the cell whose nodes it touches, found
 // with the same float64 arithmetic as the reference's stencils (quadratic:
 // int(r + 1/2); linear: the left node; spline p: int(r + 1/2) for even p, int(r) for odd p;
-// wrapped on a periodic axis, clamped on a bounded one).  Keys are row-major anchor
-// indices; a count pass (atomic ranks), an exclusive scan and a placement pass give the
-// markers sorted by anchor, the cell offsets and the permutation.  Markers whose values
-// are all zero (dead) go to a trailing bin that deposits nothing.
+// wrapped on a periodic axis, clamped on a bounded one).  3D keys put the cells of a tile
+// next to each other (tile order), so that a tile's markers are one contiguous run; a
+// count pass (atomic ranks), an exclusive scan and a placement pass give the markers
+// sorted by key, the bin offsets and the permutation.  Markers whose values are all zero
+// (dead) go to a trailing bin that deposits nothing.
 //
-// Deposit.  One workgroup per tile of TX x TY x TZ anchor cells; one lane per cell.  A
-// lane runs over its cell's markers and accumulates the (p+1)^3 x C contributions in
-// float64 registers as exact integers: every contribution is rounded to an integer
-// number of fixed-point quanta inside one fma on an accumulator biased by 1.5 * 2^52, where
-// the unit in the last place is one quantum; the factors are arranged so that the exact
-// product is never a tie (see exact_fma below), hence the sum is independent of the
-// marker order.  Each lane then adds its integers into the tile's int64 LDS accumulator
-// (covering the tile plus its stencil halo), and the workgroup adds the tile into the
-// float64 field once.  Tiles whose footprints overlap run in different launches (a 2- or
-// 3-colouring per axis), in a fixed order, so the field update needs no atomics and is
-// bit-reproducible.
+// Deposit.  One workgroup per tile of anchor cells, in parts of 256 cells whose markers are
+// staged in LDS.  The markers of a cell are cut into items of about three; a lane runs over
+// an item and accumulates its (p+1)^3 x C contributions in float64 registers as exact
+// integers: every contribution is rounded to an integer number of fixed-point quanta
+// inside one fma on an accumulator biased by 1.5 * 2^52, where the unit in the last place
+// is one quantum; the factors are arranged so that the exact product is never a tie (see
+// odd_significand below), hence the sum is independent of the marker order.  Each lane
+// then adds its integers into the tile's int64 LDS accumulator (covering the tile plus its
+// stencil halo), and the workgroup adds the tile into the float64 field once.  Tiles whose
+// footprints overlap run in different launches (a 2- or 3-colouring per axis), in a fixed
+// order, so the field update needs no atomics and is bit-reproducible.
 //
-// Gather.  One workgroup per tile: the tile's node footprint is staged in LDS; a lane
-// loads its cell's (p+1)^3 x C node values into registers once and interpolates each of
-// its markers, with the weight products and the sum in the reference's order
-// (gather_quadratic_3d_pyccel.py:103-114).
+// Gather.  One workgroup per tile: the tile's node footprint is staged in LDS and each
+// lane interpolates one marker at a time, with the weight products in the reference's
+// order (gather_quadratic_3d_pyccel.py:103-114).
 #include <hipcub/hipcub.hpp>
 
 #include "mpf_internal.h"
@@ -96,6 +96,25 @@ __device__ __forceinline__ int64_t wrap_index(int64_t k, int64_t num, int code) 
 }
 
 // ---- binning ---------------------------------------------------------------------------
+//
+// 3D markers are binned by anchor cell in tile order: the cells of a tile of
+// TILE_X x TILE_Y x TILE_Z (x-major, z fastest) are consecutive bins and tiles follow each
+// other in x-major order, so the markers of a tile (and of each x-slab of it) are one
+// contiguous run of the binned arrays.  Partial tiles at the grid's end keep their empty
+// bins.  1D and 2D markers are binned in row-major cell order.
+constexpr int TILE_X = 8, TILE_Y = 8, TILE_Z = 16, TILE_CELLS = TILE_X * TILE_Y * TILE_Z;
+
+// Bins of a grid (before the dead bin): tile-padded cells in 3D, cells otherwise.
+__host__ __device__ __forceinline__ int64_t bins_of(int ndim, const int64_t *num) {
+  if (ndim == 3) {
+    const int64_t ntx = (num[0] + TILE_X - 1) / TILE_X, nty = (num[1] + TILE_Y - 1) / TILE_Y,
+                  ntz = (num[2] + TILE_Z - 1) / TILE_Z;
+    return ntx * nty * ntz * TILE_CELLS;
+  }
+  int64_t c = 1;
+  for (int d = 0; d < ndim; ++d) c *= num[d];
+  return c;
+}
 
 template <class T, int KIND, int P, int NDIM>
 __global__ void __launch_bounds__(256) bin_count_kernel(const T *__restrict__ pos, int64_t ps,
@@ -113,7 +132,7 @@ __global__ void __launch_bounds__(256) bin_count_kernel(const T *__restrict__ po
     }
     int64_t k = dead_key;
     if (live) {
-      k = 0;
+      int cell[NDIM];
 #pragma unroll
       for (int d = 0; d < NDIM; ++d) {
         double w[4];
@@ -121,8 +140,16 @@ __global__ void __launch_bounds__(256) bin_count_kernel(const T *__restrict__ po
         const int num = (int)g.num[d];
         int a = anchor_axis<KIND, P>(rel, num, g.code[d], w);
         if (g.code[d] == 1 && a >= num) a -= num;
-        a = a < 0 ? 0 : (a > num - 1 ? num - 1 : a);  // a key in range whatever the position
-        k = k * g.num[d] + a;
+        cell[d] = a < 0 ? 0 : (a > num - 1 ? num - 1 : a);  // a key in range whatever the position
+      }
+      if constexpr (NDIM == 3) {
+        const int64_t nty = (g.num[1] + TILE_Y - 1) / TILE_Y, ntz = (g.num[2] + TILE_Z - 1) / TILE_Z;
+        const int64_t tile = ((int64_t)(cell[0] / TILE_X) * nty + cell[1] / TILE_Y) * ntz + cell[2] / TILE_Z;
+        k = tile * TILE_CELLS + ((cell[0] % TILE_X) * TILE_Y + cell[1] % TILE_Y) * TILE_Z + cell[2] % TILE_Z;
+      } else {
+        k = 0;
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) k = k * g.num[d] + cell[d];
       }
     }
     key[m] = (uint32_t)k;
@@ -193,25 +220,14 @@ __device__ __forceinline__ int tile_of_color(int k, int nt, int colors, int colo
   return 2 * k + color;
 }
 
-// The deposit of one tile, in parts of BLOCK cells (x-slabs of BLOCK / (TY * TZ) planes).
-// A part's markers are staged in LDS with coalesced loads (up to CAP of them; the rest are
-// read from HBM where they lie).  Each cell's markers are cut into items of about K_MIN
-// markers (more when the part holds many markers, so that at most ITEMS items exist), the
-// items are ordered by size, and each work-item runs over items: it sums an item's
-// contributions in exact float64 registers and adds them into the tile's int64 LDS
-// accumulator, so the lanes of a wavefront carry equal work whatever the cell counts.
-// The accumulator spans the tile's footprint before any wrap or clamp: stencil node
-// (i, j, k) of cell (lx, ly, lz) sits at ((lx + i) * LY + ly + j) * LZ + lz + k, a constant
-// offset from the cell.  Nodes beyond a bounded wall are folded onto the wall node after
-// the last part, and the tile is then added into the field once.
-// Copy markers [0, staged) of a part into LDS, stage[e * CAP + i] for e = x, y, z, values.
-// float32 goes straight from memory to LDS (global_load_lds_dword: the data takes no
-// registers, so every load of the work-item is in flight at once); float64 through
-// registers.  Ends with the work-item's loads complete; the caller synchronises.
-template <class T, int C, int BLOCK, int CAP, int ROWS>
+// Copy markers [first, first + staged) of a part into LDS, stage[e * CAP + i] for e = x, y,
+// z, values: one contiguous run of the binned arrays.  float32 goes straight from memory to
+// LDS (global_load_lds_dword: the data takes no registers, so every load of the work-item
+// is in flight at once); float64 through registers.  Ends with the work-item's loads
+// complete; the caller synchronises.
+template <class T, int C, int BLOCK, int CAP>
 __device__ __forceinline__ void stage_markers(const T *__restrict__ pos, const T *__restrict__ val, int64_t n,
-                                              uint32_t staged, const uint8_t *row_of, const uint32_t *row_first,
-                                              const uint32_t *row_offset, T *stage) {
+                                              int64_t first, uint32_t staged, T *stage) {
   constexpr int SPT = (CAP + BLOCK - 1) / BLOCK;
   const int t = threadIdx.x;
   if constexpr (sizeof(T) == 4) {
@@ -220,8 +236,7 @@ __device__ __forceinline__ void stage_markers(const T *__restrict__ pos, const T
     for (int u = 0; u < SPT; ++u) {
       const uint32_t i = t + u * BLOCK;
       if (i < staged) {
-        const int row = row_of[i];
-        const int64_t m = (int64_t)row_first[row] + (i - row_offset[row]);
+        const int64_t m = first + i;
         const int lds = u * BLOCK + wave_base;  // this wavefront's 64 consecutive slots
 #pragma unroll
         for (int d = 0; d < 3; ++d)
@@ -237,8 +252,7 @@ __device__ __forceinline__ void stage_markers(const T *__restrict__ pos, const T
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   } else {
     for (uint32_t i = t; i < staged; i += BLOCK) {
-      const int row = row_of[i];
-      const int64_t m = (int64_t)row_first[row] + (i - row_offset[row]);
+      const int64_t m = first + i;
 #pragma unroll
       for (int d = 0; d < 3; ++d) stage[d * CAP + i] = pos[d * n + m];
 #pragma unroll
@@ -274,33 +288,46 @@ __device__ __forceinline__ void deposit_marker(double (&a)[C][S][S][S], double x
   }
 }
 
-template <class T, int KIND, int P, int C, int TX, int TY, int TZ, int BLOCK, int CAP, int ITEMS, int K_MIN>
+// The deposit of one tile of TILE_X x TILE_Y x TILE_Z stencil cells, in parts of BLOCK
+// cells (x-slabs).  The bins are in tile order, so a part's markers are one contiguous run
+// of the binned arrays, staged in LDS with coalesced loads (up to CAP of them; the rest are
+// read from memory where they lie).  Each cell's markers are cut into items of about K_MIN
+// markers (more when the part holds many markers, so that at most ITEMS items exist), the
+// items are ordered by size, and each work-item runs over items: it sums an item's
+// contributions in exact float64 registers and adds them into the tile's int64 LDS
+// accumulator, so the lanes of a wavefront carry equal work whatever the cell counts.
+// The accumulator spans the tile's footprint before any wrap or clamp: stencil node
+// (i, j, k) of cell (lx, ly, lz) sits at ((lx + i) * LY + ly + j) * LZ + lz + k, a constant
+// offset from the cell.  Nodes beyond a bounded wall are folded onto the wall node after
+// the last part, and the tile is then added into the field once.
+template <class T, int KIND, int P, int C, int BLOCK, int CAP, int ITEMS, int K_MIN>
 __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ pos, const T *__restrict__ val,
-                                                       int64_t n, const uint32_t *__restrict__ start, Grid g,
-                                                       Tiling tl, double *__restrict__ field,
-                                                       const double *__restrict__ slot) {
+                                                          int64_t n, const uint32_t *__restrict__ start, Grid g,
+                                                          Tiling tl, double *__restrict__ field,
+                                                          const double *__restrict__ slot) {
   using Sh = StencilShape<KIND, P>;
+  constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, TC = TILE_CELLS;
   constexpr int S = Sh::S, OFF = Sh::OFF;
   constexpr int LX = TX + S - 1, LY = TY + S - 1, LZ = TZ + S - 1, L = LX * LY * LZ;
-  constexpr int NC = TX * TY * TZ, PC = BLOCK, PARTS = NC / PC, ROWS = PC / TZ;
+  constexpr int PC = BLOCK, PARTS = TC / PC;
   constexpr int BUCKETS = 17;  // items by size: 16 or more, 15, ..., 1 (and an empty class)
-  static_assert(NC % PC == 0 && PC % TZ == 0 && ROWS <= 64 && PC <= 256, "tile parts");
-  static_assert(ITEMS > PC && ITEMS < 4096, "item table");
+  static_assert(TC % PC == 0 && PC % (TY * TZ) == 0, "tile parts are x-slabs");
+  static_assert(ITEMS > PC && ITEMS < 4096 && PC <= 256, "item table");
   __shared__ unsigned long long acc[C * L];
   __shared__ T stage[(3 + C) * CAP];
-  __shared__ uint32_t row_first[ROWS], row_offset[ROWS + 1];
-  __shared__ uint32_t cell_first[PC], cell_count[PC];
+  __shared__ uint32_t cell_first[PC + 1], cell_count[PC];
   __shared__ uint32_t bucket[BUCKETS];
   __shared__ uint32_t item[ITEMS];  // cell | index within the cell << 8 | items of the cell << 20
-  __shared__ uint8_t row_of[CAP];
 
   const int num0 = (int)g.num[0], num1 = (int)g.num[1], num2 = (int)g.num[2];
   const int kx = blockIdx.x % tl.tiles_x;
   const int ky = (blockIdx.x / tl.tiles_x) % tl.tiles_y;
   const int kz = blockIdx.x / (tl.tiles_x * tl.tiles_y);
-  const int x0 = tile_of_color(kx, tl.ntx, tl.cx, tl.color_x) * TX;
-  const int y0 = tile_of_color(ky, tl.nty, tl.cy, tl.color_y) * TY;
-  const int z0 = tile_of_color(kz, tl.ntz, tl.cz, tl.color_z) * TZ;
+  const int tx = tile_of_color(kx, tl.ntx, tl.cx, tl.color_x);
+  const int ty = tile_of_color(ky, tl.nty, tl.cy, tl.color_y);
+  const int tz = tile_of_color(kz, tl.ntz, tl.cz, tl.color_z);
+  const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
+  const int64_t tile_key = (((int64_t)tx * tl.nty + ty) * tl.ntz + tz) * TC;
 
   for (int i = threadIdx.x; i < C * L; i += BLOCK) acc[i] = 0ull;
 
@@ -308,37 +335,17 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
   const uint32_t chunk = slot[2] < 4.0e9 ? (uint32_t)slot[2] : 4000000000u;
   const int t = threadIdx.x;
   for (int part = 0; part < PARTS; ++part) {
-    // the part's cells (z-rows of TZ cells): their offsets in the binned arrays
+    // the part's cells: their offsets (bins of cells beyond a partial tile are empty)
     {
-      const int cl = part * PC + t;
-      const int ax = x0 + cl / (TY * TZ), ay = y0 + (cl / TZ) % TY, az = z0 + cl % TZ;
-      uint32_t first = 0, count = 0;
-      if (ax < num0 && ay < num1 && az < num2) {
-        const int64_t cell = ((int64_t)ax * num1 + ay) * num2 + az;
-        first = start[cell];
-        count = start[cell + 1] - first;
-      }
+      const int64_t key = tile_key + part * PC + t;
+      const uint32_t first = start[key], next = start[key + 1];
       cell_first[t] = first;
-      cell_count[t] = count;
+      cell_count[t] = next - first;
+      if (t == PC - 1) cell_first[PC] = next;
     }
     if (t < BUCKETS) bucket[t] = 0u;
     __syncthreads();
-    if (t < 64) {  // row lengths and their prefix, one wavefront
-      uint32_t len = 0;
-      if (t < ROWS) {
-        row_first[t] = cell_first[t * TZ];
-        for (int k = 0; k < TZ; ++k) len += cell_count[t * TZ + k];
-      }
-      uint32_t incl = len;
-      for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t up = __shfl_up(incl, off, 64);
-        if (t >= off) incl += up;
-      }
-      if (t < ROWS) row_offset[t + 1] = incl;
-      if (t == 0) row_offset[0] = 0;
-    }
-    __syncthreads();
-    const uint32_t total = row_offset[ROWS];
+    const uint32_t part_first = cell_first[0], total = cell_first[PC] - part_first;
     const uint32_t staged = total < (uint32_t)CAP ? total : (uint32_t)CAP;
     // item size: K_MIN, or more when the part holds many markers (at most ITEMS items)
     const uint32_t per = total / (ITEMS - PC) + 1;
@@ -350,13 +357,7 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
     auto size_class = [](uint32_t s) { return BUCKETS - 1 - (s < BUCKETS - 1 ? s : BUCKETS - 1); };
     if (my_rem) atomicAdd(&bucket[size_class(my_base + 1)], my_rem);
     if (my_items > my_rem) atomicAdd(&bucket[size_class(my_base)], my_items - my_rem);
-    {  // each staged marker's row
-      const int wave = t / 64, lane = t % 64;
-      for (int r = wave; r < ROWS; r += BLOCK / 64) {
-        const uint32_t lo = row_offset[r], hi = row_offset[r + 1] < staged ? row_offset[r + 1] : staged;
-        for (uint32_t i = lo + lane; i < hi; i += 64) row_of[i] = (uint8_t)r;
-      }
-    }
+    stage_markers<T, C, BLOCK, CAP>(pos, val, n, part_first, staged, stage);
     __syncthreads();
     if (t == 0) {  // exclusive scan of the size classes
       uint32_t run = 0;
@@ -366,8 +367,6 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
         run += k;
       }
     }
-    // stage the part's markers: all of a work-item's loads in flight together
-    stage_markers<T, C, BLOCK, CAP, ROWS>(pos, val, n, staged, row_of, row_first, row_offset, stage);
     __syncthreads();
     const uint32_t nitems = bucket[BUCKETS - 1];  // items of size 1 and more come first
     {
@@ -387,11 +386,10 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
       const uint32_t count = cell_count[c];
       const uint32_t base_n = count / items, rem = count % items;
       const uint32_t begin = j * base_n + (j < rem ? j : rem), size = base_n + (j < rem ? 1u : 0u);
-      const int row = c / TZ;
       const int cl = part * PC + c;
       const int base = ((cl / (TY * TZ)) * LY + (cl / TZ) % TY) * LZ + cl % TZ;
       const int64_t first = (int64_t)cell_first[c] + begin;
-      const uint32_t j0 = row_offset[row] + (uint32_t)(cell_first[c] - row_first[row]) + begin;
+      const uint32_t j0 = cell_first[c] - part_first + begin;
       auto fetch = [&](uint32_t r, double &x, double &y, double &z, double *v) {
         const uint32_t jl = j0 + r;
         if (jl < staged) {
@@ -479,34 +477,32 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
 
 // ---- tiled gather ---------------------------------------------------------------------
 //
-// One workgroup per tile of TX x TY x TZ stencil cells.  The tile's node footprint is
-// staged in LDS as float64 (node-major, the C components of a node together), each
-// footprint position holding its node after the periodic wrap or the wall clamp, so that
-// stencil node (a, b, e) of a marker is a constant offset from its first node.  The tile's
-// markers are the TX*TY z-rows of TZ cells, each a contiguous run of the binned arrays; the
-// workgroup runs over their concatenation, one marker per work-item (a byte table in LDS
-// names each marker's row), so every lane has work and consecutive lanes read consecutive
-// markers.  Tiles are numbered so that each XCD (workgroups go round-robin over the eight)
-// takes a contiguous range of them and neighbouring footprints meet in its L2.
-template <class T, int KIND, int P, int C, int TX, int TY, int TZ, int BLOCK, int CAPM>
+// One workgroup per tile of TILE_X x TILE_Y x TILE_Z stencil cells.  The tile's node
+// footprint is staged in LDS as float64 (node-major, the C components of a node together),
+// each footprint position holding its node after the periodic wrap or the wall clamp, so
+// that stencil node (a, b, e) of a marker is a constant offset from its first node.  The
+// tile's markers are one contiguous run of the binned arrays (bins in tile order); the
+// workgroup runs over it, one marker per work-item, so every lane has work and consecutive
+// lanes read consecutive markers.  Tiles are numbered so that each XCD (workgroups go
+// round-robin over the eight) takes a contiguous range of them and neighbouring
+// footprints meet in its L2.
+template <class T, int KIND, int P, int C, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) gather_tiled(const T *__restrict__ pos, int64_t n,
                                                       const uint32_t *__restrict__ start, Grid g, int ntx, int nty,
-                                                      const double *__restrict__ field, T *__restrict__ res) {
+                                                      int ntz, const double *__restrict__ field,
+                                                      T *__restrict__ res) {
   using Sh = StencilShape<KIND, P>;
+  constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, TC = TILE_CELLS;
   constexpr int S = Sh::S, OFF = Sh::OFF;
   constexpr int LX = TX + S - 1, LY = TY + S - 1, LZ = TZ + S - 1, L = LX * LY * LZ;
-  constexpr int ROWS = TX * TY, WAVES = BLOCK / 64;
-  static_assert(ROWS <= 256, "row ids are bytes");
   __shared__ double tile[L * C];
-  __shared__ uint32_t row_first[ROWS];
-  __shared__ uint32_t row_offset[ROWS + 1];
-  __shared__ uint8_t row_of[CAPM];
 
   const int nb = gridDim.x, b = blockIdx.x;
   const int xcd = b % 8, per = nb / 8, extra = nb % 8;
-  const int id = xcd * per + (xcd < extra ? xcd : extra) + b / 8;
+  const int id = xcd * per + (xcd < extra ? xcd : extra) + b / 8;  // the tile, in tile-key order
+  const int tz = id % ntz, ty = (id / ntz) % nty, tx = id / (ntz * nty);
   const int num0 = (int)g.num[0], num1 = (int)g.num[1], num2 = (int)g.num[2];
-  const int x0 = (id % ntx) * TX, y0 = ((id / ntx) % nty) * TY, z0 = (id / (ntx * nty)) * TZ;
+  const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
 #pragma unroll
   for (int i = threadIdx.x; i < L; i += BLOCK) {
     const int iz = i % LZ, iy = (i / LZ) % LY, ix = i / (LZ * LY);
@@ -517,67 +513,20 @@ __global__ void __launch_bounds__(BLOCK) gather_tiled(const T *__restrict__ pos,
 #pragma unroll
     for (int c = 0; c < C; ++c) tile[i * C + c] = field[at + c];
   }
-  // the tile's z-rows: their first binned marker and their lengths, then the prefix
-  if (threadIdx.x < 64) {
-    uint32_t carry = 0;
-    for (int base = 0; base < ROWS; base += 64) {
-      const int r = base + threadIdx.x;
-      uint32_t len = 0, first = 0;
-      if (r < ROWS) {
-        const int ax = x0 + r / TY, ay = y0 + r % TY;
-        if (ax < num0 && ay < num1) {
-          const int zend = z0 + TZ < num2 ? z0 + TZ : num2;
-          const int64_t c0 = ((int64_t)ax * num1 + ay) * num2 + z0;
-          first = start[c0];
-          len = start[c0 + (zend - z0)] - first;
-        }
-        row_first[r] = first;
-      }
-      uint32_t incl = len;  // inclusive wave scan
-      for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t up = __shfl_up(incl, off, 64);
-        if ((int)threadIdx.x >= off) incl += up;
-      }
-      if (r < ROWS) row_offset[r + 1] = carry + incl;
-      carry += __shfl(incl, 63, 64);
-    }
-    if (threadIdx.x == 0) row_offset[0] = 0;
-  }
-  __syncthreads();
-  {  // each marker's row, for the first CAPM markers
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    for (int r = wave; r < ROWS; r += WAVES) {
-      const uint32_t lo = row_offset[r], hi = row_offset[r + 1] < (uint32_t)CAPM ? row_offset[r + 1] : CAPM;
-      for (uint32_t i = lo + lane; i < hi; i += 64) row_of[i] = (uint8_t)r;
-    }
-  }
+  const int64_t tile_key = (int64_t)id * TC;
+  const int64_t tile_first = start[tile_key];
+  const uint32_t total = start[tile_key + TC] - (uint32_t)tile_first;
   __syncthreads();
 
-  const uint32_t total = row_offset[ROWS];
   // U markers per work-item at a time: their position loads are in flight together
   constexpr int U = 8;
   for (uint32_t i0 = threadIdx.x; i0 < total; i0 += U * BLOCK) {
-    int64_t mm[U];
     T px[U], py[U], pz[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t i = i0 + u * BLOCK;
-      mm[u] = -1;
       if (i < total) {
-        int row;
-        if (i < (uint32_t)CAPM) {
-          row = row_of[i];
-        } else {  // the row holding marker i: row_offset[row] <= i < row_offset[row + 1]
-          int lo = 0, hi = ROWS - 1;
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (row_offset[mid] <= i) lo = mid;
-            else hi = mid - 1;
-          }
-          row = lo;
-        }
-        const int64_t m = (int64_t)row_first[row] + (i - row_offset[row]);
-        mm[u] = m;
+        const int64_t m = tile_first + i;
         px[u] = pos[m];
         py[u] = pos[n + m];
         pz[u] = pos[2 * n + m];
@@ -585,8 +534,9 @@ __global__ void __launch_bounds__(BLOCK) gather_tiled(const T *__restrict__ pos,
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (mm[u] < 0) continue;
-      const int64_t m = mm[u];
+      const uint32_t i = i0 + u * BLOCK;
+      if (i >= total) continue;
+      const int64_t m = tile_first + i;
       double wx[4], wy[4], wz[4];
       int bx = anchor_axis<KIND, P>(((double)px[u] - g.lower[0]) * g.inverse[0], num0, g.code[0], wx);
       int by = anchor_axis<KIND, P>(((double)py[u] - g.lower[1]) * g.inverse[1], num1, g.code[1], wy);
@@ -656,8 +606,6 @@ struct TileShape {
   int tx, ty, tz, block;
 };
 
-// Tile extents in stencil cells (the deposit's LDS holds C x (T + S - 1)^3 int64).
-constexpr int TILE_X = 16, TILE_Y = 8, TILE_Z = 8;
 
 static TileShape tile_shape(int kind, int degree, int ncomp) {
   (void)kind;
@@ -701,9 +649,7 @@ extern "C" int mpf_bin_sizes(int ndim, const int64_t *num_nodes, int64_t *cells)
     set_error("ndim must be 1, 2 or 3");
     return MPF_ERR_ARGUMENT;
   }
-  int64_t c = 1;
-  for (int d = 0; d < ndim; ++d) c *= num_nodes[d];
-  *cells = c;  // the offsets array holds cells + 2 entries (the dead bin last)
+  *cells = bins_of(ndim, num_nodes);  // the offsets array holds bins + 2 entries (the dead bin last)
   return MPF_OK;
 }
 
@@ -739,10 +685,10 @@ static int dispatch_count(int kind, int degree, int ndim, const void *pos, int64
   return MPF_OK;
 }
 
-/* Bin markers by stencil cell (row-major anchor order).  Writes sorted positions
- * (ndim, n) and values (ncomp, n) with contiguous rows, the permutation order[sorted] =
- * original index (int64), the cell offsets start[cells + 2] (uint32; start[cells] is the
- * first dead marker, start[cells + 1] = n) and, in scalars[0..1], the fixed-point scale
+/* Bin markers by stencil cell (tile order in 3D, row-major cells otherwise).  Writes
+ * sorted positions (ndim, n) and values (ncomp, n) with contiguous rows, the permutation
+ * order[sorted] = original index (int64), the bin offsets start[bins + 2] (uint32;
+ * start[bins] is the first dead marker, start[bins + 1] = n) and, in scalars[0..1], the fixed-point scale
  * 2^k, 2^-k and the markers per register pass (scalars[2]); scale_exp overrides k. */
 extern "C" int mpf_bin_markers(int kind, int degree, int ndim, const void *positions, int64_t pos_stride,
                                const void *values, int64_t val_stride, int dtype, int64_t n, int64_t ncomp,
@@ -770,8 +716,7 @@ extern "C" int mpf_bin_markers(int kind, int degree, int ndim, const void *posit
       set_error("bin_markers: clamped axes are not binned (the direct kernels take them)");
       return MPF_ERR_UNSUPPORTED;
     }
-  int64_t cells = 1;
-  for (int d = 0; d < ndim; ++d) cells *= num_nodes[d];
+  const int64_t cells = bins_of(ndim, num_nodes);  // bins before the dead one
   if (cells + 2 > 0x7FFFFFFFll) {
     set_error("bin_markers: too many cells for 32-bit keys");
     return MPF_ERR_ARGUMENT;
@@ -856,7 +801,7 @@ static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const 
         const int blocks = tl.tiles_x * tl.tiles_y * tl.tiles_z;
         if (blocks == 0) continue;
         ProfileScope prof("scatter_tiled", stream);
-        hipLaunchKernelGGL((scatter_tiled<T, KIND, P, C, TX, TY, TZ, BLOCK, CAP, ITEMS, K_MIN>), dim3(blocks),
+        hipLaunchKernelGGL((scatter_tiled<T, KIND, P, C, BLOCK, CAP, ITEMS, K_MIN>), dim3(blocks),
                            dim3(BLOCK), 0, stream,
                            (const T *)pos, (const T *)val, n, start, g, tl, field, scalars);
         MPF_LAUNCH_CHECK();
@@ -867,16 +812,16 @@ static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const 
 template <class T, int KIND, int P, int C>
 static int run_gather_tiled(const void *pos, int64_t n, const uint32_t *start, const Grid &g, const double *field,
                             void *res, hipStream_t stream) {
-  constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, BLOCK = 256, CAPM = 8192;
+  constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, BLOCK = 256;
   const int ntx = (int)((g.num[0] + TX - 1) / TX), nty = (int)((g.num[1] + TY - 1) / TY),
             ntz = (int)((g.num[2] + TZ - 1) / TZ);
   {
     ProfileScope prof("gather_tiled", stream);
-    hipLaunchKernelGGL((gather_tiled<T, KIND, P, C, TX, TY, TZ, BLOCK, CAPM>), dim3(ntx * nty * ntz), dim3(BLOCK), 0,
-                       stream, (const T *)pos, n, start, g, ntx, nty, field, (T *)res);
+    hipLaunchKernelGGL((gather_tiled<T, KIND, P, C, BLOCK>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
+                       (const T *)pos, n, start, g, ntx, nty, ntz, field, (T *)res);
   }
   MPF_LAUNCH_CHECK();
-  const int64_t cells = g.num[0] * g.num[1] * g.num[2];
+  const int64_t cells = (int64_t)ntx * nty * ntz * TILE_CELLS;  // the dead bin follows the tiles' bins
   hipLaunchKernelGGL((gather_dead_kernel<T, KIND, C>), dim3(64), dim3(256), 0, stream, (const T *)pos, n, P, start,
                      cells, g, field, (T *)res);
   MPF_LAUNCH_CHECK();

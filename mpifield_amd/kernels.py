@@ -135,7 +135,7 @@ def _expected_extent(num, code, degree, kind: int = _lib.KIND_SPLINE):
 
 #: batches at least this large go through the binned (tiled) kernels where they apply
 BINNED_MIN_MARKERS = 1 << 18
-_TILE = (16, 8, 8)
+_TILE = (8, 8, 16)
 
 
 def tiled_ok(kind: int, degree: int, ncomp: int, num, code) -> bool:
