@@ -63,6 +63,22 @@ def algorithmic_bytes(n: int, nodes: int, comps: int, ndim: int = 3, marker_byte
     return {"scatter": scatter, "gather": gather, "pair": scatter + gather}
 
 
+TRAFFIC_SUMMARY = ROOT / "profiles" / "quadratic3d_256cubed_64M_traffic.json"
+
+
+def measured_traffic(kernel: str, n: int, nodes: int, comps: int) -> tuple[float | None, str | None]:
+    """HBM bytes per launch of `kernel` at the default workload, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (counters scaled by a plain copy of known size); None for
+    other workloads or when the summary is absent."""
+    if (n, nodes, comps) != (1 << 26, 256, 3) or not TRAFFIC_SUMMARY.exists():
+        return None, None
+    summary = json.loads(TRAFFIC_SUMMARY.read_text())
+    for name, value in summary.get("traffic_bytes_per_launch", {}).items():
+        if name.startswith(kernel) or (kernel == "gather_tiled" and name.startswith("gather_items")):
+            return float(value), str(TRAFFIC_SUMMARY.relative_to(ROOT))
+    return None, None
+
+
 def log(msg: str) -> None:
     print(f"# {msg}", file=sys.stderr, flush=True)
 
@@ -190,6 +206,8 @@ def main() -> None:
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     dk = kernels[dominant]
     achieved = dk["GBs"]
+    launches = dk["launches_per_step"]
+    traffic, traffic_source = measured_traffic(dominant, n, nodes, comps)
     line = {
         "metric": METRIC,
         "value": value,
@@ -220,7 +238,10 @@ def main() -> None:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_source,
+            "algorithmic_bytes_per_launch": dk["algorithmic_bytes"] / launches if launches else None,
+            "launches_per_step": launches,
         },
     }
     if not args.only_headline:

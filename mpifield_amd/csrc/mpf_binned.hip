@@ -220,44 +220,148 @@ __device__ __forceinline__ int tile_of_color(int k, int nt, int colors, int colo
   return 2 * k + color;
 }
 
-// Copy markers [first, first + staged) of a part into LDS, stage[e * CAP + i] for e = x, y,
-// z, values: one contiguous run of the binned arrays.  float32 goes straight from memory to
-// LDS (global_load_lds_dword: the data takes no registers, so every load of the work-item
-// is in flight at once); float64 through registers.  Ends with the work-item's loads
-// complete; the caller synchronises.
-template <class T, int C, int BLOCK, int CAP>
-__device__ __forceinline__ void stage_markers(const T *__restrict__ pos, const T *__restrict__ val, int64_t n,
-                                              int64_t first, uint32_t staged, T *stage) {
-  constexpr int SPT = (CAP + BLOCK - 1) / BLOCK;
-  const int t = threadIdx.x;
-  if constexpr (sizeof(T) == 4) {
-    const int wave_base = t & ~63;
+// ---- work items of a tile ---------------------------------------------------------------
+//
+// The markers of a cell are cut into at most SWEEPS items of about ITEM markers (sizes
+// differ by at most one).  A tile's items are listed part by part (a part is BLOCK
+// consecutive cells of the tile, one per work-item), within a part sweep by sweep (the
+// first item of every cell, then the second, ...) and within a sweep in cell order.  The
+// lanes of a wavefront therefore take items of neighbouring cells: their LDS addresses for
+// one stencil node are consecutive (no bank conflicts) and their markers lie close together
+// in the binned arrays.  An item is a uint16: cell | sweep << 12.
+constexpr int ITEM = 3, SWEEPS = 4;
+
+__device__ __forceinline__ uint32_t items_of(uint32_t count) {
+  const uint32_t k = (count + ITEM - 1) / ITEM;
+  return k < (uint32_t)SWEEPS ? k : (uint32_t)SWEEPS;
+}
+
+__device__ __forceinline__ unsigned long long wave_inclusive_sum(unsigned long long v) {
+  const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int u = 0; u < SPT; ++u) {
-      const uint32_t i = t + u * BLOCK;
-      if (i < staged) {
-        const int64_t m = first + i;
-        const int lds = u * BLOCK + wave_base;  // this wavefront's 64 consecutive slots
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned long long o = __shfl_up(v, off, 64);
+    if (lane >= off) v += o;
+  }
+  return v;
+}
+
+// Fill items[] for the cells whose marker offsets are cfirst[0..TC]; returns the number of
+// items.  One block-wide scan: each work-item packs its cell's "has a j-th item" flags
+// (j < SWEEPS) into 16-bit fields of one 64-bit word, so a single prefix sum ranks the
+// cell in every sweep at once.  Ends with a barrier.
+template <int BLOCK, int TC>
+__device__ uint32_t build_items(const uint32_t *cfirst, uint16_t *items, unsigned long long *wave_sums) {
+  constexpr int PARTS = TC / BLOCK, WAVES = BLOCK / 64;
+  static_assert(TC % BLOCK == 0 && TC <= 4096 && BLOCK % 64 == 0, "item table");
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  unsigned long long mine[PARTS], incl[PARTS];
+  uint32_t nitems[PARTS];
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-          __builtin_amdgcn_global_load_lds((const void *)(pos + d * n + m),
-                                           (__attribute__((address_space(3))) void *)(stage + d * CAP + lds), 4, 0, 0);
+  for (int p = 0; p < PARTS; ++p) {
+    const int c = p * BLOCK + t;
+    nitems[p] = items_of(cfirst[c + 1] - cfirst[c]);
+    unsigned long long f = 0;
 #pragma unroll
-        for (int c = 0; c < C; ++c)
-          __builtin_amdgcn_global_load_lds((const void *)(val + c * n + m),
-                                           (__attribute__((address_space(3))) void *)(stage + (3 + c) * CAP + lds), 4,
-                                           0, 0);
-      }
+    for (int j = 0; j < SWEEPS; ++j) f |= (unsigned long long)(nitems[p] > (uint32_t)j) << (16 * j);
+    mine[p] = f;
+    incl[p] = wave_inclusive_sum(f);
+    if (lane == 63) wave_sums[w * PARTS + p] = incl[p];
+  }
+  __syncthreads();
+  uint32_t base = 0;
+#pragma unroll
+  for (int p = 0; p < PARTS; ++p) {
+    unsigned long long before = 0, total = 0;
+#pragma unroll
+    for (int v = 0; v < WAVES; ++v) {
+      const unsigned long long s = wave_sums[v * PARTS + p];
+      before += v < w ? s : 0ull;
+      total += s;
     }
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    const unsigned long long rank = before + incl[p] - mine[p];
+#pragma unroll
+    for (int j = 0; j < SWEEPS; ++j) {
+      if (nitems[p] > (uint32_t)j) items[base + (uint32_t)((rank >> (16 * j)) & 0xFFFF)] = (uint16_t)((p * BLOCK + t) | (j << 12));
+      base += (uint32_t)((total >> (16 * j)) & 0xFFFF);
+    }
+  }
+  __syncthreads();
+  return base;
+}
+
+// The tile's TC + 1 cell offsets into LDS, every load issued before the first store.
+template <int BLOCK, int TC>
+__device__ __forceinline__ void load_offsets(const uint32_t *__restrict__ start, uint32_t *cfirst) {
+  constexpr int E = (TC + 1 + BLOCK - 1) / BLOCK;
+  uint32_t v[E];
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
+    const int i = threadIdx.x + u * BLOCK;
+    if (i <= TC) v[u] = start[i];
+  }
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
+    const int i = threadIdx.x + u * BLOCK;
+    if (i <= TC) cfirst[i] = v[u];
+  }
+}
+
+struct Item {
+  int64_t begin;  // first marker (binned index)
+  uint32_t size;  // markers, at least one
+  int cell;       // cell within the tile
+};
+
+__device__ __forceinline__ Item item_at(const uint16_t *items, const uint32_t *cfirst, uint32_t it) {
+  const uint32_t e = items[it];
+  const int c = (int)(e & 0xFFF);
+  const uint32_t j = e >> 12;
+  const uint32_t first = cfirst[c], count = cfirst[c + 1] - first;
+  const uint32_t k = items_of(count);
+  uint32_t q, r;
+  if (k == 1) {
+    q = count;
+    r = 0;
+  } else if (k == 2) {
+    q = count >> 1;
+    r = count & 1;
+  } else if (k == 3) {
+    q = count / 3;
+    r = count - 3 * q;
   } else {
-    for (uint32_t i = t; i < staged; i += BLOCK) {
-      const int64_t m = first + i;
+    q = count >> 2;
+    r = count & 3;
+  }
+  Item out;
+  out.begin = (int64_t)first + j * q + (j < r ? j : r);
+  out.size = q + (j < r ? 1u : 0u);
+  out.cell = c;
+  return out;
+}
+
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima): every vector-memory load
+// of the work-item has completed.
+__device__ __forceinline__ void wait_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// The first ITEM markers of an item, loaded ahead of their use (indices past the item's end
+// repeat its first marker, which exists, so every load is in bounds).
+template <class T, int C>
+struct Ahead {
+  T x[ITEM], y[ITEM], z[ITEM], v[ITEM][C > 0 ? C : 1];
+};
+
+template <class T, int C>
+__device__ __forceinline__ void load_ahead(const T *__restrict__ pos, const T *__restrict__ val, int64_t n,
+                                           const Item &I, Ahead<T, C> &a) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d) stage[d * CAP + i] = pos[d * n + m];
+  for (int r = 0; r < ITEM; ++r) {
+    const int64_t m = I.begin + (r < (int)I.size ? r : 0);
+    a.x[r] = pos[m];
+    a.y[r] = pos[n + m];
+    a.z[r] = pos[2 * n + m];
 #pragma unroll
-      for (int c = 0; c < C; ++c) stage[(3 + c) * CAP + i] = val[c * n + m];
-    }
+    for (int q = 0; q < C; ++q) a.v[r][q] = val[q * n + m];
   }
 }
 
@@ -288,19 +392,42 @@ __device__ __forceinline__ void deposit_marker(double (&a)[C][S][S][S], double x
   }
 }
 
-// The deposit of one tile of TILE_X x TILE_Y x TILE_Z stencil cells, in parts of BLOCK
-// cells (x-slabs).  The bins are in tile order, so a part's markers are one contiguous run
-// of the binned arrays, staged in LDS with coalesced loads (up to CAP of them; the rest are
-// read from memory where they lie).  Each cell's markers are cut into items of about K_MIN
-// markers (more when the part holds many markers, so that at most ITEMS items exist), the
-// items are ordered by size, and each work-item runs over items: it sums an item's
+// The register sums back to zero (the bias) after a flush.
+template <int C, int S>
+__device__ __forceinline__ void rebias(double (&a)[C][S][S][S]) {
+#pragma unroll
+  for (int q = 0; q < C; ++q)
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+      for (int j = 0; j < S; ++j)
+#pragma unroll
+        for (int k = 0; k < S; ++k) a[q][i][j][k] = BIAS;
+}
+
+// The register sums of an item into the tile's LDS accumulator.  Stencil node (i, j, k) of
+// the cell whose footprint position is `base` sits at base + (i * LY + j) * LZ + k.
+template <int C, int S, int L, int LY, int LZ>
+__device__ __forceinline__ void flush_item(unsigned long long *acc, int base, const double (&a)[C][S][S][S]) {
+#pragma unroll
+  for (int q = 0; q < C; ++q)
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+      for (int j = 0; j < S; ++j)
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+          atomicAdd(&acc[q * L + base + (i * LY + j) * LZ + k], (unsigned long long)biased_to_int(a[q][i][j][k]));
+}
+
+// The deposit of one tile of TILE_X x TILE_Y x TILE_Z stencil cells.  The bins are in tile
+// order, so the tile's cells are consecutive bins; their offsets go to LDS and the markers
+// are cut into items (build_items).  A work-item runs over items: it sums an item's
 // contributions in exact float64 registers and adds them into the tile's int64 LDS
-// accumulator, so the lanes of a wavefront carry equal work whatever the cell counts.
-// The accumulator spans the tile's footprint before any wrap or clamp: stencil node
-// (i, j, k) of cell (lx, ly, lz) sits at ((lx + i) * LY + ly + j) * LZ + lz + k, a constant
-// offset from the cell.  Nodes beyond a bounded wall are folded onto the wall node after
-// the last part, and the tile is then added into the field once.
-template <class T, int KIND, int P, int C, int BLOCK, int CAP, int ITEMS, int K_MIN>
+// accumulator, which spans the tile's footprint before any wrap or clamp.  The markers of
+// the next item are loaded while the current one is deposited.  Nodes beyond a bounded wall
+// are folded onto the wall node, and the tile is then added into the field once.
+template <class T, int KIND, int P, int C, int BLOCK>
 __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ pos, const T *__restrict__ val,
                                                           int64_t n, const uint32_t *__restrict__ start, Grid g,
                                                           Tiling tl, double *__restrict__ field,
@@ -309,15 +436,11 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
   constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, TC = TILE_CELLS;
   constexpr int S = Sh::S, OFF = Sh::OFF;
   constexpr int LX = TX + S - 1, LY = TY + S - 1, LZ = TZ + S - 1, L = LX * LY * LZ;
-  constexpr int PC = BLOCK, PARTS = TC / PC;
-  constexpr int BUCKETS = 17;  // items by size: 16 or more, 15, ..., 1 (and an empty class)
-  static_assert(TC % PC == 0 && PC % (TY * TZ) == 0, "tile parts are x-slabs");
-  static_assert(ITEMS > PC && ITEMS < 4096 && PC <= 256, "item table");
+  constexpr int PARTS = TC / BLOCK;
   __shared__ unsigned long long acc[C * L];
-  __shared__ T stage[(3 + C) * CAP];
-  __shared__ uint32_t cell_first[PC + 1], cell_count[PC];
-  __shared__ uint32_t bucket[BUCKETS];
-  __shared__ uint32_t item[ITEMS];  // cell | index within the cell << 8 | items of the cell << 20
+  __shared__ uint32_t cfirst[TC + 1];
+  __shared__ uint16_t items[SWEEPS * TC];
+  __shared__ unsigned long long wave_sums[(BLOCK / 64) * PARTS];
 
   const int num0 = (int)g.num[0], num1 = (int)g.num[1], num2 = (int)g.num[2];
   const int kx = blockIdx.x % tl.tiles_x;
@@ -328,113 +451,81 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
   const int tz = tile_of_color(kz, tl.ntz, tl.cz, tl.color_z);
   const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
   const int64_t tile_key = (((int64_t)tx * tl.nty + ty) * tl.ntz + tz) * TC;
-
-  for (int i = threadIdx.x; i < C * L; i += BLOCK) acc[i] = 0ull;
-
-  const double scale = slot[0], inv = slot[1];
-  const uint32_t chunk = slot[2] < 4.0e9 ? (uint32_t)slot[2] : 4000000000u;
   const int t = threadIdx.x;
-  for (int part = 0; part < PARTS; ++part) {
-    // the part's cells: their offsets (bins of cells beyond a partial tile are empty)
-    {
-      const int64_t key = tile_key + part * PC + t;
-      const uint32_t first = start[key], next = start[key + 1];
-      cell_first[t] = first;
-      cell_count[t] = next - first;
-      if (t == PC - 1) cell_first[PC] = next;
-    }
-    if (t < BUCKETS) bucket[t] = 0u;
-    __syncthreads();
-    const uint32_t part_first = cell_first[0], total = cell_first[PC] - part_first;
-    const uint32_t staged = total < (uint32_t)CAP ? total : (uint32_t)CAP;
-    // item size: K_MIN, or more when the part holds many markers (at most ITEMS items)
-    const uint32_t per = total / (ITEMS - PC) + 1;
-    const uint32_t k_eff = per > (uint32_t)K_MIN ? per : (uint32_t)K_MIN;
-    const uint32_t my_count = cell_count[t];
-    const uint32_t my_items = (my_count + k_eff - 1) / k_eff;
-    const uint32_t my_base = my_items ? my_count / my_items : 0, my_rem = my_items ? my_count % my_items : 0;
-    // sizes my_base + 1 (my_rem of them) and my_base (the rest)
-    auto size_class = [](uint32_t s) { return BUCKETS - 1 - (s < BUCKETS - 1 ? s : BUCKETS - 1); };
-    if (my_rem) atomicAdd(&bucket[size_class(my_base + 1)], my_rem);
-    if (my_items > my_rem) atomicAdd(&bucket[size_class(my_base)], my_items - my_rem);
-    stage_markers<T, C, BLOCK, CAP>(pos, val, n, part_first, staged, stage);
-    __syncthreads();
-    if (t == 0) {  // exclusive scan of the size classes
-      uint32_t run = 0;
-      for (int b = 0; b < BUCKETS; ++b) {
-        const uint32_t k = bucket[b];
-        bucket[b] = run;
-        run += k;
-      }
-    }
-    __syncthreads();
-    const uint32_t nitems = bucket[BUCKETS - 1];  // items of size 1 and more come first
-    {
-      const uint32_t b1 = size_class(my_base + 1), b0 = size_class(my_base);
-      for (uint32_t j = 0; j < my_items; ++j) {
-        const uint32_t at = atomicAdd(&bucket[j < my_rem ? b1 : b0], 1u);
-        item[at] = (uint32_t)t | (j << 8) | (my_items << 20);
-      }
-    }
-    __syncthreads();
 
-    // the items, largest first; consecutive lanes take consecutive items
-    for (uint32_t it = t; it < nitems; it += BLOCK) {
-      const uint32_t d = item[it];
-      const int c = d & 0xFF;
-      const uint32_t j = (d >> 8) & 0xFFF, items = d >> 20;
-      const uint32_t count = cell_count[c];
-      const uint32_t base_n = count / items, rem = count % items;
-      const uint32_t begin = j * base_n + (j < rem ? j : rem), size = base_n + (j < rem ? 1u : 0u);
-      const int cl = part * PC + c;
-      const int base = ((cl / (TY * TZ)) * LY + (cl / TZ) % TY) * LZ + cl % TZ;
-      const int64_t first = (int64_t)cell_first[c] + begin;
-      const uint32_t j0 = cell_first[c] - part_first + begin;
-      auto fetch = [&](uint32_t r, double &x, double &y, double &z, double *v) {
-        const uint32_t jl = j0 + r;
-        if (jl < staged) {
-          x = (double)stage[jl];
-          y = (double)stage[CAP + jl];
-          z = (double)stage[2 * CAP + jl];
+  load_offsets<BLOCK, TC>(start + tile_key, cfirst);
+  for (int i = t; i < C * L; i += BLOCK) acc[i] = 0ull;
+  __syncthreads();
+  const uint32_t total = build_items<BLOCK, TC>(cfirst, items, wave_sums);
+
+  const double scale = slot[0];
+  // markers per register pass: the exact sums stay below 2^51 quanta
+  const uint32_t chunk = slot[2] < 4.0e9 ? (slot[2] >= 1.0 ? (uint32_t)slot[2] : 1u) : 4000000000u;
+
+  // the lookahead loads are unconditional (past the last item they repeat it) and the first
+  // ones are drained before the loop, so that no load of an earlier item is outstanding at
+  // the loop head and the wait before a deposit never covers the next item's loads
+  uint32_t it = t;
+  Item cur{0, 0, 0};
+  Ahead<T, C> now;
+  if (total > 0) {
+    cur = item_at(items, cfirst, it < total ? it : total - 1);
+    load_ahead<T, C>(pos, val, n, cur, now);
+    wait_loads();
+  }
+  while (it < total) {
+    const uint32_t nit = it + BLOCK;
+    const Item nxt = item_at(items, cfirst, nit < total ? nit : total - 1);
+    Ahead<T, C> next;
+    load_ahead<T, C>(pos, val, n, nxt, next);
+    const int c = cur.cell;
+    const int base = ((c / (TY * TZ)) * LY + (c / TZ) % TY) * LZ + c % TZ;
+    double a[C][S][S][S];
+    {
+      double v[C];
 #pragma unroll
-          for (int q = 0; q < C; ++q) v[q] = (double)stage[(3 + q) * CAP + jl];
-        } else {
-          const int64_t m = first + r;
-          x = (double)pos[m];
-          y = (double)pos[n + m];
-          z = (double)pos[2 * n + m];
+      for (int q = 0; q < C; ++q) v[q] = (double)now.v[0][q];
+      deposit_marker<KIND, P, C, S, true>(a, (double)now.x[0], (double)now.y[0], (double)now.z[0], v, g, num0, num1,
+                                          num2, scale);
+    }
+    uint32_t held = 1;  // markers in the registers
 #pragma unroll
-          for (int q = 0; q < C; ++q) v[q] = (double)val[q * n + m];
+    for (int r = 1; r < ITEM; ++r) {
+      if ((uint32_t)r < cur.size) {
+        if (held == chunk) {
+          flush_item<C, S, L, LY, LZ>(acc, base, a);
+          rebias<C, S>(a);
+          held = 0;
         }
-      };
-      double a[C][S][S][S];
-      for (uint32_t done = 0; done < size;) {
-        const uint32_t stop = size - done > chunk ? done + chunk : size;
-        {
-          double x, y, z, v[C];
-          fetch(done, x, y, z, v);
-          deposit_marker<KIND, P, C, S, true>(a, x, y, z, v, g, num0, num1, num2, scale);
-        }
-        for (uint32_t r = done + 1; r < stop; ++r) {
-          double x, y, z, v[C];
-          fetch(r, x, y, z, v);
-          deposit_marker<KIND, P, C, S, false>(a, x, y, z, v, g, num0, num1, num2, scale);
-        }
+        double v[C];
 #pragma unroll
-        for (int q = 0; q < C; ++q)
-#pragma unroll
-          for (int i = 0; i < S; ++i)
-#pragma unroll
-            for (int jj = 0; jj < S; ++jj)
-#pragma unroll
-              for (int k = 0; k < S; ++k)
-                atomicAdd(&acc[q * L + base + (i * LY + jj) * LZ + k],
-                          (unsigned long long)biased_to_int(a[q][i][jj][k]));
-        done = stop;
+        for (int q = 0; q < C; ++q) v[q] = (double)now.v[r][q];
+        deposit_marker<KIND, P, C, S, false>(a, (double)now.x[r], (double)now.y[r], (double)now.z[r], v, g, num0,
+                                             num1, num2, scale);
+        ++held;
       }
     }
-    __syncthreads();  // the staging and the tables are reused by the next part
+    // items of more than ITEM markers (cells of more than ITEM * SWEEPS markers)
+    for (uint32_t r = ITEM; r < cur.size; ++r) {
+      if (held == chunk) {
+        flush_item<C, S, L, LY, LZ>(acc, base, a);
+        rebias<C, S>(a);
+        held = 0;
+      }
+      const int64_t m = cur.begin + r;
+      double v[C];
+#pragma unroll
+      for (int q = 0; q < C; ++q) v[q] = (double)val[q * n + m];
+      deposit_marker<KIND, P, C, S, false>(a, (double)pos[m], (double)pos[n + m], (double)pos[2 * n + m], v, g, num0,
+                                           num1, num2, scale);
+      ++held;
+    }
+    flush_item<C, S, L, LY, LZ>(acc, base, a);
+    cur = nxt;
+    now = next;
+    it = nit;
   }
+  __syncthreads();
 
   // bounded axes: fold the footprint beyond a wall onto the wall node, axis by axis, so
   // that a corner reaches the wall nodes in two steps (the reference folds the stencil)
@@ -458,21 +549,39 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
     __syncthreads();
   }
 
-  // the tile into the field: plain read-modify-write, the colouring keeps tiles apart
-  for (int i = t; i < L; i += BLOCK) {
-    const int iz = i % LZ, iy = (i / LZ) % LY, ix = i / (LZ * LY);
-    int gx = x0 - OFF + ix, gy = y0 - OFF + iy, gz = z0 - OFF + iz;
-    if (g.code[0] == 1) gx = gx < 0 ? gx + num0 : (gx >= num0 ? gx - num0 : gx);
-    if (g.code[1] == 1) gy = gy < 0 ? gy + num1 : (gy >= num1 ? gy - num1 : gy);
-    if (g.code[2] == 1) gz = gz < 0 ? gz + num2 : (gz >= num2 ? gz - num2 : gz);
-    if (gx < 0 || gx >= num0 || gy < 0 || gy >= num1 || gz < 0 || gz >= num2) continue;
-    const int64_t at = (((int64_t)gx * num1 + gy) * num2 + gz) * C;
+  // the tile into the field, one (node, component) per work-item so that neighbouring lanes
+  // touch neighbouring words; every load of the work-item is issued before the first add,
+  // so the read-modify-writes cost one memory latency, not one per pass
+  const double inv = slot[1];
+  constexpr int E = (L * C + BLOCK - 1) / BLOCK;
+  long long sum[E];
+  int64_t at[E];
+  double old[E];
 #pragma unroll
-    for (int q = 0; q < C; ++q) {
+  for (int u = 0; u < E; ++u) {
+    const int e = t + u * BLOCK;
+    sum[u] = 0;
+    at[u] = -1;
+    if (e < L * C) {
+      const int i = C == 1 ? e : e / C, q = C == 1 ? 0 : e - (e / C) * C;
+      const int iz = i % LZ, iy = (i / LZ) % LY, ix = i / (LZ * LY);
+      int gx = x0 - OFF + ix, gy = y0 - OFF + iy, gz = z0 - OFF + iz;
+      if (g.code[0] == 1) gx = gx < 0 ? gx + num0 : (gx >= num0 ? gx - num0 : gx);
+      if (g.code[1] == 1) gy = gy < 0 ? gy + num1 : (gy >= num1 ? gy - num1 : gy);
+      if (g.code[2] == 1) gz = gz < 0 ? gz + num2 : (gz >= num2 ? gz - num2 : gz);
       const long long v = (long long)acc[q * L + i];
-      if (v != 0) field[at + q] += (double)v * inv;
+      if (v != 0 && gx >= 0 && gx < num0 && gy >= 0 && gy < num1 && gz >= 0 && gz < num2) {
+        sum[u] = v;
+        at[u] = (((int64_t)gx * num1 + gy) * num2 + gz) * C + q;
+      }
     }
   }
+#pragma unroll
+  for (int u = 0; u < E; ++u)
+    if (at[u] >= 0) old[u] = field[at[u]];
+#pragma unroll
+  for (int u = 0; u < E; ++u)
+    if (at[u] >= 0) field[at[u]] = old[u] + (double)sum[u] * inv;
 }
 
 // ---- tiled gather ---------------------------------------------------------------------
@@ -568,6 +677,229 @@ __global__ void __launch_bounds__(BLOCK) gather_tiled(const T *__restrict__ pos,
 #pragma unroll
       for (int c = 0; c < C; ++c) res[c * n + m] = (T)total_c[c];
     }
+  }
+}
+
+// The weights of rel on one axis in float32, from the float64 cell selection of anchor_axis
+// (the same cell for every position); returns nothing, the cell comes from the bins.
+template <int KIND, int P>
+__device__ __forceinline__ void weights_f32(double rel, int num, int code, float *w) {
+  const double nd = (double)num;
+  double r = rel;
+  if (KIND == LINEAR) {
+    int l;
+    if (code == 1) {
+      if (!(r >= 0.0 && r < nd)) r = r - nd * floor(r / nd);
+      const int i = (int)fmin(fmax(r, 0.0), nd);
+      l = i >= num ? num - 1 : i;
+    } else {
+      const int i = (int)fmin(fmax(r, 0.0), nd);
+      l = i > num - 2 ? num - 2 : i;
+    }
+    const double d = r - (double)l;
+    const float f = (float)(d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d));
+    w[0] = 1.0f - f;
+    w[1] = f;
+    return;
+  }
+  if (code == 1) {
+    if (!(r >= 0.0 && r < nd)) r = r - nd * floor(r / nd);
+  } else {
+    if (r < 0.0) r = 0.0;
+    if (r > nd - 1.0) r = nd - 1.0;
+  }
+  const double rc = fmin(fmax(r, 0.0), nd);
+  const int middle = (P % 2 == 0) ? (int)(rc + 0.5) : (int)rc;
+  const float o = (float)(r - (double)middle);
+  if (P == 0) {
+    w[0] = 1.0f;
+  } else if (P == 1) {
+    w[0] = 1.0f - o;
+    w[1] = o;
+  } else if (P == 2) {
+    w[0] = 0.5f * (0.5f - o) * (0.5f - o);
+    w[1] = 0.75f - o * o;
+    w[2] = 0.5f * (0.5f + o) * (0.5f + o);
+  } else {
+    const float cm = 1.0f - o, sq = o * o, cu = sq * o;
+    w[0] = cm * cm * cm * (1.0f / 6.0f);
+    w[1] = (3.0f * cu - 6.0f * sq + 4.0f) * (1.0f / 6.0f);
+    w[2] = (-3.0f * cu + 3.0f * sq + 3.0f * o + 1.0f) * (1.0f / 6.0f);
+    w[3] = cu * (1.0f / 6.0f);
+  }
+}
+
+template <int C>
+struct NodeSlot {
+  using type = float4;  // three components and a pad: one 16-byte LDS read per node
+};
+template <>
+struct NodeSlot<1> {
+  using type = float;
+};
+
+template <int C>
+__device__ __forceinline__ typename NodeSlot<C>::type make_slot(const double *f) {
+  if constexpr (C == 1)
+    return (float)f[0];
+  else
+    return make_float4((float)f[0], (float)f[1], (float)f[2], 0.0f);
+}
+
+// Interpolation at one marker from the (S^3) node slots of its cell, in float32.
+template <int KIND, int P, int C, int S>
+__device__ __forceinline__ void interpolate_marker(const typename NodeSlot<C>::type (&nd)[S][S][S], float x, float y,
+                                                   float z, const Grid &g, int num0, int num1, int num2,
+                                                   float *out) {
+  float wx[4], wy[4], wz[4];
+  weights_f32<KIND, P>(((double)x - g.lower[0]) * g.inverse[0], num0, g.code[0], wx);
+  weights_f32<KIND, P>(((double)y - g.lower[1]) * g.inverse[1], num1, g.code[1], wy);
+  weights_f32<KIND, P>(((double)z - g.lower[2]) * g.inverse[2], num2, g.code[2], wz);
+  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+  for (int a = 0; a < S; ++a)
+#pragma unroll
+    for (int b = 0; b < S; ++b) {
+      const float wab = wx[a] * wy[b];
+#pragma unroll
+      for (int e = 0; e < S; ++e) {
+        const float w = wab * wz[e];
+        if constexpr (C == 1) {
+          s0 = fmaf(w, nd[a][b][e], s0);
+        } else {
+          s0 = fmaf(w, nd[a][b][e].x, s0);
+          s1 = fmaf(w, nd[a][b][e].y, s1);
+          s2 = fmaf(w, nd[a][b][e].z, s2);
+        }
+      }
+    }
+  out[0] = s0;
+  if constexpr (C == 3) {
+    out[1] = s1;
+    out[2] = s2;
+  }
+}
+
+template <int C>
+struct PosAhead {
+  float x[ITEM], y[ITEM], z[ITEM];
+};
+
+__device__ __forceinline__ void load_pos_ahead(const float *__restrict__ pos, int64_t n, const Item &I,
+                                               PosAhead<1> &a) {
+#pragma unroll
+  for (int r = 0; r < ITEM; ++r) {
+    const int64_t m = I.begin + (r < (int)I.size ? r : 0);
+    a.x[r] = pos[m];
+    a.y[r] = pos[n + m];
+    a.z[r] = pos[2 * n + m];
+  }
+}
+
+// Tiled gather of float32 markers.  The tile's node footprint is staged in LDS in float32
+// (each position holding its node after the periodic wrap or the wall clamp), the weights
+// are float32 from the float64 cell selection, and the sums are float32: the result is
+// float32 either way.  A work-item runs over items (build_items): it reads its cell's S^3
+// node slots from LDS into registers once and interpolates the item's markers, whose
+// positions were loaded while the previous item was computed.  Tiles are numbered so that
+// each XCD (workgroups go round-robin over the eight) takes a contiguous range of them.
+template <int KIND, int P, int C, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) gather_items(const float *__restrict__ pos, int64_t n,
+                                                      const uint32_t *__restrict__ start, Grid g, int ntx, int nty,
+                                                      int ntz, const double *__restrict__ field,
+                                                      float *__restrict__ res) {
+  using Sh = StencilShape<KIND, P>;
+  using Slot = typename NodeSlot<C>::type;
+  constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, TC = TILE_CELLS;
+  constexpr int S = Sh::S, OFF = Sh::OFF;
+  constexpr int LX = TX + S - 1, LY = TY + S - 1, LZ = TZ + S - 1, L = LX * LY * LZ;
+  constexpr int PARTS = TC / BLOCK;
+  static_assert(C == 1 || C == 3, "float32 gather tiles hold 1 or 3 components");
+  __shared__ Slot tile[L];
+  __shared__ uint32_t cfirst[TC + 1];
+  __shared__ uint16_t items[SWEEPS * TC];
+  __shared__ unsigned long long wave_sums[(BLOCK / 64) * PARTS];
+
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int xcd = b % 8, per = nb / 8, extra = nb % 8;
+  const int id = xcd * per + (xcd < extra ? xcd : extra) + b / 8;  // the tile, in tile-key order
+  const int tz = id % ntz, ty = (id / ntz) % nty, tx = id / (ntz * nty);
+  const int num0 = (int)g.num[0], num1 = (int)g.num[1], num2 = (int)g.num[2];
+  const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
+  const int t = threadIdx.x;
+  const int64_t tile_key = (int64_t)id * TC;
+  load_offsets<BLOCK, TC>(start + tile_key, cfirst);
+  {
+    // the footprint, one (node, component) per work-item and pass; every load of the
+    // work-item is issued before the first one is used
+    constexpr int E = (L * C + BLOCK - 1) / BLOCK, CV = C == 1 ? 1 : 4;
+    double v[E];
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int e = t + u * BLOCK < L * C ? t + u * BLOCK : L * C - 1;  // unconditional loads
+      const int i = e / C, q = e - (e / C) * C;
+      const int iz = i % LZ, iy = (i / LZ) % LY, ix = i / (LZ * LY);
+      const int64_t gx = wrap_index(x0 - OFF + ix, num0, g.code[0]);
+      const int64_t gy = wrap_index(y0 - OFF + iy, num1, g.code[1]);
+      const int64_t gz = wrap_index(z0 - OFF + iz, num2, g.code[2]);
+      v[u] = field[((gx * num1 + gy) * num2 + gz) * C + q];
+    }
+    float *flat = reinterpret_cast<float *>(tile);
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int e = t + u * BLOCK;
+      if (e < L * C) {
+        const int i = e / C, q = e - (e / C) * C;
+        flat[i * CV + q] = (float)v[u];
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t total = build_items<BLOCK, TC>(cfirst, items, wave_sums);
+
+  // lookahead as in scatter_tiled: unconditional loads, the first ones drained before the loop
+  uint32_t it = t;
+  Item cur{0, 0, 0};
+  PosAhead<1> now;
+  if (total > 0) {
+    cur = item_at(items, cfirst, it < total ? it : total - 1);
+    load_pos_ahead(pos, n, cur, now);
+    wait_loads();
+  }
+  while (it < total) {
+    const uint32_t nit = it + BLOCK;
+    const Item nxt = item_at(items, cfirst, nit < total ? nit : total - 1);
+    PosAhead<1> next;
+    load_pos_ahead(pos, n, nxt, next);
+    const int c = cur.cell;
+    const int base = ((c / (TY * TZ)) * LY + (c / TZ) % TY) * LZ + c % TZ;
+    Slot nd[S][S][S];
+#pragma unroll
+    for (int a = 0; a < S; ++a)
+#pragma unroll
+      for (int b2 = 0; b2 < S; ++b2)
+#pragma unroll
+        for (int e = 0; e < S; ++e) nd[a][b2][e] = tile[base + (a * LY + b2) * LZ + e];
+#pragma unroll
+    for (int r = 0; r < ITEM; ++r) {
+      if ((uint32_t)r < cur.size) {
+        float out[C];
+        interpolate_marker<KIND, P, C, S>(nd, now.x[r], now.y[r], now.z[r], g, num0, num1, num2, out);
+        const int64_t m = cur.begin + r;
+#pragma unroll
+        for (int q = 0; q < C; ++q) res[q * n + m] = out[q];
+      }
+    }
+    for (uint32_t r = ITEM; r < cur.size; ++r) {
+      const int64_t m = cur.begin + r;
+      float out[C];
+      interpolate_marker<KIND, P, C, S>(nd, pos[m], pos[n + m], pos[2 * n + m], g, num0, num1, num2, out);
+#pragma unroll
+      for (int q = 0; q < C; ++q) res[q * n + m] = out[q];
+    }
+    cur = nxt;
+    now = next;
+    it = nit;
   }
 }
 
@@ -773,7 +1105,7 @@ extern "C" int mpf_bin_markers(int kind, int degree, int ndim, const void *posit
 template <class T, int KIND, int P, int C>
 static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const uint32_t *start, const Grid &g,
                              double *field, const double *scalars, hipStream_t stream) {
-  constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, BLOCK = 256, CAP = 1152, ITEMS = 768, K_MIN = 3;
+  constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, BLOCK = 256;
   Tiling tl;
   tl.tx = TX;
   tl.ty = TY;
@@ -801,7 +1133,7 @@ static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const 
         const int blocks = tl.tiles_x * tl.tiles_y * tl.tiles_z;
         if (blocks == 0) continue;
         ProfileScope prof("scatter_tiled", stream);
-        hipLaunchKernelGGL((scatter_tiled<T, KIND, P, C, BLOCK, CAP, ITEMS, K_MIN>), dim3(blocks),
+        hipLaunchKernelGGL((scatter_tiled<T, KIND, P, C, BLOCK>), dim3(blocks),
                            dim3(BLOCK), 0, stream,
                            (const T *)pos, (const T *)val, n, start, g, tl, field, scalars);
         MPF_LAUNCH_CHECK();
@@ -817,8 +1149,12 @@ static int run_gather_tiled(const void *pos, int64_t n, const uint32_t *start, c
             ntz = (int)((g.num[2] + TZ - 1) / TZ);
   {
     ProfileScope prof("gather_tiled", stream);
-    hipLaunchKernelGGL((gather_tiled<T, KIND, P, C, BLOCK>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
-                       (const T *)pos, n, start, g, ntx, nty, ntz, field, (T *)res);
+    if constexpr (sizeof(T) == 4)
+      hipLaunchKernelGGL((gather_items<KIND, P, C, BLOCK>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
+                         (const float *)pos, n, start, g, ntx, nty, ntz, field, (float *)res);
+    else
+      hipLaunchKernelGGL((gather_tiled<T, KIND, P, C, BLOCK>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
+                         (const T *)pos, n, start, g, ntx, nty, ntz, field, (T *)res);
   }
   MPF_LAUNCH_CHECK();
   const int64_t cells = (int64_t)ntx * nty * ntz * TILE_CELLS;  // the dead bin follows the tiles' bins
