@@ -462,20 +462,23 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
   // markers per register pass: the exact sums stay below 2^51 quanta
   const uint32_t chunk = slot[2] < 4.0e9 ? (slot[2] >= 1.0 ? (uint32_t)slot[2] : 1u) : 4000000000u;
 
-  // the lookahead loads are unconditional (past the last item they repeat it) and the first
-  // ones are drained before the loop, so that no load of an earlier item is outstanding at
-  // the loop head and the wait before a deposit never covers the next item's loads
+  // The lookahead: the markers of the next item load while the current one is deposited.
+  // The loads are unconditional (past the last item they repeat it) and the first ones are
+  // drained before the loop, so the wait before a deposit never covers the next item's
+  // loads.  Item descriptors are read from LDS one item further ahead, between a deposit
+  // and its flush: LDS operations complete in order, so a descriptor read issued after a
+  // flush would wait for its 81 atomics.
+  const uint32_t last = total > 0 ? total - 1 : 0;
   uint32_t it = t;
-  Item cur{0, 0, 0};
+  Item cur{0, 0, 0}, nxt{0, 0, 0};
   Ahead<T, C> now;
   if (total > 0) {
-    cur = item_at(items, cfirst, it < total ? it : total - 1);
+    cur = item_at(items, cfirst, it < total ? it : last);
     load_ahead<T, C>(pos, val, n, cur, now);
+    nxt = item_at(items, cfirst, it + BLOCK < total ? it + BLOCK : last);
     wait_loads();
   }
   while (it < total) {
-    const uint32_t nit = it + BLOCK;
-    const Item nxt = item_at(items, cfirst, nit < total ? nit : total - 1);
     Ahead<T, C> next;
     load_ahead<T, C>(pos, val, n, nxt, next);
     const int c = cur.cell;
@@ -520,10 +523,13 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
                                            num1, num2, scale);
       ++held;
     }
+    const uint32_t after = it + 2 * BLOCK;
+    const Item nn = item_at(items, cfirst, after < total ? after : last);
     flush_item<C, S, L, LY, LZ>(acc, base, a);
     cur = nxt;
+    nxt = nn;
     now = next;
-    it = nit;
+    it += BLOCK;
   }
   __syncthreads();
 
