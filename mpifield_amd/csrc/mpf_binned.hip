@@ -420,6 +420,32 @@ __device__ __forceinline__ void flush_item(unsigned long long *acc, int base, co
           atomicAdd(&acc[q * L + base + (i * LY + j) * LZ + k], (unsigned long long)biased_to_int(a[q][i][j][k]));
 }
 
+// The field index (times C) of node (gx, gy, z0 - OFF) for each footprint row (ix, iy) of a
+// tile, wrapped on periodic axes; -1 for a row beyond a bounded wall (clamp = false) or the
+// clamped row (clamp = true, the gather's view).  A row is LZ nodes along z.
+template <int LX, int LY, int OFF, int C, int BLOCK>
+__device__ __forceinline__ void footprint_rows(const Grid &g, int x0, int y0, int z0, bool clamp, long long *rows) {
+  const int num0 = (int)g.num[0], num1 = (int)g.num[1], num2 = (int)g.num[2];
+  for (int r = threadIdx.x; r < LX * LY; r += BLOCK) {
+    int gx = x0 - OFF + r / LY, gy = y0 - OFF + r % LY;
+    bool ok = true;
+    if (g.code[0] == 1) gx = gx < 0 ? gx + num0 : (gx >= num0 ? gx - num0 : gx);
+    else if (gx < 0 || gx >= num0) { ok = clamp; gx = gx < 0 ? 0 : num0 - 1; }
+    if (g.code[1] == 1) gy = gy < 0 ? gy + num1 : (gy >= num1 ? gy - num1 : gy);
+    else if (gy < 0 || gy >= num1) { ok = ok && clamp; gy = gy < 0 ? 0 : num1 - 1; }
+    rows[r] = ok ? ((long long)gx * num1 + gy) * num2 * C : -1ll;
+  }
+}
+
+// The z node of footprint position iz of a tile, wrapped on a periodic axis; -1 beyond a
+// bounded wall (clamp = false) or the clamped node (clamp = true).
+__device__ __forceinline__ int footprint_z(const Grid &g, int gz, bool clamp) {
+  const int num2 = (int)g.num[2];
+  if (g.code[2] == 1) return gz < 0 ? gz + num2 : (gz >= num2 ? gz - num2 : gz);
+  if (gz >= 0 && gz < num2) return gz;
+  return clamp ? (gz < 0 ? 0 : num2 - 1) : -1;
+}
+
 // The deposit of one tile of TILE_X x TILE_Y x TILE_Z stencil cells.  The bins are in tile
 // order, so the tile's cells are consecutive bins; their offsets go to LDS and the markers
 // are cut into items (build_items).  A work-item runs over items: it sums an item's
@@ -441,6 +467,7 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
   __shared__ uint32_t cfirst[TC + 1];
   __shared__ uint16_t items[SWEEPS * TC];
   __shared__ unsigned long long wave_sums[(BLOCK / 64) * PARTS];
+  __shared__ long long rows[LX * LY];
 
   const int num0 = (int)g.num[0], num1 = (int)g.num[1], num2 = (int)g.num[2];
   const int kx = blockIdx.x % tl.tiles_x;
@@ -455,6 +482,7 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
 
   load_offsets<BLOCK, TC>(start + tile_key, cfirst);
   for (int i = t; i < C * L; i += BLOCK) acc[i] = 0ull;
+  footprint_rows<LX, LY, OFF, C, BLOCK>(g, x0, y0, z0, false, rows);
   __syncthreads();
   const uint32_t total = build_items<BLOCK, TC>(cfirst, items, wave_sums);
 
@@ -556,12 +584,13 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
   }
 
   // the tile into the field, one (node, component) per work-item so that neighbouring lanes
-  // touch neighbouring words; every load of the work-item is issued before the first add,
-  // so the read-modify-writes cost one memory latency, not one per pass
+  // touch neighbouring words (a footprint row is LZ * C consecutive words of the field, its
+  // start in the row table); every load of the work-item is issued before the first add, so
+  // the read-modify-writes cost one memory latency, not one per pass
   const double inv = slot[1];
-  constexpr int E = (L * C + BLOCK - 1) / BLOCK;
-  long long sum[E];
-  int64_t at[E];
+  const bool plain_z = g.code[2] != 2 && z0 - OFF >= 0 && z0 - OFF + LZ <= num2;  // rows neither wrap nor clamp
+  constexpr int E = (L * C + BLOCK - 1) / BLOCK, ROW = LZ * C;
+  long long sum[E], at[E];
   double old[E];
 #pragma unroll
   for (int u = 0; u < E; ++u) {
@@ -569,16 +598,13 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
     sum[u] = 0;
     at[u] = -1;
     if (e < L * C) {
-      const int i = C == 1 ? e : e / C, q = C == 1 ? 0 : e - (e / C) * C;
-      const int iz = i % LZ, iy = (i / LZ) % LY, ix = i / (LZ * LY);
-      int gx = x0 - OFF + ix, gy = y0 - OFF + iy, gz = z0 - OFF + iz;
-      if (g.code[0] == 1) gx = gx < 0 ? gx + num0 : (gx >= num0 ? gx - num0 : gx);
-      if (g.code[1] == 1) gy = gy < 0 ? gy + num1 : (gy >= num1 ? gy - num1 : gy);
-      if (g.code[2] == 1) gz = gz < 0 ? gz + num2 : (gz >= num2 ? gz - num2 : gz);
-      const long long v = (long long)acc[q * L + i];
-      if (v != 0 && gx >= 0 && gx < num0 && gy >= 0 && gy < num1 && gz >= 0 && gz < num2) {
+      const int r = e / ROW, k = e - r * ROW, iz = k / C, q = k - iz * C;
+      const long long rb = rows[r];
+      const int gz = plain_z ? z0 - OFF + iz : footprint_z(g, z0 - OFF + iz, false);
+      const long long v = (long long)acc[q * L + r * LZ + iz];
+      if (rb >= 0 && gz >= 0 && v != 0) {
         sum[u] = v;
-        at[u] = (((int64_t)gx * num1 + gy) * num2 + gz) * C + q;
+        at[u] = rb + (long long)gz * C + q;
       }
     }
   }
@@ -825,6 +851,7 @@ __global__ void __launch_bounds__(BLOCK) gather_items(const float *__restrict__ 
   __shared__ uint32_t cfirst[TC + 1];
   __shared__ uint16_t items[SWEEPS * TC];
   __shared__ unsigned long long wave_sums[(BLOCK / 64) * PARTS];
+  __shared__ long long rows[LX * LY];
 
   const int nb = gridDim.x, b = blockIdx.x;
   const int xcd = b % 8, per = nb / 8, extra = nb % 8;
@@ -835,20 +862,21 @@ __global__ void __launch_bounds__(BLOCK) gather_items(const float *__restrict__ 
   const int t = threadIdx.x;
   const int64_t tile_key = (int64_t)id * TC;
   load_offsets<BLOCK, TC>(start + tile_key, cfirst);
+  footprint_rows<LX, LY, OFF, C, BLOCK>(g, x0, y0, z0, true, rows);
+  __syncthreads();
   {
-    // the footprint, one (node, component) per work-item and pass; every load of the
-    // work-item is issued before the first one is used
-    constexpr int E = (L * C + BLOCK - 1) / BLOCK, CV = C == 1 ? 1 : 4;
+    // the footprint, one (node, component) per work-item and pass (a row is LZ * C
+    // consecutive words of the field); every load of the work-item is issued before the
+    // first one is used
+    constexpr int E = (L * C + BLOCK - 1) / BLOCK, CV = C == 1 ? 1 : 4, ROW = LZ * C;
+    const bool plain_z = g.code[2] != 2 && z0 - OFF >= 0 && z0 - OFF + LZ <= num2;
     double v[E];
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       const int e = t + u * BLOCK < L * C ? t + u * BLOCK : L * C - 1;  // unconditional loads
-      const int i = e / C, q = e - (e / C) * C;
-      const int iz = i % LZ, iy = (i / LZ) % LY, ix = i / (LZ * LY);
-      const int64_t gx = wrap_index(x0 - OFF + ix, num0, g.code[0]);
-      const int64_t gy = wrap_index(y0 - OFF + iy, num1, g.code[1]);
-      const int64_t gz = wrap_index(z0 - OFF + iz, num2, g.code[2]);
-      v[u] = field[((gx * num1 + gy) * num2 + gz) * C + q];
+      const int r = e / ROW, k = e - r * ROW, iz = k / C, q = k - iz * C;
+      const int gz = plain_z ? z0 - OFF + iz : footprint_z(g, z0 - OFF + iz, true);
+      v[u] = field[rows[r] + (long long)gz * C + q];
     }
     float *flat = reinterpret_cast<float *>(tile);
 #pragma unroll
