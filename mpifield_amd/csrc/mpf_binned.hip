@@ -585,35 +585,20 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
 
   // the tile into the field, one (node, component) per work-item so that neighbouring lanes
   // touch neighbouring words (a footprint row is LZ * C consecutive words of the field, its
-  // start in the row table); every load of the work-item is issued before the first add, so
-  // the read-modify-writes cost one memory latency, not one per pass
+  // start in the row table).  Same-colour footprints never overlap, so every node receives
+  // at most one addition per launch and the colours run in a fixed order: the addition is
+  // a float64 atomic add performed in L2 (no wait for the old value), and the result is the
+  // same as a read-modify-write, bit for bit, whatever the timing.
   const double inv = slot[1];
   const bool plain_z = g.code[2] != 2 && z0 - OFF >= 0 && z0 - OFF + LZ <= num2;  // rows neither wrap nor clamp
-  constexpr int E = (L * C + BLOCK - 1) / BLOCK, ROW = LZ * C;
-  long long sum[E], at[E];
-  double old[E];
-#pragma unroll
-  for (int u = 0; u < E; ++u) {
-    const int e = t + u * BLOCK;
-    sum[u] = 0;
-    at[u] = -1;
-    if (e < L * C) {
-      const int r = e / ROW, k = e - r * ROW, iz = k / C, q = k - iz * C;
-      const long long rb = rows[r];
-      const int gz = plain_z ? z0 - OFF + iz : footprint_z(g, z0 - OFF + iz, false);
-      const long long v = (long long)acc[q * L + r * LZ + iz];
-      if (rb >= 0 && gz >= 0 && v != 0) {
-        sum[u] = v;
-        at[u] = rb + (long long)gz * C + q;
-      }
-    }
+  constexpr int ROW = LZ * C;
+  for (int e = t; e < L * C; e += BLOCK) {
+    const int r = e / ROW, k = e - r * ROW, iz = k / C, q = k - iz * C;
+    const long long rb = rows[r];
+    const int gz = plain_z ? z0 - OFF + iz : footprint_z(g, z0 - OFF + iz, false);
+    const long long v = (long long)acc[q * L + r * LZ + iz];
+    if (rb >= 0 && gz >= 0 && v != 0) unsafeAtomicAdd(field + rb + (long long)gz * C + q, (double)v * inv);
   }
-#pragma unroll
-  for (int u = 0; u < E; ++u)
-    if (at[u] >= 0) old[u] = field[at[u]];
-#pragma unroll
-  for (int u = 0; u < E; ++u)
-    if (at[u] >= 0) field[at[u]] = old[u] + (double)sum[u] * inv;
 }
 
 // ---- tiled gather ---------------------------------------------------------------------
