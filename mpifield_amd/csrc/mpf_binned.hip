@@ -846,15 +846,21 @@ __global__ void __launch_bounds__(BLOCK) gather_items(const float *__restrict__ 
   const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
   const int t = threadIdx.x;
   const int64_t tile_key = (int64_t)id * TC;
-  load_offsets<BLOCK, TC>(start + tile_key, cfirst);
   footprint_rows<LX, LY, OFF, C, BLOCK>(g, x0, y0, z0, true, rows);
   __syncthreads();
   {
-    // the footprint, one (node, component) per work-item and pass (a row is LZ * C
-    // consecutive words of the field); every load of the work-item is issued before the
-    // first one is used
+    // the cell offsets and the footprint (one (node, component) per work-item and pass; a
+    // row is LZ * C consecutive words of the field), every load of the work-item issued
+    // before the first one is used: one memory latency for both
+    constexpr int EO = (TC + 1 + BLOCK - 1) / BLOCK;
     constexpr int E = (L * C + BLOCK - 1) / BLOCK, CV = C == 1 ? 1 : 4, ROW = LZ * C;
     const bool plain_z = g.code[2] != 2 && z0 - OFF >= 0 && z0 - OFF + LZ <= num2;
+    uint32_t o[EO];
+#pragma unroll
+    for (int u = 0; u < EO; ++u) {
+      const int i = t + u * BLOCK;
+      o[u] = start[tile_key + (i <= TC ? i : TC)];
+    }
     double v[E];
 #pragma unroll
     for (int u = 0; u < E; ++u) {
@@ -862,6 +868,11 @@ __global__ void __launch_bounds__(BLOCK) gather_items(const float *__restrict__ 
       const int r = e / ROW, k = e - r * ROW, iz = k / C, q = k - iz * C;
       const int gz = plain_z ? z0 - OFF + iz : footprint_z(g, z0 - OFF + iz, true);
       v[u] = field[rows[r] + (long long)gz * C + q];
+    }
+#pragma unroll
+    for (int u = 0; u < EO; ++u) {
+      const int i = t + u * BLOCK;
+      if (i <= TC) cfirst[i] = o[u];
     }
     float *flat = reinterpret_cast<float *>(tile);
 #pragma unroll
