@@ -51,6 +51,10 @@ def parse() -> argparse.Namespace:
     p.add_argument("--only-headline", action="store_true", help="time the headline step only (for rocprofv3)")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only copy MIB MiB device-to-device (the counter calibration for rocprofv3 --pmc)")
+    p.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5),
+                   help="2: the headline (default); 3: 2D quadratic-edge path deposit, 1024^2, 2^25 paths; "
+                   "4: 4096 batched 2D patches, ~2e8 markers; 5: differentiable 3D cubic gather of scatter, "
+                   "128^3, 2^24 markers (one JSON line each)")
     return p.parse_args()
 
 
@@ -142,6 +146,151 @@ def time_steps(torch, step, steps: int, warmup: int, stream) -> tuple[float, flo
     return start.elapsed_time(stop) / steps, wall
 
 
+def time_events(torch, fn, steps: int, warmup: int) -> float:
+    """ms per call of fn by HIP events on the current stream."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        fn()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps
+
+
+def config3(torch, args) -> dict:
+    """2D charge-conserving path deposit (QUADRATIC_EDGES, spline degree 2) on a 1024^2 periodic
+    grid: 2^25 float32 paths, start uniform, per-axis displacement clip(N(0, h/4), +-h/2),
+    unit charges.  The deposit time and the discrete continuity G^T j = rho(end) - rho(start)
+    against the quadratic node charge, all on the GPU."""
+    from mpifield_amd import _lib
+    from mpifield_amd import kernels as K
+
+    nodes, n = 1024, 1 << 25
+    h = 1.0 / nodes
+    g = (0.0, 0.0, h, h, nodes, nodes, 1, 1)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    start = torch.rand((2, n), generator=gen, device="cuda", dtype=torch.float32)
+    d = (torch.randn((2, n), generator=gen, device="cuda") * (0.25 * h)).clamp(-0.5 * h, 0.5 * h)
+    end = (start.double() + d.double()).float()
+    q = torch.ones(n, device="cuda", dtype=torch.float32)
+    j = torch.zeros((nodes, nodes, 2), dtype=torch.float64, device="cuda")
+    out = {}
+    for label, mode in (("fixed", _lib.MODE_FIXED), ("atomic", _lib.MODE_ATOMIC)):
+        ms = time_events(torch, lambda: K.deposit_paths(1, 2, start, end, q, j, g, mode=mode), args.steps, 1)
+        out[label] = ms
+    j.zero_()
+    K.deposit_paths(1, 2, start, end, q, j, g, mode=_lib.MODE_FIXED)
+    rho0 = torch.zeros((nodes, nodes, 1), dtype=torch.float64, device="cuda")
+    rho1 = torch.zeros_like(rho0)
+    K.scatter(2, 2, start, q[None], rho0, g, mode=_lib.MODE_FIXED)
+    K.scatter(2, 2, end, q[None], rho1, g, mode=_lib.MODE_FIXED)
+    residual = (K.divergence(j, (1, 1)) - (rho1 - rho0)[..., 0]).abs().max().item()
+    alg = n * (16 + 4) + 2 * nodes * nodes * 2 * 8
+    return {
+        "metric": "paths/s, 2D QUADRATIC_EDGES path deposit 1024^2, 2^25 paths", "value": n / (out["fixed"] * 1e-3),
+        "unit": "paths/s", "n_gpus": 1, "steps": args.steps, "ms_per_step": out["fixed"], "higher_is_better": True,
+        "dtype": "int64", "data": "synthetic",
+        "config": {"workload": "2D spline-edge (degree 2) charge-conserving path deposit, 1024^2 periodic, "
+                   f"{n} float32 paths, displacements clip(N(0,h/4), h/2); one thread per path"},
+        "roofline": {"bound": "hbm", "achieved": alg / (out["fixed"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / (out["fixed"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "variants": {"fp64_atomic_ms": out["atomic"], "fp64_atomic_paths_per_s": n / (out["atomic"] * 1e-3)},
+        "continuity_max_abs_residual": residual, "total_charge": float(n),
+    }
+
+
+def config4(torch, args) -> dict:
+    """4096 2D patches of 16^2..128^2 nodes, spline degree 1..3 per patch, markers per patch
+    log-uniform in [1e3, 1e6] rescaled to about 2e8 in total, float32 positions uniform in the
+    patch, one component, bounded patches: one persistent deposit launch and one gather."""
+    import mpifield_amd as m
+    from mpifield_amd import patches as P
+
+    rng = np.random.default_rng(4)
+    count = 4096
+    sides = rng.integers(16, 129, count)
+    degrees = rng.integers(1, 4, count)
+    raw = np.exp(rng.uniform(np.log(1e3), np.log(1e6), count))
+    markers = np.clip(np.round(raw * 2e8 / raw.sum()), 1e3, 1e6).astype(np.int64)
+    weightings = {1: m.LINEAR, 2: m.QUADRATIC, 3: m.CUBIC}
+    batch = P.PatchBatch(ndim=2, ncomp=1)
+    lows, cells = [], []
+    for i in range(count):
+        cell = 1.0 / float(sides[i] - 1)
+        lower = (float(i % 64), float(i // 64))
+        batch.add(lower, (cell, cell), (int(sides[i]), int(sides[i])), weightings[int(degrees[i])],
+                          int(markers[i]))
+        lows.append(lower)
+        cells.append(cell)
+    total = int(markers.sum())
+    batch.build()
+    # positions uniform in each patch, made on the GPU
+    dev = torch.device("cuda")
+    patch_of = torch.repeat_interleave(torch.arange(count, device=dev), torch.as_tensor(markers, device=dev))
+    lo = torch.as_tensor(np.array(lows), device=dev, dtype=torch.float64)[patch_of].T
+    span = torch.as_tensor((sides - 1) * np.array(cells), device=dev, dtype=torch.float64)[patch_of]
+    gen = torch.Generator(device="cuda").manual_seed(4)
+    pos = (lo + torch.rand((2, total), generator=gen, device=dev, dtype=torch.float64) * span).float()
+    val = torch.ones((1, total), device=dev, dtype=torch.float32)
+    fields = torch.zeros(batch.field_values, dtype=torch.float64, device=dev)
+    result = torch.empty((1, total), dtype=torch.float32, device=dev)
+    sc = time_events(torch, lambda: batch.scatter(pos, val, fields), max(1, args.steps // 4), 1)
+    ga = time_events(torch, lambda: batch.gather(pos, fields, result), max(1, args.steps // 4), 1)
+    alg = total * 12 * 2 + int((3 * sides.astype(np.int64) ** 2 * 8).sum())
+    return {
+        "metric": "markers/s, 4096 batched 2D patches scatter+gather, ~2e8 markers", "value": total / ((sc + ga) * 1e-3),
+        "unit": "markers/s", "n_gpus": 1, "ms_per_step": sc + ga, "higher_is_better": True, "dtype": "int64",
+        "data": "synthetic",
+        "config": {"workload": f"4096 2D patches 16^2-128^2 nodes, spline degree 1-3, {total} float32 markers "
+                   "(log-uniform per patch, rescaled to ~2e8), one component, one persistent launch each way"},
+        "roofline": {"bound": "hbm", "achieved": alg / ((sc + ga) * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / ((sc + ga) * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "scatter_ms": sc, "gather_ms": ga, "markers": total,
+    }
+
+
+def config5(torch, args) -> dict:
+    """The differentiable 3D cubic gather of scatter on a 128^3 periodic grid: 2^24 float32
+    markers, field = sum of 8 Gaussians; one forward (scatter, gather) and backward (gradients
+    with respect to positions, weights and field)."""
+    import mpifield_amd as m
+    from mpifield_amd import autograd as AG
+
+    nodes, n = 128, 1 << 24
+    cell = 1.0 / nodes
+    grid = m.BlockGrid.from_geometry([0.0] * 3, [cell] * 3, [nodes] * 3, [True] * 3)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    pos = torch.rand((3, n), generator=gen, device="cuda", dtype=torch.float32).requires_grad_(True)
+    w = torch.ones((1, n), device="cuda", dtype=torch.float32).requires_grad_(True)
+    axis = (torch.arange(nodes, device="cuda", dtype=torch.float64) + 0.0) * cell
+    X, Y, Z = torch.meshgrid(axis, axis, axis, indexing="ij")
+    field = torch.zeros((nodes, nodes, nodes, 1), dtype=torch.float64, device="cuda")
+    for _ in range(8):
+        c = torch.rand(3, generator=gen, device="cuda", dtype=torch.float64)
+        sigma = 0.05 + 0.15 * torch.rand((), generator=gen, device="cuda", dtype=torch.float64)
+        amp = torch.randn((), generator=gen, device="cuda", dtype=torch.float64)
+        field[..., 0] += amp * torch.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) / (2 * sigma**2))
+    field.requires_grad_(True)
+
+    def step():
+        pos.grad = w.grad = field.grad = None
+        rho = AG.scatter(pos, w, m.CUBIC, grid)
+        g = AG.gather(pos, rho + field, m.CUBIC, grid)
+        g.double().sum().backward()
+
+    ms = time_events(torch, step, max(1, args.steps // 4), 1)
+    return {
+        "metric": "markers/s, differentiable 3D cubic gather(scatter) 128^3, 2^24 markers, forward+backward",
+        "value": n / (ms * 1e-3), "unit": "markers/s", "n_gpus": 1, "ms_per_step": ms, "higher_is_better": True,
+        "dtype": "int64", "data": "synthetic",
+        "config": {"workload": f"128^3 periodic, {n} float32 markers, CUBIC, field = 8 Gaussians; scatter, gather, "
+                   "backward to positions, weights and field"},
+    }
+
+
 def main() -> None:
     args = parse()
     if args.gpus != 1:
@@ -157,6 +306,11 @@ def main() -> None:
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_each_way": src.numel() * 8, "copies": 3}), flush=True)
+        return
+
+    if args.config != 2:
+        line = {3: config3, 4: config4, 5: config5}[args.config](torch, args)
+        print(json.dumps(line), flush=True)
         return
 
     from mpifield_amd import _lib, binning
