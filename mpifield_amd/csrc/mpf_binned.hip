@@ -116,65 +116,314 @@ __host__ __device__ __forceinline__ int64_t bins_of(int ndim, const int64_t *num
   return c;
 }
 
+// The bin of one marker: its anchor cell's key, or dead_key when every value is zero (a
+// gather's markers, ncomp == 0, are all live).  Folds |value| into vmax for the scale.
 template <class T, int KIND, int P, int NDIM>
-__global__ void __launch_bounds__(256) bin_count_kernel(const T *__restrict__ pos, int64_t ps,
-                                                        const T *__restrict__ val, int64_t vs, int64_t n, int ncomp,
-                                                        Grid g, int64_t dead_key, uint32_t *__restrict__ key,
-                                                        uint32_t *__restrict__ rank, uint32_t *__restrict__ count,
-                                                        unsigned long long *__restrict__ maxbits) {
-  double vmax = 0.0;
-  for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x) {
-    bool live = ncomp == 0;  // a gather's markers carry no values and are all live
-    for (int c = 0; c < ncomp; ++c) {
-      const double v = fabs((double)val[c * vs + m]);
-      live |= v != 0.0;
-      if (v > vmax && v < 1.0e300) vmax = v;
-    }
-    int64_t k = dead_key;
-    if (live) {
-      int cell[NDIM];
-#pragma unroll
-      for (int d = 0; d < NDIM; ++d) {
-        double w[4];
-        const double rel = ((double)pos[d * ps + m] - g.lower[d]) * g.inverse[d];
-        const int num = (int)g.num[d];
-        int a = anchor_axis<KIND, P>(rel, num, g.code[d], w);
-        if (g.code[d] == 1 && a >= num) a -= num;
-        cell[d] = a < 0 ? 0 : (a > num - 1 ? num - 1 : a);  // a key in range whatever the position
-      }
-      if constexpr (NDIM == 3) {
-        const int64_t nty = (g.num[1] + TILE_Y - 1) / TILE_Y, ntz = (g.num[2] + TILE_Z - 1) / TILE_Z;
-        const int64_t tile = ((int64_t)(cell[0] / TILE_X) * nty + cell[1] / TILE_Y) * ntz + cell[2] / TILE_Z;
-        k = tile * TILE_CELLS + ((cell[0] % TILE_X) * TILE_Y + cell[1] % TILE_Y) * TILE_Z + cell[2] % TILE_Z;
-      } else {
-        k = 0;
-#pragma unroll
-        for (int d = 0; d < NDIM; ++d) k = k * g.num[d] + cell[d];
-      }
-    }
-    key[m] = (uint32_t)k;
-    rank[m] = atomicAdd(count + k, 1u);
+__device__ __forceinline__ uint32_t marker_key(const T *__restrict__ pos, int64_t ps, const T *__restrict__ val,
+                                               int64_t vs, int ncomp, int64_t m, const Grid &g, uint32_t dead_key,
+                                               double &vmax) {
+  bool live = ncomp == 0;
+  for (int c = 0; c < ncomp; ++c) {
+    const double v = fabs((double)val[c * vs + m]);
+    live |= v != 0.0;
+    if (v > vmax && v < 1.0e300) vmax = v;
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    double o = __shfl_xor(vmax, off, 64);
-    vmax = o > vmax ? o : vmax;
+  if (!live) return dead_key;
+  int cell[NDIM];
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    double w[4];
+    const double rel = ((double)pos[d * ps + m] - g.lower[d]) * g.inverse[d];
+    const int num = (int)g.num[d];
+    int a = anchor_axis<KIND, P>(rel, num, g.code[d], w);
+    if (g.code[d] == 1 && a >= num) a -= num;
+    cell[d] = a < 0 ? 0 : (a > num - 1 ? num - 1 : a);  // a key in range whatever the position
   }
-  if ((threadIdx.x & 63) == 0 && vmax > 0.0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(vmax));
+  if constexpr (NDIM == 3) {
+    const int64_t nty = (g.num[1] + TILE_Y - 1) / TILE_Y, ntz = (g.num[2] + TILE_Z - 1) / TILE_Z;
+    const int64_t tile = ((int64_t)(cell[0] / TILE_X) * nty + cell[1] / TILE_Y) * ntz + cell[2] / TILE_Z;
+    return (uint32_t)(tile * TILE_CELLS + ((cell[0] % TILE_X) * TILE_Y + cell[1] % TILE_Y) * TILE_Z + cell[2] % TILE_Z);
+  } else {
+    int64_t k = 0;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) k = k * g.num[d] + cell[d];
+    return (uint32_t)k;
+  }
 }
 
-template <class T>
-__global__ void __launch_bounds__(256) bin_place_kernel(const T *__restrict__ pos, int64_t ps, int ndim,
-                                                        const T *__restrict__ val, int64_t vs, int ncomp, int64_t n,
-                                                        const uint32_t *__restrict__ key,
-                                                        const uint32_t *__restrict__ rank,
-                                                        const uint32_t *__restrict__ start, T *__restrict__ spos,
-                                                        T *__restrict__ sval, int64_t *__restrict__ order) {
-  for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t dst = (int64_t)start[key[m]] + rank[m];
-    for (int d = 0; d < ndim; ++d) spos[d * n + dst] = pos[d * ps + m];
-    for (int c = 0; c < ncomp; ++c) sval[c * n + dst] = val[c * vs + m];
-    order[dst] = m;
+// The counting sort runs in three passes without a global atomic per marker.  Keys are
+// split into groups of 2^shift consecutive keys (shift chosen so that a group holds a few
+// thousand markers on average).  Pass 1: each of B workgroups takes one contiguous chunk
+// of markers, writes their keys and counts its markers per group in LDS; the counts land
+// in hist[group][block].  An exclusive scan of hist (group-major) gives every (group,
+// block) pair its slot range.  Pass 2: each workgroup walks its chunk again, ranks its
+// markers within their group with LDS atomics and writes each marker as one packed record
+// (key, index, positions, values) at its slot, so that a marker costs one scattered line
+// write rather than one per array.  Pass 3: one workgroup per group holds the group's
+// records in registers, counts them per key in LDS, scans, writes the bin offsets of its
+// keys, ranks every marker within its bin and writes the sorted arrays one row at a time
+// through an LDS stage, so that the global writes are contiguous.  The order of markers
+// within one bin is whatever the LDS atomics give; the deposit and the gather do not
+// depend on it.
+constexpr int BIN_THREADS = 1024;      // workgroup of passes 1, 2 and 3
+constexpr int BIN_GROUPS_CAP = 24576;  // LDS counters of passes 1 and 2 (96 KiB)
+constexpr int BIN_MAX_BLOCKS = 256;    // workgroups of passes 1 and 2: about one per CU
+constexpr int BIN_UNROLL = 4;          // markers per lane per loop trip (loads issued together)
+constexpr int BIN_PER_LANE = 8;        // markers a pass-3 lane holds in registers
+constexpr int BIN_STAGE = BIN_THREADS * BIN_PER_LANE;  // the largest group pass 3 stages
+constexpr int BIN_KEYS_CAP = 4096;     // keys per group of the staged pass 3 (shift <= 12)
+
+// 32-bit words of one packed record: key, index, ndim positions, ncomp values (even, so
+// that float64 fields are 8-byte aligned).
+__host__ __device__ __forceinline__ int record_words(int ndim, int ncomp, int elem) {
+  const int w = 2 + (ndim + ncomp) * (elem / 4);
+  return (w + 1) & ~1;
+}
+
+template <class T, int KIND, int P, int NDIM>
+__global__ void __launch_bounds__(BIN_THREADS) bin_histogram_kernel(
+    const T *__restrict__ pos, int64_t ps, const T *__restrict__ val, int64_t vs, int64_t n, int ncomp, Grid g,
+    uint32_t dead_key, int shift, int groups, int64_t chunk, uint32_t *__restrict__ key, uint32_t *__restrict__ hist,
+    unsigned long long *__restrict__ maxbits) {
+  __shared__ uint32_t count[BIN_GROUPS_CAP];
+  const int t = threadIdx.x, B = gridDim.x, b = blockIdx.x;
+  for (int i = t; i < groups; i += BIN_THREADS) count[i] = 0;
+  __syncthreads();
+  const int64_t begin = (int64_t)b * chunk, end = begin + chunk < n ? begin + chunk : n;
+  double vmax = 0.0;
+  for (int64_t base = begin; base < end; base += (int64_t)BIN_THREADS * BIN_UNROLL) {
+    uint32_t k[BIN_UNROLL];
+#pragma unroll
+    for (int u = 0; u < BIN_UNROLL; ++u) {
+      const int64_t m = base + u * BIN_THREADS + t;
+      k[u] = m < end ? marker_key<T, KIND, P, NDIM>(pos, ps, val, vs, ncomp, m, g, dead_key, vmax) : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < BIN_UNROLL; ++u) {
+      const int64_t m = base + u * BIN_THREADS + t;
+      if (m < end) {
+        key[m] = k[u];
+        atomicAdd(&count[k[u] >> shift], 1u);
+      }
+    }
   }
+  __syncthreads();
+  for (int i = t; i < groups; i += BIN_THREADS) hist[(int64_t)i * B + b] = count[i];
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(vmax, off, 64);
+    vmax = o > vmax ? o : vmax;
+  }
+  if ((t & 63) == 0 && vmax > 0.0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(vmax));
+}
+
+// NDIM / NCOMP fixed: every load of BIN_UNROLL markers is issued before their LDS ranks
+// are taken; NCOMP < 0: any ndim / ncomp, loads after the rank.
+template <class T, int NDIM, int NCOMP>
+__global__ void __launch_bounds__(BIN_THREADS) bin_group_kernel(
+    const uint32_t *__restrict__ key, const T *__restrict__ pos, int64_t ps, int ndim, const T *__restrict__ val,
+    int64_t vs, int ncomp, int64_t n, int shift, int groups, int64_t chunk, const uint32_t *__restrict__ slots,
+    uint32_t *__restrict__ records) {
+  constexpr int F = NCOMP >= 0 ? NDIM + NCOMP : 1;
+  __shared__ uint32_t next[BIN_GROUPS_CAP];
+  const int t = threadIdx.x, B = gridDim.x, b = blockIdx.x;
+  const int stride = record_words(ndim, ncomp, (int)sizeof(T));
+  for (int i = t; i < groups; i += BIN_THREADS) next[i] = slots[(int64_t)i * B + b];
+  __syncthreads();
+  const int64_t begin = (int64_t)b * chunk, end = begin + chunk < n ? begin + chunk : n;
+  for (int64_t base = begin; base < end; base += (int64_t)BIN_THREADS * BIN_UNROLL) {
+    uint32_t k[BIN_UNROLL];
+    T f[BIN_UNROLL][F];
+#pragma unroll
+    for (int u = 0; u < BIN_UNROLL; ++u) {
+      const int64_t m = base + u * BIN_THREADS + t;
+      const bool in = m < end;
+      k[u] = in ? key[m] : 0u;
+      if constexpr (NCOMP >= 0) {
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) f[u][d] = in ? pos[d * ps + m] : T(0);
+#pragma unroll
+        for (int c = 0; c < NCOMP; ++c) f[u][NDIM + c] = in ? val[c * vs + m] : T(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < BIN_UNROLL; ++u) {
+      const int64_t m = base + u * BIN_THREADS + t;
+      if (m < end) {
+        const uint32_t dst = atomicAdd(&next[k[u] >> shift], 1u);
+        uint32_t *r = records + (int64_t)dst * stride;
+        *(unsigned long long *)r = (unsigned long long)k[u] | ((unsigned long long)m << 32);
+        T *out = (T *)(r + 2);
+        if constexpr (NCOMP >= 0) {
+#pragma unroll
+          for (int q = 0; q < NDIM + NCOMP; ++q) out[q] = f[u][q];
+        } else {
+          for (int d = 0; d < ndim; ++d) out[d] = pos[d * ps + m];
+          for (int c = 0; c < ncomp; ++c) out[ndim + c] = val[c * vs + m];
+        }
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ unsigned long long wave_inclusive_sum(unsigned long long v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned long long o = __shfl_up(v, off, 64);
+    if (lane >= off) v += o;
+  }
+  return v;
+}
+
+// In-place exclusive scan of a[0..count) by a workgroup of BLOCK lanes (each lane scans a
+// run of consecutive entries).  Ends with a barrier.
+template <int BLOCK>
+__device__ __forceinline__ void block_exclusive_scan(uint32_t *a, int count, uint32_t *wave_sums) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int per = (count + BLOCK - 1) / BLOCK, lo = t * per, hi = lo + per < count ? lo + per : count;
+  uint32_t sum = 0;
+  for (int i = lo; i < hi; ++i) sum += a[i];
+  unsigned long long incl = wave_inclusive_sum(sum);
+  if (lane == 63) wave_sums[w] = (uint32_t)incl;
+  __syncthreads();
+  uint32_t before = (uint32_t)incl - sum;
+  for (int v = 0; v < w; ++v) before += wave_sums[v];
+  for (int i = lo; i < hi; ++i) {
+    const uint32_t c = a[i];
+    a[i] = before;
+    before += c;
+  }
+  __syncthreads();
+}
+
+// Pass 3 for one group whose markers are records [lo, hi): counts per key into cnt
+// (keys entries, zero on entry), scans and writes start[] for the group's keys.  On return
+// cnt holds each key's first slot relative to lo.
+__device__ __forceinline__ void group_offsets(const uint32_t *__restrict__ records, int stride, uint32_t lo,
+                                              uint32_t hi, uint32_t first_key, int keys, uint32_t *cnt,
+                                              uint32_t *wave_sums, uint32_t *__restrict__ start) {
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += BIN_THREADS)
+    atomicAdd(&cnt[records[(int64_t)i * stride] - first_key], 1u);
+  __syncthreads();
+  block_exclusive_scan<BIN_THREADS>(cnt, keys, wave_sums);
+  for (int i = threadIdx.x; i < keys; i += BIN_THREADS) start[first_key + i] = lo + cnt[i];
+  __syncthreads();  // every offset read for start[] before the placement moves them on
+}
+
+// Placement straight from the records (any group size, any ndim / ncomp).
+template <class T>
+__device__ __forceinline__ void place_direct(const uint32_t *__restrict__ records, int stride, uint32_t lo,
+                                             uint32_t hi, uint32_t first_key, uint32_t *cnt, int ndim, int ncomp,
+                                             int64_t n, T *__restrict__ spos, T *__restrict__ sval,
+                                             int64_t *__restrict__ order) {
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += BIN_THREADS) {
+    const uint32_t *r = records + (int64_t)i * stride;
+    const uint32_t dst = lo + atomicAdd(&cnt[r[0] - first_key], 1u);
+    order[dst] = (int64_t)r[1];
+    const T *f = (const T *)(r + 2);
+    for (int d = 0; d < ndim; ++d) spos[d * n + dst] = f[d];
+    for (int c = 0; c < ncomp; ++c) sval[c * n + dst] = f[ndim + c];
+  }
+}
+
+// The group range and key range of pass-3 workgroup grp.
+__device__ __forceinline__ void group_range(const uint32_t *__restrict__ slots, int grp, int groups, int blocks,
+                                            int64_t n, int shift, uint32_t dead_key, uint32_t &lo, uint32_t &hi,
+                                            uint32_t &first_key, int &keys) {
+  lo = slots[(int64_t)grp * blocks];
+  hi = grp + 1 < groups ? slots[(int64_t)(grp + 1) * blocks] : (uint32_t)n;
+  first_key = (uint32_t)grp << shift;
+  const uint32_t left = dead_key + 1 - first_key;
+  keys = (int)(left < (1u << shift) ? left : (1u << shift));
+}
+
+// Staged pass 3: the group's records in registers (BIN_PER_LANE per lane), each output
+// row permuted in LDS and written contiguously.  Groups larger than BIN_STAGE take the
+// direct placement.
+template <class T, int NDIM, int NCOMP>
+__global__ void __launch_bounds__(BIN_THREADS) bin_cells_staged(
+    const uint32_t *__restrict__ records, int64_t n, int shift, int groups, int blocks, uint32_t dead_key,
+    const uint32_t *__restrict__ slots, T *__restrict__ spos, T *__restrict__ sval, int64_t *__restrict__ order,
+    uint32_t *__restrict__ start) {
+  constexpr int F = NDIM + NCOMP;
+  __shared__ uint32_t cnt[BIN_KEYS_CAP];
+  __shared__ T stage[BIN_STAGE];
+  __shared__ uint32_t wave_sums[BIN_THREADS / 64];
+  const int t = threadIdx.x, stride = record_words(NDIM, NCOMP, (int)sizeof(T));
+  uint32_t lo, hi, first_key;
+  int keys;
+  group_range(slots, blockIdx.x, groups, blocks, n, shift, dead_key, lo, hi, first_key, keys);
+  for (int i = t; i < keys; i += BIN_THREADS) cnt[i] = 0;
+  if (blockIdx.x == groups - 1 && t == 0) start[dead_key + 1] = (uint32_t)n;
+  __syncthreads();
+  if (hi - lo > (uint32_t)BIN_STAGE) {
+    group_offsets(records, stride, lo, hi, first_key, keys, cnt, wave_sums, start);
+    place_direct<T>(records, stride, lo, hi, first_key, cnt, NDIM, NCOMP, n, spos, sval, order);
+    return;
+  }
+  uint32_t k[BIN_PER_LANE], idx[BIN_PER_LANE];
+  T f[BIN_PER_LANE][F > 0 ? F : 1];
+#pragma unroll
+  for (int j = 0; j < BIN_PER_LANE; ++j) {
+    const uint32_t i = lo + t + j * BIN_THREADS;
+    k[j] = 0xFFFFFFFFu;
+    if (i < hi) {
+      const uint32_t *r = records + (int64_t)i * stride;
+      const unsigned long long head = *(const unsigned long long *)r;
+      k[j] = (uint32_t)head - first_key;
+      idx[j] = (uint32_t)(head >> 32);
+#pragma unroll
+      for (int q = 0; q < F; ++q) f[j][q] = ((const T *)(r + 2))[q];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < BIN_PER_LANE; ++j)
+    if (k[j] != 0xFFFFFFFFu) atomicAdd(&cnt[k[j]], 1u);
+  __syncthreads();
+  block_exclusive_scan<BIN_THREADS>(cnt, keys, wave_sums);
+  for (int i = t; i < keys; i += BIN_THREADS) start[first_key + i] = lo + cnt[i];
+  __syncthreads();
+  uint32_t dst[BIN_PER_LANE];
+#pragma unroll
+  for (int j = 0; j < BIN_PER_LANE; ++j) dst[j] = k[j] != 0xFFFFFFFFu ? atomicAdd(&cnt[k[j]], 1u) : 0u;
+  const int size = (int)(hi - lo);
+  // the input index, as the permutation
+  uint32_t *words = (uint32_t *)stage;
+#pragma unroll
+  for (int j = 0; j < BIN_PER_LANE; ++j)
+    if (k[j] != 0xFFFFFFFFu) words[dst[j]] = idx[j];
+  __syncthreads();
+  for (int i = t; i < size; i += BIN_THREADS) order[lo + i] = (int64_t)words[i];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < F; ++q) {
+#pragma unroll
+    for (int j = 0; j < BIN_PER_LANE; ++j)
+      if (k[j] != 0xFFFFFFFFu) stage[dst[j]] = f[j][q];
+    __syncthreads();
+    T *row = q < NDIM ? spos + q * n : sval + (q - NDIM) * n;
+    for (int i = t; i < size; i += BIN_THREADS) row[lo + i] = stage[i];
+    __syncthreads();
+  }
+}
+
+// Pass 3 for any ndim / ncomp and any group size (direct placement).
+template <class T, int KEYS>
+__global__ void __launch_bounds__(BIN_THREADS) bin_cells_direct(
+    const uint32_t *__restrict__ records, int ndim, int ncomp, int64_t n, int shift, int groups, int blocks,
+    uint32_t dead_key, const uint32_t *__restrict__ slots, T *__restrict__ spos, T *__restrict__ sval,
+    int64_t *__restrict__ order, uint32_t *__restrict__ start) {
+  __shared__ uint32_t cnt[KEYS];
+  __shared__ uint32_t wave_sums[BIN_THREADS / 64];
+  const int t = threadIdx.x, stride = record_words(ndim, ncomp, (int)sizeof(T));
+  uint32_t lo, hi, first_key;
+  int keys;
+  group_range(slots, blockIdx.x, groups, blocks, n, shift, dead_key, lo, hi, first_key, keys);
+  for (int i = t; i < keys; i += BIN_THREADS) cnt[i] = 0;
+  if (blockIdx.x == groups - 1 && t == 0) start[dead_key + 1] = (uint32_t)n;
+  __syncthreads();
+  group_offsets(records, stride, lo, hi, first_key, keys, cnt, wave_sums, start);
+  place_direct<T>(records, stride, lo, hi, first_key, cnt, ndim, ncomp, n, spos, sval, order);
 }
 
 // ---- exact fixed-point accumulation in float64 registers --------------------------------
@@ -236,15 +485,6 @@ __device__ __forceinline__ uint32_t items_of(uint32_t count) {
   return k < (uint32_t)SWEEPS ? k : (uint32_t)SWEEPS;
 }
 
-__device__ __forceinline__ unsigned long long wave_inclusive_sum(unsigned long long v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const unsigned long long o = __shfl_up(v, off, 64);
-    if (lane >= off) v += o;
-  }
-  return v;
-}
 
 // Fill items[] for the cells whose marker offsets are cfirst[0..TC]; returns the number of
 // items.  One block-wide scan: each work-item packs its cell's "has a j-th item" flags
@@ -1016,34 +1256,84 @@ extern "C" int mpf_bin_sizes(int ndim, const int64_t *num_nodes, int64_t *cells)
 }
 
 template <class T, int KIND, int P, int NDIM>
-static void launch_count(const void *pos, int64_t ps, const void *val, int64_t vs, int64_t n, int ncomp, const Grid &g,
-                         int64_t dead_key, uint32_t *key, uint32_t *rank, uint32_t *count, unsigned long long *maxbits,
-                         hipStream_t stream) {
-  hipLaunchKernelGGL((bin_count_kernel<T, KIND, P, NDIM>), dim3(blocks_for(n, 256, 1 << 16)), dim3(256), 0, stream,
-                     (const T *)pos, ps, (const T *)val, vs, n, ncomp, g, dead_key, key, rank, count, maxbits);
+static void launch_histogram(const void *pos, int64_t ps, const void *val, int64_t vs, int64_t n, int ncomp,
+                             const Grid &g, uint32_t dead_key, int shift, int groups, int blocks, int64_t chunk,
+                             uint32_t *key, uint32_t *hist, unsigned long long *maxbits, hipStream_t stream) {
+  hipLaunchKernelGGL((bin_histogram_kernel<T, KIND, P, NDIM>), dim3(blocks), dim3(BIN_THREADS), 0, stream,
+                     (const T *)pos, ps, (const T *)val, vs, n, ncomp, g, dead_key, shift, groups, chunk, key, hist,
+                     maxbits);
 }
 
 template <class T, int KIND, int P>
-static void dispatch_count_nd(int ndim, const void *pos, int64_t ps, const void *val, int64_t vs, int64_t n, int ncomp,
-                              const Grid &g, int64_t dead_key, uint32_t *key, uint32_t *rank, uint32_t *count,
-                              unsigned long long *maxbits, hipStream_t stream) {
-  if (ndim == 1) launch_count<T, KIND, P, 1>(pos, ps, val, vs, n, ncomp, g, dead_key, key, rank, count, maxbits, stream);
-  if (ndim == 2) launch_count<T, KIND, P, 2>(pos, ps, val, vs, n, ncomp, g, dead_key, key, rank, count, maxbits, stream);
-  if (ndim == 3) launch_count<T, KIND, P, 3>(pos, ps, val, vs, n, ncomp, g, dead_key, key, rank, count, maxbits, stream);
+static void dispatch_histogram_nd(int ndim, const void *pos, int64_t ps, const void *val, int64_t vs, int64_t n,
+                                  int ncomp, const Grid &g, uint32_t dead_key, int shift, int groups, int blocks,
+                                  int64_t chunk, uint32_t *key, uint32_t *hist, unsigned long long *maxbits,
+                                  hipStream_t stream) {
+#define HIST(D) \
+  launch_histogram<T, KIND, P, D>(pos, ps, val, vs, n, ncomp, g, dead_key, shift, groups, blocks, chunk, key, hist, \
+                                  maxbits, stream)
+  if (ndim == 1) HIST(1);
+  if (ndim == 2) HIST(2);
+  if (ndim == 3) HIST(3);
+#undef HIST
 }
 
 template <class T>
-static int dispatch_count(int kind, int degree, int ndim, const void *pos, int64_t ps, const void *val, int64_t vs,
-                          int64_t n, int ncomp, const Grid &g, int64_t dead_key, uint32_t *key, uint32_t *rank,
-                          uint32_t *count, unsigned long long *maxbits, hipStream_t stream) {
-#define CNT(K, P) dispatch_count_nd<T, K, P>(ndim, pos, ps, val, vs, n, ncomp, g, dead_key, key, rank, count, maxbits, stream)
-  if (kind == LINEAR) CNT(LINEAR, 1);
-  else if (kind == QUADRATIC) CNT(QUADRATIC, 2);
-  else if (degree == 0) CNT(SPLINE, 0);
-  else if (degree == 1) CNT(SPLINE, 1);
-  else if (degree == 2) CNT(SPLINE, 2);
-  else CNT(SPLINE, 3);
-#undef CNT
+static void dispatch_histogram(int kind, int degree, int ndim, const void *pos, int64_t ps, const void *val,
+                               int64_t vs, int64_t n, int ncomp, const Grid &g, uint32_t dead_key, int shift,
+                               int groups, int blocks, int64_t chunk, uint32_t *key, uint32_t *hist,
+                               unsigned long long *maxbits, hipStream_t stream) {
+#define HIST(K, P) \
+  dispatch_histogram_nd<T, K, P>(ndim, pos, ps, val, vs, n, ncomp, g, dead_key, shift, groups, blocks, chunk, key, \
+                                 hist, maxbits, stream)
+  if (kind == LINEAR) HIST(LINEAR, 1);
+  else if (kind == QUADRATIC) HIST(QUADRATIC, 2);
+  else if (degree == 0) HIST(SPLINE, 0);
+  else if (degree == 1) HIST(SPLINE, 1);
+  else if (degree == 2) HIST(SPLINE, 2);
+  else HIST(SPLINE, 3);
+#undef HIST
+}
+
+template <class T>
+static int launch_group_and_cells(const void *pos, int64_t ps, int ndim, const void *val, int64_t vs, int ncomp,
+                                  int64_t n, int shift, int groups, int blocks, int64_t chunk, uint32_t dead_key,
+                                  const uint32_t *key, const uint32_t *slots, uint32_t *records, void *spos,
+                                  void *sval, int64_t *order, uint32_t *start, hipStream_t stream) {
+  if (n > 0) {
+#define GROUP(D, C)                                                                                               \
+  hipLaunchKernelGGL((bin_group_kernel<T, D, C>), dim3(blocks), dim3(BIN_THREADS), 0, stream, key, (const T *)pos, \
+                     ps, ndim, (const T *)val, vs, ncomp, n, shift, groups, chunk, slots, records)
+    if (ndim == 3 && ncomp == 1) GROUP(3, 1);
+    else if (ndim == 3 && ncomp == 3) GROUP(3, 3);
+    else if (ndim == 3 && ncomp == 0) GROUP(3, 0);
+    else if (ndim == 2 && ncomp == 1) GROUP(2, 1);
+    else if (ndim == 2 && ncomp == 2) GROUP(2, 2);
+    else if (ndim == 2 && ncomp == 0) GROUP(2, 0);
+    else if (ndim == 1 && ncomp == 1) GROUP(1, 1);
+    else if (ndim == 1 && ncomp == 0) GROUP(1, 0);
+    else GROUP(3, -1);
+#undef GROUP
+    MPF_LAUNCH_CHECK();
+  }
+  const dim3 grid(groups), block(BIN_THREADS);
+#define STAGED(D, C)                                                                                            \
+  if (ndim == D && ncomp == C) {                                                                                \
+    hipLaunchKernelGGL((bin_cells_staged<T, D, C>), grid, block, 0, stream, records, n, shift, groups, blocks,    \
+                       dead_key, slots, (T *)spos, (T *)sval, order, start);                                    \
+    MPF_LAUNCH_CHECK();                                                                                         \
+    return MPF_OK;                                                                                              \
+  }
+  if (shift <= 12) {
+    STAGED(3, 1) STAGED(3, 3) STAGED(3, 0) STAGED(2, 1) STAGED(2, 2) STAGED(2, 0) STAGED(1, 1) STAGED(1, 0)
+    hipLaunchKernelGGL((bin_cells_direct<T, BIN_KEYS_CAP>), grid, block, 0, stream, records, ndim, ncomp, n, shift,
+                       groups, blocks, dead_key, slots, (T *)spos, (T *)sval, order, start);
+  } else {
+    hipLaunchKernelGGL((bin_cells_direct<T, 32768>), grid, block, 0, stream, records, ndim, ncomp, n, shift, groups,
+                       blocks, dead_key, slots, (T *)spos, (T *)sval, order, start);
+  }
+#undef STAGED
+  MPF_LAUNCH_CHECK();
   return MPF_OK;
 }
 
@@ -1083,48 +1373,69 @@ extern "C" int mpf_bin_markers(int kind, int degree, int ndim, const void *posit
     set_error("bin_markers: too many cells for 32-bit keys");
     return MPF_ERR_ARGUMENT;
   }
+  // groups of 2^shift keys: about BIN_STAGE / 2 markers per group on average (at most
+  // BIN_KEYS_CAP keys), and few enough groups for the LDS counters of passes 1 and 2
+  int shift = 0;
+  const double keys_per_half_stage = (BIN_STAGE / 2) * (double)cells / (double)(n > 0 ? n : 1);
+  while (shift < 12 && (double)(1 << (shift + 1)) <= keys_per_half_stage) ++shift;
+  while (shift < 15 && (cells >> shift) + 1 > BIN_GROUPS_CAP) ++shift;
+  const int groups = (int)((cells >> shift) + 1);
+  if (groups > BIN_GROUPS_CAP) {
+    set_error("bin_markers: %lld cells are more than the sort handles", (long long)cells);
+    return MPF_ERR_ARGUMENT;
+  }
+  int64_t blocks = (n + 8191) / 8192;
+  if (blocks < 1) blocks = 1;
+  if (blocks > BIN_MAX_BLOCKS) blocks = BIN_MAX_BLOCKS;
+  const int64_t chunk = n > 0 ? (n + blocks - 1) / blocks : 1;
   hipStream_t stream = (hipStream_t)stream_;
   if (!ctx) ctx = default_context();
   if (!ctx) return MPF_ERR_HIP;
-  // scratch: key + rank per marker, the counts, the scan's temporary storage
-  rc = ensure(ctx->sort_keys, (size_t)(n > 0 ? n : 1) * 8, false, stream);
+  // scratch: the keys in input order, the markers as packed records in group order, the
+  // per-(group, block) counts and their scan
+  const size_t nn = (size_t)(n > 0 ? n : 1), elem = dtype == F32 ? 4 : 8;
+  rc = ensure(ctx->sort_keys, nn * 4, false, stream);
   if (rc) return rc;
-  uint32_t *key = (uint32_t *)ctx->sort_keys.ptr, *rank = key + (n > 0 ? n : 1);
-  rc = ensure(ctx->offsets, (size_t)(cells + 2) * 4, false, stream);
+  uint32_t *key = (uint32_t *)ctx->sort_keys.ptr;
+  rc = ensure(ctx->staging, nn * 4 * (size_t)record_words(ndim, (int)ncomp, (int)elem), false, stream);
   if (rc) return rc;
-  uint32_t *count = (uint32_t *)ctx->offsets.ptr;
-  rc = zero_async(count, (size_t)(cells + 2) * 4, stream);
+  uint32_t *records = (uint32_t *)ctx->staging.ptr;
+  const size_t entries = (size_t)groups * (size_t)blocks;
+  rc = ensure(ctx->offsets, entries * 8, false, stream);
   if (rc) return rc;
+  uint32_t *hist = (uint32_t *)ctx->offsets.ptr, *slots = hist + entries;
   rc = ensure(ctx->scalars, 64, true, stream);
   if (rc) return rc;
   unsigned long long *maxbits = (unsigned long long *)ctx->scalars.ptr;
   rc = zero_async(maxbits, 8, stream);
   if (rc) return rc;
+  const uint32_t dead_key = (uint32_t)cells;
   if (n > 0) {
     if (dtype == F32)
-      dispatch_count<float>(kind, degree, ndim, positions, pos_stride, values, val_stride, n, (int)ncomp, g, cells,
-                            key, rank, count, maxbits, stream);
+      dispatch_histogram<float>(kind, degree, ndim, positions, pos_stride, values, val_stride, n, (int)ncomp, g,
+                                dead_key, shift, groups, (int)blocks, chunk, key, hist, maxbits, stream);
     else
-      dispatch_count<double>(kind, degree, ndim, positions, pos_stride, values, val_stride, n, (int)ncomp, g, cells,
-                             key, rank, count, maxbits, stream);
+      dispatch_histogram<double>(kind, degree, ndim, positions, pos_stride, values, val_stride, n, (int)ncomp, g,
+                                 dead_key, shift, groups, (int)blocks, chunk, key, hist, maxbits, stream);
     MPF_LAUNCH_CHECK();
+  } else {
+    rc = zero_async(hist, entries * 4, stream);
+    if (rc) return rc;
   }
   size_t temp = 0;
-  MPF_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, temp, count, start, (int)(cells + 2), stream));
+  MPF_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, temp, hist, slots, (int)entries, stream));
   rc = ensure(ctx->sort_tmp, temp, false, stream);
   if (rc) return rc;
-  MPF_CHECK(hipcub::DeviceScan::ExclusiveSum(ctx->sort_tmp.ptr, temp, count, start, (int)(cells + 2), stream));
-  if (n > 0) {
-    if (dtype == F32)
-      hipLaunchKernelGGL(bin_place_kernel<float>, dim3(blocks_for(n, 256, 1 << 16)), dim3(256), 0, stream,
-                         (const float *)positions, pos_stride, ndim, (const float *)values, val_stride, (int)ncomp, n,
-                         key, rank, start, (float *)sorted_positions, (float *)sorted_values, order);
-    else
-      hipLaunchKernelGGL(bin_place_kernel<double>, dim3(blocks_for(n, 256, 1 << 16)), dim3(256), 0, stream,
-                         (const double *)positions, pos_stride, ndim, (const double *)values, val_stride, (int)ncomp,
-                         n, key, rank, start, (double *)sorted_positions, (double *)sorted_values, order);
-    MPF_LAUNCH_CHECK();
-  }
+  MPF_CHECK(hipcub::DeviceScan::ExclusiveSum(ctx->sort_tmp.ptr, temp, hist, slots, (int)entries, stream));
+  if (dtype == F32)
+    rc = launch_group_and_cells<float>(positions, pos_stride, ndim, values, val_stride, (int)ncomp, n, shift, groups,
+                                       (int)blocks, chunk, dead_key, key, slots, records, sorted_positions,
+                                       sorted_values, order, start, stream);
+  else
+    rc = launch_group_and_cells<double>(positions, pos_stride, ndim, values, val_stride, (int)ncomp, n, shift, groups,
+                                        (int)blocks, chunk, dead_key, key, slots, records, sorted_positions,
+                                        sorted_values, order, start, stream);
+  if (rc) return rc;
   // the scale, from the maximum the count pass found
   rc = prepare_scale(ctx, nullptr, 0, dtype, 1, n, 1.0, scale_exp, stream);
   if (rc) return rc;

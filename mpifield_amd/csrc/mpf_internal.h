@@ -27,7 +27,7 @@ struct mpf_context {
   mpf::Buffer sort_keys;
   mpf::Buffer sort_tmp;
   mpf::Buffer offsets;
-  mpf::Buffer perm;
+  mpf::Buffer staging;   // markers in group order while binning
 };
 
 namespace mpf {

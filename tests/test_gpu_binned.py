@@ -81,6 +81,106 @@ def test_binned_scatter_and_gather_match_oracle(kind, degree, ncomp, num, code):
     assert gerr < 1e-12, gerr
 
 
+def anchor_cell(kind, degree, rel, num, code):
+    """The stencil cell of one marker on one axis: the node at the stencil's offset (0 for
+    linear, 1 for quadratic, degree // 2 for a spline) in the oracle's node list."""
+    nodes, _ = orc.axis_stencil(NAMES[(kind, degree)], rel, num, code, degree)
+    off = 0 if kind == 0 else (1 if kind == 1 else degree // 2)
+    return int(nodes[off])
+
+
+def anchor_cells(kind, degree, rel, num, code):
+    """anchor_cell for many markers at once (numpy); pinned to anchor_cell on a sample."""
+    r = rel.astype(np.float64).copy()
+    if code == 1:
+        bad = ~((r >= 0.0) & (r < num))
+        r[bad] = r[bad] - num * np.floor(r[bad] / num)
+    if kind == 0:
+        i = np.floor(np.clip(np.nan_to_num(r, nan=0.0), 0.0, num)).astype(np.int64)
+        return np.minimum(i, num - 1) if code == 1 else np.minimum(i, num - 2)
+    if code != 1:
+        r = np.clip(r, 0.0, num - 1.0)
+    p = 2 if kind == 1 else degree
+    mid = np.floor(r + 0.5) if p % 2 == 0 else np.floor(r)
+    mid = mid.astype(np.int64)
+    return np.where(mid >= num, mid - num, mid) if code == 1 else np.clip(mid, 0, num - 1)
+
+
+def decode_bin(k, num):
+    """(cell per axis) of a 3D bin in tile order (tiles of 8 x 8 x 16, x-major, z fastest)."""
+    t, c = divmod(int(k), 8 * 8 * 16)
+    nty, ntz = -(-num[1] // 8), -(-num[2] // 16)
+    tx, rest = divmod(t, nty * ntz)
+    ty, tz = divmod(rest, ntz)
+    cx, rest = divmod(c, 8 * 16)
+    cy, cz = divmod(rest, 16)
+    return (tx * 8 + cx, ty * 8 + cy, tz * 16 + cz)
+
+
+@pytest.mark.parametrize("case", ["3d-quadratic", "3d-linear-bounded", "3d-cubic-gather", "2d-large", "1d"])
+def test_bins_hold_each_marker_in_its_stencil_cell(case):
+    """Every binned marker sits in the bin of the cell the oracle's stencil anchors it to; the
+    2D case has enough cells (2^25) that the sort works on groups of 2048 keys."""
+    print(f"bins {case}", flush=True)
+    rng = np.random.default_rng(7)
+    if case.startswith("3d"):
+        num, code = ((40, 24, 48), (1, 0, 1)) if case != "3d-linear-bounded" else ((21, 13, 37), (0, 0, 0))
+        kind, degree = {"3d-quadratic": (1, 2), "3d-linear-bounded": (0, 1), "3d-cubic-gather": (2, 3)}[case]
+        lower, cell = np.array([0.0, -1.0, 0.5]), np.array([0.0625, 0.125, 0.03125])
+    elif case == "2d-large":
+        num, code, kind, degree = (8192, 4096), (1, 1), 1, 2
+        lower, cell = np.array([0.0, 0.0]), np.array([1 / 8192, 1 / 4096])
+    else:
+        num, code, kind, degree = (5000,), (0,), 2, 3
+        lower, cell = np.array([-2.0]), np.array([0.001])
+    ndim = len(num)
+    span = cell * np.array(num)
+    n = 400000
+    pos = rng.uniform(lower - 0.2 * span, lower + 1.2 * span, (n, ndim)).T.copy()
+    gather_only = case == "3d-cubic-gather"
+    val = None if gather_only else rng.normal(size=(1, n))
+    if val is not None:
+        val[:, ::50] = 0.0
+    g = (*lower, *cell, *num, *code)
+    bins = binning.bin_markers(cuda(pos), None if val is None else cuda(val), (kind, degree), g)
+    start = bins.start.cpu().numpy().astype(np.int64)
+    order = bins.order.cpu().numpy()
+    cells = start.size - 2
+    assert start[0] == 0 and start[-1] == n and np.all(np.diff(start) >= 0)
+    assert np.array_equal(np.sort(order), np.arange(n))
+    np.testing.assert_array_equal(bins.positions.cpu().numpy(), pos[:, order])
+    if val is not None:
+        np.testing.assert_array_equal(bins.values.cpu().numpy(), val[:, order])
+        dead = (val == 0).all(axis=0)
+        assert start[-1] - start[-2] == dead.sum() and dead[order[start[-2]:]].all()
+    else:
+        assert start[-1] == start[-2]
+    # every binned marker sits in the bin of its anchor cell; the vectorised anchors are
+    # checked against the oracle's stencils on a sample first
+    spos = bins.positions.cpu().numpy()
+    live = int(start[-2])
+    rel = [(spos[d, :live] - lower[d]) * (1.0 / cell[d]) for d in range(ndim)]
+    anchors = [anchor_cells(kind, degree, rel[d], num[d], code[d]) for d in range(ndim)]
+    for i in rng.choice(live, size=min(500, live), replace=False):
+        want = tuple(anchor_cell(kind, degree, rel[d][i], num[d], code[d]) for d in range(ndim))
+        assert tuple(int(a[i]) for a in anchors) == want, (i, want)
+    bin_of = np.searchsorted(start, np.arange(live), side="right") - 1
+    if ndim == 3:
+        t, c = np.divmod(bin_of, 8 * 8 * 16)
+        nty, ntz = -(-num[1] // 8), -(-num[2] // 16)
+        tx, rest = np.divmod(t, nty * ntz)
+        ty, tz = np.divmod(rest, ntz)
+        cx, rest = np.divmod(c, 8 * 16)
+        cy, cz = np.divmod(rest, 16)
+        got = (tx * 8 + cx, ty * 8 + cy, tz * 16 + cz)
+        assert decode_bin(bin_of[live // 2], num) == tuple(int(x[live // 2]) for x in got)
+    else:
+        got = np.unravel_index(bin_of, num)
+    for d in range(ndim):
+        bad = np.nonzero(got[d] != anchors[d])[0]
+        assert bad.size == 0, (d, bad[:5], got[d][bad[:5]], anchors[d][bad[:5]])
+
+
 def test_binned_deposit_is_bit_reproducible_under_any_marker_order():
     num, code = (32, 32, 32), (1, 1, 1)
     g = geom(num, code)
