@@ -238,6 +238,26 @@ int mpf_gather_binned(int kind, int degree, int ndim, const void *sorted_positio
                       const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
                       mpf_stream stream);
 
+/* The gradient of the interpolant at binned markers, contracted with per-marker
+ * coefficients: out[d * out_stride + order[m]] = sum_c coef[c * coef_stride + order[m]] *
+ * dF_c/dx_d for binned marker m (float64 sums, written in the marker dtype), so that
+ * coefficients and result are in the input order.  3D, the cases of mpf_scatter_binned.
+ * No counterpart in the reference (which has no differentiation): the tiled form of
+ * mpf_gather_derivative followed by the contraction the backward of gather and scatter
+ * takes (the adjoint identity of test_catalog_and_pyccel.py:160-182). */
+int mpf_gather_derivative_binned(int kind, int degree, int ndim, const void *sorted_positions, int dtype, int64_t n,
+                                 const uint32_t *start, const int64_t *order, const double *field, int64_t ncomp,
+                                 const void *coef, int64_t coef_stride, void *out, int64_t out_stride,
+                                 const double *lower, const double *cell_size, const int64_t *num_nodes,
+                                 const int64_t *code, mpf_context *ctx, mpf_stream stream);
+
+/* The fixed-point scale of a deposit of values (ncomp rows of n, row stride val_stride):
+ * scalars gets [2^k, 2^-k, markers per register pass] as mpf_bin_markers writes it, so that
+ * bins made from positions alone (ncomp 0) can take values later.  No counterpart in the
+ * reference; the scale rule is that of the fixed-point deposit (mpf_scatter). */
+int mpf_fixed_scale(const void *values, int64_t val_stride, int dtype, int64_t n, int64_t ncomp, int scale_exp,
+                    double *scalars, mpf_context *ctx, mpf_stream stream);
+
 /* ---- many patches in one launch ---------------------------------------------------- */
 
 /* One refined-mesh patch: the arguments of one Weighting.scatter_block / gather_block call

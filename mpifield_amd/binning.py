@@ -141,3 +141,38 @@ def gather_binned(bins: MarkerBins, field: torch.Tensor, result: torch.Tensor) -
                   K._dtype_code(bins.positions), bins.n, bins.start.data_ptr(), field.data_ptr(), ncomp,
                   result.data_ptr(), lower.ctypes.data, cell.ctypes.data, num.ctypes.data, code.ctypes.data, None,
                   K.stream_handle(field.device))
+
+
+def with_values(bins: MarkerBins, values, scale_exp: int = _lib.SCALE_AUTO) -> MarkerBins:
+    """The same bins carrying ``values`` (ncomp, N) in the input order: the values permuted to
+    the bins' order and their fixed-point scale.  ``bins`` must hold every marker in a cell
+    (binned from positions alone), since a marker in the dead bin deposits nothing."""
+    if bins.values is not None:
+        raise ValueError("these bins were made with values; bin positions alone to give them new values")
+    val, vs = K.marker_rows(values, None, bins.positions.dtype)
+    if val.shape[1] != bins.n:
+        raise ValueError(f"{bins.n} binned markers for {val.shape[1]} values")
+    sval = torch.index_select(val, 1, bins.order)
+    scalars = torch.empty(3, dtype=torch.float64, device=val.device)
+    with torch.cuda.device(val.device):
+        _lib.call("mpf_fixed_scale", val.data_ptr(), vs, K._dtype_code(val), bins.n, val.shape[0], int(scale_exp),
+                  scalars.data_ptr(), None, K.stream_handle(val.device))
+    return MarkerBins(bins.kind, bins.degree, bins.geometry, bins.positions, sval, bins.order, bins.start, scalars)
+
+
+def gather_derivative_binned(bins: MarkerBins, field: torch.Tensor, coef: torch.Tensor) -> torch.Tensor:
+    """sum_c coef[c] * grad F_c at the binned markers: (ndim, N) in the input order, with
+    ``coef`` (ncomp, N) in the input order and ``field`` num_nodes + (ncomp,)."""
+    field = K._field(field)
+    ndim, lower, cell, num, code = _geometry_arrays(bins.geometry)
+    ncomp = field.shape[-1]
+    k, ks = K.marker_rows(coef, ncomp, bins.positions.dtype)
+    if k.shape[1] != bins.n:
+        raise ValueError(f"{bins.n} binned markers for {k.shape[1]} coefficients")
+    out = torch.empty((ndim, bins.n), dtype=bins.positions.dtype, device=field.device)
+    with torch.cuda.device(field.device):
+        _lib.call("mpf_gather_derivative_binned", bins.kind, bins.degree, ndim, bins.positions.data_ptr(),
+                  K._dtype_code(bins.positions), bins.n, bins.start.data_ptr(), bins.order.data_ptr(),
+                  field.data_ptr(), ncomp, k.data_ptr(), ks, out.data_ptr(), bins.n, lower.ctypes.data,
+                  cell.ctypes.data, num.ctypes.data, code.ctypes.data, None, K.stream_handle(field.device))
+    return out

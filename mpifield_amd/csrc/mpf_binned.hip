@@ -1202,6 +1202,172 @@ __global__ void __launch_bounds__(256) gather_dead_kernel(const T *__restrict__ 
   }
 }
 
+// ---- derivative gather ------------------------------------------------------------------
+//
+// The weights and their derivatives on one axis (anchor_axis's cell and weights; the
+// derivative of each weight with respect to rel, zero where a bounded axis clamps the
+// position or for degree 0), as axis_derivative computes them for the direct kernels.
+template <int KIND, int P>
+__device__ __forceinline__ int anchor_axis_d(double rel, int num, int code, double *w, double *dw) {
+  const int a = anchor_axis<KIND, P>(rel, num, code, w);
+  constexpr int S = StencilShape<KIND, P>::S;
+  const bool clamped = code == 0 && (rel < 0.0 || rel > (double)(num - 1));
+  if (clamped || (KIND == SPLINE && P == 0)) {
+#pragma unroll
+    for (int i = 0; i < S; ++i) dw[i] = 0.0;
+    return a;
+  }
+  if (KIND == LINEAR || P == 1) {
+    dw[0] = -1.0;
+    dw[1] = 1.0;
+    return a;
+  }
+  double r = rel;
+  if (code == 1 && !(r >= 0.0 && r < (double)num)) r = r - (double)num * floor(r / (double)num);
+  const double o = r - (double)a;  // a is the unwrapped middle node of the wrapped r
+  if (P == 2) {
+    dw[0] = -(0.5 - o);
+    dw[1] = -2.0 * o;
+    dw[2] = 0.5 + o;
+  } else {
+    dw[0] = -0.5 * (1.0 - o) * (1.0 - o);
+    dw[1] = 1.5 * o * o - 2.0 * o;
+    dw[2] = -1.5 * o * o + o + 0.5;
+    dw[3] = 0.5 * o * o;
+  }
+  return a;
+}
+
+// The gradient of the interpolant contracted with per-marker coefficients, for binned 3D
+// markers: out[d * os + order[m]] = sum_c coef[c * cs + order[m]] * dF_c/dx_d at binned
+// marker m (coefficients and result in the input order).  One workgroup per tile: the tile's
+// node footprint in float64 in LDS, float64 weights and sums, U markers per work-item with
+// their position loads in flight together.
+template <class T, int KIND, int P, int C, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) derivative_tiled(const T *__restrict__ pos, int64_t n,
+                                                          const uint32_t *__restrict__ start, Grid g, int ntx,
+                                                          int nty, int ntz, const double *__restrict__ field,
+                                                          const T *__restrict__ coef, int64_t cs,
+                                                          const int64_t *__restrict__ order, T *__restrict__ out,
+                                                          int64_t os) {
+  using Sh = StencilShape<KIND, P>;
+  constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, TC = TILE_CELLS;
+  constexpr int S = Sh::S, OFF = Sh::OFF;
+  constexpr int LX = TX + S - 1, LY = TY + S - 1, LZ = TZ + S - 1, L = LX * LY * LZ;
+  __shared__ double tile[L * C];
+
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int xcd = b % 8, per = nb / 8, extra = nb % 8;
+  const int id = xcd * per + (xcd < extra ? xcd : extra) + b / 8;  // the tile, in tile-key order
+  const int tz = id % ntz, ty = (id / ntz) % nty, tx = id / (ntz * nty);
+  const int num0 = (int)g.num[0], num1 = (int)g.num[1], num2 = (int)g.num[2];
+  const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
+  for (int i = threadIdx.x; i < L; i += BLOCK) {
+    const int iz = i % LZ, iy = (i / LZ) % LY, ix = i / (LZ * LY);
+    const int64_t gx = wrap_index(x0 - OFF + ix, num0, g.code[0]);
+    const int64_t gy = wrap_index(y0 - OFF + iy, num1, g.code[1]);
+    const int64_t gz = wrap_index(z0 - OFF + iz, num2, g.code[2]);
+    const int64_t at = ((gx * num1 + gy) * num2 + gz) * C;
+#pragma unroll
+    for (int c = 0; c < C; ++c) tile[i * C + c] = field[at + c];
+  }
+  const int64_t tile_key = (int64_t)id * TC;
+  const int64_t tile_first = start[tile_key];
+  const uint32_t total = start[tile_key + TC] - (uint32_t)tile_first;
+  __syncthreads();
+
+  constexpr int U = 4;
+  for (uint32_t i0 = threadIdx.x; i0 < total; i0 += U * BLOCK) {
+    T px[U], py[U], pz[U];
+    int64_t src[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = i0 + u * BLOCK;
+      const int64_t m = tile_first + (i < total ? i : 0);
+      px[u] = pos[m];
+      py[u] = pos[n + m];
+      pz[u] = pos[2 * n + m];
+      src[u] = order[m];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = i0 + u * BLOCK;
+      if (i >= total) continue;
+      double wx[4], wy[4], wz[4], dx[4], dy[4], dz[4];
+      int bx = anchor_axis_d<KIND, P>(((double)px[u] - g.lower[0]) * g.inverse[0], num0, g.code[0], wx, dx);
+      int by = anchor_axis_d<KIND, P>(((double)py[u] - g.lower[1]) * g.inverse[1], num1, g.code[1], wy, dy);
+      int bz = anchor_axis_d<KIND, P>(((double)pz[u] - g.lower[2]) * g.inverse[2], num2, g.code[2], wz, dz);
+      bx = (g.code[0] == 1 && bx >= num0 ? bx - num0 : bx) - x0;
+      by = (g.code[1] == 1 && by >= num1 ? by - num1 : by) - y0;
+      bz = (g.code[2] == 1 && bz >= num2 ? bz - num2 : bz) - z0;
+      bx = bx < 0 ? 0 : (bx > TX - 1 ? TX - 1 : bx);
+      by = by < 0 ? 0 : (by > TY - 1 ? TY - 1 : by);
+      bz = bz < 0 ? 0 : (bz > TZ - 1 ? TZ - 1 : bz);
+      const double *node = tile + ((bx * LY + by) * LZ + bz) * C;
+      double k[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) k[c] = (double)coef[c * cs + src[u]];
+      double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+#pragma unroll
+      for (int a = 0; a < S; ++a)
+#pragma unroll
+        for (int b2 = 0; b2 < S; ++b2) {
+          double sw = 0.0, sd = 0.0;
+#pragma unroll
+          for (int e = 0; e < S; ++e) {
+            const double *f = node + ((a * LY + b2) * LZ + e) * C;
+            double kf = k[0] * f[0];
+#pragma unroll
+            for (int c = 1; c < C; ++c) kf = __builtin_fma(k[c], f[c], kf);
+            sw = __builtin_fma(wz[e], kf, sw);
+            sd = __builtin_fma(dz[e], kf, sd);
+          }
+          t0 = __builtin_fma(dx[a] * wy[b2], sw, t0);
+          t1 = __builtin_fma(wx[a] * dy[b2], sw, t1);
+          t2 = __builtin_fma(wx[a] * wy[b2], sd, t2);
+        }
+      out[src[u]] = (T)(t0 * g.inverse[0]);
+      out[os + src[u]] = (T)(t1 * g.inverse[1]);
+      out[2 * os + src[u]] = (T)(t2 * g.inverse[2]);
+    }
+  }
+}
+
+// The dead bin's markers (when the bins were made with values) for derivative_tiled, one
+// marker per work-item from the global field.
+template <class T, int KIND>
+__global__ void __launch_bounds__(256) derivative_dead_kernel(const T *__restrict__ pos, int64_t n, int degree,
+                                                              const uint32_t *__restrict__ start, int64_t cells, Grid g,
+                                                              const double *__restrict__ field, int C,
+                                                              const T *__restrict__ coef, int64_t cs,
+                                                              const int64_t *__restrict__ order, T *__restrict__ out,
+                                                              int64_t os) {
+  const int64_t first = start[cells], last = start[cells + 1];
+  for (int64_t m = first + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < last;
+       m += (int64_t)gridDim.x * blockDim.x) {
+    int64_t node[3][4];
+    double w[3][4], dw[3][4];
+    int size[3];
+    for (int d = 0; d < 3; ++d)
+      size[d] = axis_derivative<KIND>(((double)pos[d * n + m] - g.lower[d]) * g.inverse[d], g.num[d], g.code[d],
+                                      degree, node[d], w[d], dw[d]);
+    const int64_t src = order[m];
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < C; ++c) {
+      const double k = (double)coef[c * cs + src];
+      for (int a = 0; a < size[0]; ++a)
+        for (int b = 0; b < size[1]; ++b)
+          for (int e = 0; e < size[2]; ++e) {
+            const double f = k * field[((node[0][a] * g.num[1] + node[1][b]) * g.num[2] + node[2][e]) * C + c];
+            t[0] += dw[0][a] * w[1][b] * w[2][e] * f;
+            t[1] += w[0][a] * dw[1][b] * w[2][e] * f;
+            t[2] += w[0][a] * w[1][b] * dw[2][e] * f;
+          }
+    }
+    for (int d = 0; d < 3; ++d) out[d * os + src] = (T)(t[d] * g.inverse[d]);
+  }
+}
+
 // ---- host side ------------------------------------------------------------------------
 
 struct TileShape {
@@ -1570,4 +1736,80 @@ extern "C" int mpf_gather_binned(int kind, int degree, int ndim, const void *sor
 #undef GA
   set_error("no tiled kernel for this case");
   return MPF_ERR_UNSUPPORTED;
+}
+
+template <class T, int KIND, int P, int C>
+static int run_derivative_tiled(const void *pos, int64_t n, const uint32_t *start, const Grid &g, const double *field,
+                                const void *coef, int64_t cs, const int64_t *order, void *out, int64_t os,
+                                hipStream_t stream) {
+  constexpr int BLOCK = 256;
+  const int ntx = (int)((g.num[0] + TILE_X - 1) / TILE_X), nty = (int)((g.num[1] + TILE_Y - 1) / TILE_Y),
+            ntz = (int)((g.num[2] + TILE_Z - 1) / TILE_Z);
+  {
+    ProfileScope prof("derivative_tiled", stream);
+    hipLaunchKernelGGL((derivative_tiled<T, KIND, P, C, BLOCK>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
+                       (const T *)pos, n, start, g, ntx, nty, ntz, field, (const T *)coef, cs, order, (T *)out, os);
+  }
+  MPF_LAUNCH_CHECK();
+  const int64_t cells = (int64_t)ntx * nty * ntz * TILE_CELLS;
+  hipLaunchKernelGGL((derivative_dead_kernel<T, KIND>), dim3(64), dim3(256), 0, stream, (const T *)pos, n, P, start,
+                     cells, g, field, C, (const T *)coef, cs, order, (T *)out, os);
+  MPF_LAUNCH_CHECK();
+  return MPF_OK;
+}
+
+/* The gradient of the interpolant at binned markers, contracted with per-marker
+ * coefficients: out[d * out_stride + order[m]] = sum_c coef[c * coef_stride + order[m]] *
+ * dF_c/dx_d (float64 sums, the marker dtype out).  sorted_positions, order and start come
+ * from mpf_bin_markers of the same kind/degree/grid; coef and out are in the input order. */
+extern "C" int mpf_gather_derivative_binned(int kind, int degree, int ndim, const void *sorted_positions, int dtype,
+                                            int64_t n, const uint32_t *start, const int64_t *order,
+                                            const double *field, int64_t ncomp, const void *coef,
+                                            int64_t coef_stride, void *out, int64_t out_stride, const double *lower,
+                                            const double *cell_size, const int64_t *num_nodes, const int64_t *code,
+                                            mpf_context *ctx, mpf_stream stream_) {
+  (void)ctx;
+  Grid g;
+  int rc = make_grid(g, ndim, kind, kind == LINEAR ? 1 : (kind == QUADRATIC ? 2 : degree), lower, cell_size,
+                     num_nodes, code);
+  if (rc) return rc;
+  const TileShape ts = tile_shape(kind, degree, ncomp);
+  if (!tiled_supported(kind, degree, ndim, ncomp, g, ts)) {
+    set_error("no tiled kernel for kind %d degree %d, %dD, %lld components on this grid", kind, degree, ndim,
+              (long long)ncomp);
+    return MPF_ERR_UNSUPPORTED;
+  }
+  hipStream_t stream = (hipStream_t)stream_;
+#define DV(T, K, P, C) \
+  run_derivative_tiled<T, K, P, C>(sorted_positions, n, start, g, field, coef, coef_stride, order, out, out_stride, stream)
+  if (dtype == F32) {
+    TILED_CASES(DV, float)
+  } else {
+    TILED_CASES(DV, double)
+  }
+#undef DV
+  set_error("no tiled kernel for this case");
+  return MPF_ERR_UNSUPPORTED;
+}
+
+/* The fixed-point scale of a deposit of these values (rows of ncomp, n markers): scalars
+ * gets [2^k, 2^-k, markers per register pass] as mpf_bin_markers writes it, for bins made
+ * without values (scale_exp overrides k). */
+extern "C" int mpf_fixed_scale(const void *values, int64_t val_stride, int dtype, int64_t n, int64_t ncomp,
+                               int scale_exp, double *scalars, mpf_context *ctx, mpf_stream stream_) {
+  if (dtype != F32 && dtype != F64) {
+    set_error("dtype must be MPF_F32 or MPF_F64, got %d", dtype);
+    return MPF_ERR_ARGUMENT;
+  }
+  if (n < 0 || ncomp < 1 || !values) {
+    set_error("fixed_scale: need n >= 0, ncomp >= 1 and values");
+    return MPF_ERR_ARGUMENT;
+  }
+  hipStream_t stream = (hipStream_t)stream_;
+  if (!ctx) ctx = default_context();
+  if (!ctx) return MPF_ERR_HIP;
+  int rc = prepare_scale(ctx, values, val_stride, dtype, ncomp, n, 1.0, scale_exp, stream);
+  if (rc) return rc;
+  MPF_CHECK(hipMemcpyAsync(scalars, scale_slot(ctx), 24, hipMemcpyDeviceToDevice, stream));
+  return MPF_OK;
 }

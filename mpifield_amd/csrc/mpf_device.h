@@ -168,6 +168,42 @@ __device__ __forceinline__ int axis_stencil(double rel, int64_t num, int code, i
   return spline_axis(rel, num, code, degree, node, w);
 }
 
+// d/dx of the stencil weights along one axis, in cells (times the inverse spacing by the
+// caller).  Zero where the position is clamped to a bounded wall or for degree 0.
+template <int KIND>
+__device__ inline int axis_derivative(double rel, int64_t num, int code, int degree, int64_t *node, double *w,
+                                      double *dw) {
+  const int size = axis_stencil<KIND>(rel, num, code, degree, node, w);
+  const bool clamped = code == 0 && (rel < 0.0 || rel > (double)(num - 1));
+  const int p = KIND == LINEAR ? 1 : (KIND == QUADRATIC ? 2 : degree);
+  if (clamped || p == 0) {
+    for (int a = 0; a < size; ++a) dw[a] = 0.0;
+    return size;
+  }
+  if (p == 1) {
+    // linear and spline p=1: w = (1 - f, f), f = rel - left within the cell
+    dw[0] = -1.0;
+    dw[1] = 1.0;
+    if (KIND == LINEAR && code == 0 && rel > (double)(num - 1)) dw[0] = dw[1] = 0.0;
+    return size;
+  }
+  // the offset from the middle node, recomputed as the stencil did
+  double r = rel;
+  if (code == 1 && !(r >= 0.0 && r < (double)num)) r = r - (double)num * floor(r / (double)num);
+  const double o = p == 2 ? r - (double)(int64_t)(r + 0.5) : r - (double)(int64_t)r;
+  if (p == 2) {
+    dw[0] = -(0.5 - o);
+    dw[1] = -2.0 * o;
+    dw[2] = 0.5 + o;
+  } else {
+    dw[0] = -0.5 * (1.0 - o) * (1.0 - o);
+    dw[1] = 1.5 * o * o - 2.0 * o;
+    dw[2] = -1.5 * o * o + o + 0.5;
+    dw[3] = 0.5 * o * o;
+  }
+  return size;
+}
+
 template <class T>
 __device__ __forceinline__ double load_as_double(const T *p) {
   return (double)(*p);

@@ -87,42 +87,6 @@ __global__ void __launch_bounds__(256) gather_direct(const T *__restrict__ pos, 
   }
 }
 
-// d/dx of the stencil weights along one axis, in cells (times the inverse spacing by the
-// caller).  Zero where the position is clamped to a bounded wall or for degree 0.
-template <int KIND>
-__device__ inline int axis_derivative(double rel, int64_t num, int code, int degree, int64_t *node, double *w,
-                                      double *dw) {
-  const int size = axis_stencil<KIND>(rel, num, code, degree, node, w);
-  const bool clamped = code == 0 && (rel < 0.0 || rel > (double)(num - 1));
-  const int p = KIND == LINEAR ? 1 : (KIND == QUADRATIC ? 2 : degree);
-  if (clamped || p == 0) {
-    for (int a = 0; a < size; ++a) dw[a] = 0.0;
-    return size;
-  }
-  if (p == 1) {
-    // linear and spline p=1: w = (1 - f, f), f = rel - left within the cell
-    dw[0] = -1.0;
-    dw[1] = 1.0;
-    if (KIND == LINEAR && code == 0 && rel > (double)(num - 1)) dw[0] = dw[1] = 0.0;
-    return size;
-  }
-  // the offset from the middle node, recomputed as the stencil did
-  double r = rel;
-  if (code == 1 && !(r >= 0.0 && r < (double)num)) r = r - (double)num * floor(r / (double)num);
-  const double o = p == 2 ? r - (double)(int64_t)(r + 0.5) : r - (double)(int64_t)r;
-  if (p == 2) {
-    dw[0] = -(0.5 - o);
-    dw[1] = -2.0 * o;
-    dw[2] = 0.5 + o;
-  } else {
-    dw[0] = -0.5 * (1.0 - o) * (1.0 - o);
-    dw[1] = 1.5 * o * o - 2.0 * o;
-    dw[2] = -1.5 * o * o + o + 0.5;
-    dw[3] = 0.5 * o * o;
-  }
-  return size;
-}
-
 // out[(d * ncomp + c) * os + m] = d/dx_d of the interpolant of component c at marker m.
 template <class T, int KIND, int NDIM>
 __global__ void __launch_bounds__(256) gather_derivative_direct(const T *__restrict__ pos, int64_t ps, int64_t n,

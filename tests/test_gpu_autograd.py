@@ -123,3 +123,66 @@ def test_cubic_config_shape_runs_in_float32():
     assert torch.isfinite(pos.grad).all() and torch.isfinite(w.grad).all() and torch.isfinite(field.grad).all()
     # d loss / d field = scatter of ones: the total charge
     assert abs(field.grad.sum().item() - n) <= 1e-6 * n
+
+
+TILED_CASES = [("LINEAR", 1), ("LINEAR", 3), ("QUADRATIC", 1), ("QUADRATIC", 3), ("CUBIC", 1)]
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("periodic", [True, False])
+@pytest.mark.parametrize("name,ncomp", TILED_CASES)
+def test_binned_gradient_matches_the_direct_derivative(name, ncomp, periodic, dtype):
+    """The tiled gradient kernel on the bins equals the direct derivative gather contracted
+    with the same coefficients (float64: to rounding; float32: to float32 rounding)."""
+    from mpifield_amd import binning
+
+    print(f"binned gradient {name} C={ncomp} periodic={periodic} {dtype}", flush=True)
+    weighting = getattr(m, name)
+    kind, degree = kind_of(weighting)
+    num = (32, 16, 48)
+    cell = (1 / 32, 1 / 16, 1 / 48)
+    grid = m.BlockGrid.from_geometry([0.0] * 3, list(cell), list(num), [periodic] * 3)
+    g = tuple(grid.geometry)
+    rng = np.random.default_rng(11)
+    n = 50000
+    pos = torch.as_tensor(rng.uniform(-0.1, 1.1, (3, n)), device="cuda", dtype=dtype)
+    field = torch.as_tensor(rng.normal(size=(*num, ncomp)), device="cuda")
+    coef = torch.as_tensor(rng.normal(size=(ncomp, n)), device="cuda", dtype=dtype)
+    bins = binning.bin_markers(pos, None, (kind, degree), g)
+    got = binning.gather_derivative_binned(bins, field, coef)
+    d = AG.gather_derivative(kind, degree, pos.double(), field, g)
+    want = (d * coef.double().unsqueeze(0)).sum(dim=1)
+    scale = want.abs().max().item()
+    tol = 1e-12 if dtype == torch.float64 else 2e-6
+    assert (got.double() - want).abs().max().item() <= tol * scale
+
+
+@pytest.mark.parametrize("mode", ["fixed", "atomic"])
+def test_large_batches_bin_once_and_match_the_direct_gradients(monkeypatch, mode):
+    """gather(scatter) above the binning threshold (tiled kernels, positions binned once) gives
+    the gradients of the direct kernels."""
+    nodes, n = 32, 300000
+    cell = 1.0 / nodes
+    grid = m.BlockGrid.from_geometry([0.0] * 3, [cell] * 3, [nodes] * 3, [True] * 3)
+    rng = np.random.default_rng(21)
+    pos0 = torch.as_tensor(rng.uniform(0, 1, (3, n)), device="cuda")
+    w0 = torch.as_tensor(rng.normal(1.0, 0.2, (1, n)), device="cuda")
+    field0 = gaussians(nodes, cell)
+    a = torch.as_tensor(rng.normal(size=(1, n)), device="cuda")
+
+    def run():
+        pos, w, field = (t.clone().requires_grad_(True) for t in (pos0, w0, field0))
+        with K.deposit_mode(mode):
+            rho = AG.scatter(pos, w, m.CUBIC, grid)
+            g = AG.gather(pos, rho + field, m.CUBIC, grid)
+            (g * a).sum().backward()
+        return g.detach(), pos.grad, w.grad, field.grad
+
+    AG._BINS.clear()
+    tiled = run()
+    assert len(AG._BINS) == 1  # one binning for the scatter, the gather and both backward passes
+    monkeypatch.setattr(K, "BINNED_MIN_MARKERS", 1 << 40)
+    direct = run()
+    for got, want in zip(tiled, direct):
+        scale = want.abs().max().item()
+        assert (got - want).abs().max().item() <= 1e-9 * scale
