@@ -100,24 +100,23 @@ __device__ __forceinline__ int64_t wrap_index(int64_t k, int64_t num, int code) 
 // 3D markers are binned by anchor cell in tile order: the cells of a tile of
 // TILE_X x TILE_Y x TILE_Z (x-major, z fastest) are consecutive bins and tiles follow each
 // other in x-major order, so the markers of a tile (and of each x-slab of it) are one
-// contiguous run of the binned arrays.  Partial tiles at the grid's end keep their empty
-// bins.  1D and 2D markers are binned in row-major cell order.
+// contiguous run of the binned arrays.  2D markers likewise in tiles of TILE2 x TILE2
+// cells (x-major, y fastest).  Partial tiles at the grid's end keep their empty bins.  1D
+// markers are binned in cell order.
 constexpr int TILE_X = 8, TILE_Y = 8, TILE_Z = 16, TILE_CELLS = TILE_X * TILE_Y * TILE_Z;
+constexpr int TILE2 = 32, TILE2_CELLS = TILE2 * TILE2;
 
-// Bins of a grid (before the dead bin): tile-padded cells in 3D, cells otherwise.
+// Bins of a grid (before the dead bin): tile-padded cells in 2D and 3D, cells in 1D.
 __host__ __device__ __forceinline__ int64_t bins_of(int ndim, const int64_t *num) {
   if (ndim == 3) {
     const int64_t ntx = (num[0] + TILE_X - 1) / TILE_X, nty = (num[1] + TILE_Y - 1) / TILE_Y,
                   ntz = (num[2] + TILE_Z - 1) / TILE_Z;
     return ntx * nty * ntz * TILE_CELLS;
   }
-  int64_t c = 1;
-  for (int d = 0; d < ndim; ++d) c *= num[d];
-  return c;
+  if (ndim == 2) return ((num[0] + TILE2 - 1) / TILE2) * ((num[1] + TILE2 - 1) / TILE2) * TILE2_CELLS;
+  return num[0];
 }
 
-// The bin of one marker: its anchor cell's key, or dead_key when every value is zero (a
-// gather's markers, ncomp == 0, are all live).  Folds |value| into vmax for the scale.
 template <class T, int KIND, int P, int NDIM>
 __device__ __forceinline__ uint32_t marker_key(const T *__restrict__ pos, int64_t ps, const T *__restrict__ val,
                                                int64_t vs, int ncomp, int64_t m, const Grid &g, uint32_t dead_key,
@@ -143,11 +142,12 @@ __device__ __forceinline__ uint32_t marker_key(const T *__restrict__ pos, int64_
     const int64_t nty = (g.num[1] + TILE_Y - 1) / TILE_Y, ntz = (g.num[2] + TILE_Z - 1) / TILE_Z;
     const int64_t tile = ((int64_t)(cell[0] / TILE_X) * nty + cell[1] / TILE_Y) * ntz + cell[2] / TILE_Z;
     return (uint32_t)(tile * TILE_CELLS + ((cell[0] % TILE_X) * TILE_Y + cell[1] % TILE_Y) * TILE_Z + cell[2] % TILE_Z);
+  } else if constexpr (NDIM == 2) {
+    const int64_t nty = (g.num[1] + TILE2 - 1) / TILE2;
+    const int64_t tile = (int64_t)(cell[0] / TILE2) * nty + cell[1] / TILE2;
+    return (uint32_t)(tile * TILE2_CELLS + (cell[0] % TILE2) * TILE2 + cell[1] % TILE2);
   } else {
-    int64_t k = 0;
-#pragma unroll
-    for (int d = 0; d < NDIM; ++d) k = k * g.num[d] + cell[d];
-    return (uint32_t)k;
+    return (uint32_t)cell[0];
   }
 }
 
@@ -1471,6 +1471,8 @@ static int launch_group_and_cells(const void *pos, int64_t ps, int ndim, const v
   hipLaunchKernelGGL((bin_group_kernel<T, D, C>), dim3(blocks), dim3(BIN_THREADS), 0, stream, key, (const T *)pos, \
                      ps, ndim, (const T *)val, vs, ncomp, n, shift, groups, chunk, slots, records)
     if (ndim == 3 && ncomp == 1) GROUP(3, 1);
+    else if (ndim == 2 && ncomp == 3) GROUP(2, 3);
+    else if (ndim == 3 && ncomp == 4) GROUP(3, 4);
     else if (ndim == 3 && ncomp == 3) GROUP(3, 3);
     else if (ndim == 3 && ncomp == 0) GROUP(3, 0);
     else if (ndim == 2 && ncomp == 1) GROUP(2, 1);
@@ -1492,6 +1494,7 @@ static int launch_group_and_cells(const void *pos, int64_t ps, int ndim, const v
   }
   if (shift <= 12) {
     STAGED(3, 1) STAGED(3, 3) STAGED(3, 0) STAGED(2, 1) STAGED(2, 2) STAGED(2, 0) STAGED(1, 1) STAGED(1, 0)
+    STAGED(2, 3) STAGED(3, 4)  // path deposits: start binned with (charge, end)
     hipLaunchKernelGGL((bin_cells_direct<T, BIN_KEYS_CAP>), grid, block, 0, stream, records, ndim, ncomp, n, shift,
                        groups, blocks, dead_key, slots, (T *)spos, (T *)sval, order, start);
   } else {

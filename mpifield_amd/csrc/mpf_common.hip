@@ -303,7 +303,7 @@ mpf_context *mpf_context_create(int device) {
 void mpf_context_destroy(mpf_context *ctx) {
   if (!ctx) return;
   (void)hipDeviceSynchronize();
-  mpf::Buffer *all[] = {&ctx->accum, &ctx->scalars, &ctx->sort_keys, &ctx->sort_tmp, &ctx->offsets, &ctx->staging};
+  mpf::Buffer *all[] = {&ctx->accum, &ctx->scalars, &ctx->sort_keys, &ctx->sort_tmp, &ctx->offsets, &ctx->staging, &ctx->path_in, &ctx->path_sorted};
   for (mpf::Buffer *b : all)
     if (b->ptr) (void)hipFree(b->ptr);
   delete ctx;

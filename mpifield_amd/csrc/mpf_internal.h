@@ -28,6 +28,8 @@ struct mpf_context {
   mpf::Buffer sort_tmp;
   mpf::Buffer offsets;
   mpf::Buffer staging;   // markers in group order while binning
+  mpf::Buffer path_in;   // a path deposit's charge and end rows, to be binned with the start
+  mpf::Buffer path_sorted;  // the binned paths: start, rows, order, bin offsets, scalars
 };
 
 namespace mpf {

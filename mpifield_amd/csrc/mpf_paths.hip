@@ -150,7 +150,7 @@ __device__ __forceinline__ int64_t floor_int(double x) {
 
 // spline_edges_pyccel.py:86-138: the `degree` edges along an axis around rel, weights
 // B_{p-1}(rel - k - 1/2); on a bounded axis an edge past a wall does not exist.
-__device__ inline void along_stencil(double rel, int64_t num, int code, int degree, int64_t *edge, double *w,
+__device__ __forceinline__ void along_stencil(double rel, int64_t num, int code, int degree, int64_t *edge, double *w,
                                      int *exists) {
   if (code == 2) {
     clamped_cell(rel, num, degree, 1, edge, w);
@@ -195,7 +195,7 @@ __device__ inline void along_stencil(double rel, int64_t num, int code, int degr
   }
 }
 
-__device__ inline void gauss_rule(int count, double *point, double *weight) {
+__device__ __forceinline__ void gauss_rule(int count, double *point, double *weight) {
   if (count == 1) {
     point[0] = 0.5;
     weight[0] = 1.0;
@@ -234,140 +234,516 @@ __device__ inline void gauss_rule(int count, double *point, double *weight) {
   }
 }
 
-// spline_edges_pyccel.py:180-230 path_cuts, with the reference's insertion sort.
+// The cut times of one stretch [a0, a1] (spline_edges_pyccel.py:180-230 path_cuts): the
+// knot planes (shifted by `shift` for even degrees) and, on bounded axes, the walls, each
+// kept when strictly inside (0, 1).  A stretch spans less than 4 cells per axis, so it
+// crosses at most 4 planes per axis; every candidate has a fixed register slot (2.0 when
+// absent), with the reference's arithmetic: plane = floor(low - offset) + 1 + offset,
+// stepped by 1.0, t = (plane - a0) / delta.
 template <int NDIM>
-__device__ inline int path_cuts(const double *r0, const double *r1, double shift, const Grid &g, double *cuts) {
-  int count = 1;
-  cuts[0] = 0.0;
+struct Cuts {
+  static constexpr int SLOTS = NDIM * 6;
+  double t[SLOTS];
+};
+
+template <int NDIM>
+__device__ __forceinline__ void stretch_cuts(const double *a0, const double *a1, double shift, const Grid &g,
+                                             Cuts<NDIM> &c) {
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
-    const double delta = r1[d] - r0[d];
-    if (delta == 0.0) continue;
-    const double low = delta > 0.0 ? r0[d] : r1[d];
-    const double high = delta > 0.0 ? r1[d] : r0[d];
+    const double delta = a1[d] - a0[d];
+    const double low = delta > 0.0 ? a0[d] : a1[d];
+    const double high = delta > 0.0 ? a1[d] : a0[d];
     const double offset = g.code[d] == 2 ? 0.0 : shift;
     double plane = (double)floor_int(low - offset) + 1.0 + offset;
-    while (plane < high) {
-      const double t = (plane - r0[d]) / delta;
-      if (t > 0.0 && t < 1.0 && count < 31) cuts[count++] = t;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double t = (plane - a0[d]) / delta;
+      c.t[d * 6 + k] = (delta != 0.0 && plane < high && t > 0.0 && t < 1.0) ? t : 2.0;
       plane += 1.0;
     }
-    if (g.code[d] != 1) {
-      double t = (0.0 - r0[d]) / delta;
-      if (t > 0.0 && t < 1.0 && count < 31) cuts[count++] = t;
-      t = ((double)g.num[d] - 1.0 - r0[d]) / delta;
-      if (t > 0.0 && t < 1.0 && count < 31) cuts[count++] = t;
+    double tw0 = 2.0, tw1 = 2.0;
+    if (g.code[d] != 1 && delta != 0.0) {
+      tw0 = (0.0 - a0[d]) / delta;
+      tw1 = ((double)g.num[d] - 1.0 - a0[d]) / delta;
     }
+    c.t[d * 6 + 4] = (tw0 > 0.0 && tw0 < 1.0) ? tw0 : 2.0;
+    c.t[d * 6 + 5] = (tw1 > 0.0 && tw1 < 1.0) ? tw1 : 2.0;
   }
-  cuts[count++] = 1.0;
-  for (int i = 2; i < count - 1; ++i) {
-    const double value = cuts[i];
-    int j = i - 1;
-    while (j > 0 && cuts[j] > value) {
-      cuts[j + 1] = cuts[j];
-      --j;
-    }
-    cuts[j + 1] = value;
-  }
-  return count;
 }
 
-template <class T, int NDIM, int MODE>
-__global__ void __launch_bounds__(128) spline_paths_kernel(const T *__restrict__ st, int64_t ss,
+// The smallest cut strictly after `after` (1.0 when none): the pieces of the stretch are
+// the intervals between consecutive distinct cut times, as after the reference's sort
+// (equal times give the empty pieces it skips).
+template <int NDIM>
+__device__ __forceinline__ double next_cut(const Cuts<NDIM> &c, double after) {
+  double best = 1.0;
+#pragma unroll
+  for (int i = 0; i < Cuts<NDIM>::SLOTS; ++i) best = (c.t[i] > after && c.t[i] < best) ? c.t[i] : best;
+  return best;
+}
+
+// One path's spline-edge deposit (deposit_paths_spline_{1,2,3}d_pyccel.py; the cuts above,
+// Gauss-Legendre with (p-1+(ndim-1)p)//2+1 points per piece).  sink(idx, axis, v) receives
+// every contribution: idx[d] is the node (edge) index along axis d, already wrapped or
+// clamped by the stencils.
+template <int NDIM, int DEG, class Sink>
+__device__ __forceinline__ void spline_path(const double *r0, const double *r1, double charge, const Grid &g,
+                                            Sink &sink) {
+  constexpr double SHIFT = DEG % 2 == 0 ? 0.5 : 0.0;
+  constexpr int POINTS = (DEG - 1 + (NDIM - 1) * DEG) / 2 + 1;
+  double point[5], gw[5];
+  gauss_rule(POINTS, point, gw);
+  double extent = 0.0;
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    const double span = r1[d] > r0[d] ? r1[d] - r0[d] : r0[d] - r1[d];
+    if (span > extent) extent = span;
+  }
+  const int64_t stretches = (int64_t)(extent / 4.0) + 1;
+  for (int64_t st = 0; st < stretches; ++st) {
+    double a0[NDIM], a1[NDIM];
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+      a0[d] = r0[d] + (r1[d] - r0[d]) * (double)st / (double)stretches;
+      a1[d] = r0[d] + (r1[d] - r0[d]) * (double)(st + 1) / (double)stretches;
+    }
+    Cuts<NDIM> cuts;
+    stretch_cuts<NDIM>(a0, a1, SHIFT, g, cuts);
+    double t0 = 0.0;
+    while (t0 < 1.0) {
+      const double t1 = next_cut<NDIM>(cuts, t0);
+      double p0[NDIM], stp[NDIM];
+#pragma unroll
+      for (int d = 0; d < NDIM; ++d) {
+        double begin = a0[d] + t0 * (a1[d] - a0[d]);
+        double finish = a0[d] + t1 * (a1[d] - a0[d]);
+        if (g.code[d] != 1) {
+          const double top = (double)(g.num[d] - 1);
+          if (begin < 0.0) begin = 0.0;
+          if (begin > top) begin = (double)g.num[d] - 1.0;
+          if (finish < 0.0) finish = 0.0;
+          if (finish > top) finish = (double)g.num[d] - 1.0;
+        }
+        p0[d] = begin;
+        stp[d] = finish - begin;
+      }
+#pragma unroll
+      for (int gp = 0; gp < POINTS; ++gp) {
+        double rel[NDIM];
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) rel[d] = p0[d] + point[gp] * stp[d];
+        int64_t node[NDIM][DEG + 1];
+        double wn[NDIM][DEG + 1];
+        int64_t edge[NDIM][DEG > 0 ? DEG : 1];
+        double we[NDIM][DEG > 0 ? DEG : 1];
+        int ex[NDIM][DEG > 0 ? DEG : 1];
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) {
+          spline_axis(rel[d], g.num[d], g.code[d], DEG, node[d], wn[d]);
+          along_stencil(rel[d], g.num[d], g.code[d], DEG, edge[d], we[d], ex[d]);
+        }
+#pragma unroll
+        for (int axis = 0; axis < NDIM; ++axis) {
+          const double amount = charge * gw[gp] * stp[axis];
+          if (amount == 0.0) continue;
+          constexpr int S = DEG + 1;
+          const int s0 = axis == 0 ? DEG : S, s1 = NDIM > 1 ? (axis == 1 ? DEG : S) : 1,
+                    s2 = NDIM > 2 ? (axis == 2 ? DEG : S) : 1;
+#pragma unroll
+          for (int a = 0; a < S; ++a) {
+            if (a >= s0) continue;
+            if (axis == 0 && !ex[0][a]) continue;
+            const double va = amount * (axis == 0 ? we[0][a] : wn[0][a]);
+            const int64_t ia = axis == 0 ? edge[0][a] : node[0][a];
+#pragma unroll
+            for (int b = 0; b < S; ++b) {
+              if (b >= s1) continue;
+              if constexpr (NDIM >= 2) {
+                if (axis == 1 && !ex[1][b]) continue;
+              }
+              const double vb = NDIM >= 2 ? va * (axis == 1 ? we[NDIM > 1 ? 1 : 0][b] : wn[NDIM > 1 ? 1 : 0][b]) : va;
+              const int64_t ib = NDIM >= 2 ? (axis == 1 ? edge[NDIM > 1 ? 1 : 0][b] : node[NDIM > 1 ? 1 : 0][b]) : 0;
+#pragma unroll
+              for (int e = 0; e < S; ++e) {
+                if (e >= s2) continue;
+                if constexpr (NDIM == 3) {
+                  if (axis == 2 && !ex[2][e]) continue;
+                }
+                const double v = NDIM == 3 ? vb * (axis == 2 ? we[NDIM - 1][e] : wn[NDIM - 1][e]) : vb;
+                const int64_t ie = NDIM == 3 ? (axis == 2 ? edge[NDIM - 1][e] : node[NDIM - 1][e]) : 0;
+                const int64_t idx[3] = {ia, ib, ie};
+                sink(idx, axis, v);
+              }
+            }
+          }
+        }
+      }
+      t0 = t1;
+    }
+  }
+}
+
+// The path's rel coordinates and whether they are finite.
+template <class T, int NDIM>
+__device__ __forceinline__ bool path_rel(const T *st, int64_t ss, const T *en, int64_t es, int64_t p, const Grid &g,
+                                         double *r0, double *r1) {
+  bool finite = true;
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    const double s = (double)st[d * ss + p], e = (double)en[d * es + p];
+    finite = finite && (s - s == 0.0) && (e - e == 0.0);
+    r0[d] = (s - g.lower[d]) * g.inverse[d];
+    r1[d] = (e - g.lower[d]) * g.inverse[d];
+  }
+  return finite;
+}
+
+template <int NDIM, int MODE>
+struct GlobalSink {
+  double *edges;
+  unsigned long long *accum;
+  double scale;
+  const Grid *g;
+  __device__ __forceinline__ void operator()(const int64_t *idx, int axis, double v) {
+    const int64_t at = (idx[0] * g->stride[0] + (NDIM > 1 ? idx[1] * g->stride[1] : 0) +
+                        (NDIM > 2 ? idx[2] * g->stride[2] : 0)) * NDIM + axis;
+    deposit<MODE>(edges, accum, at, v, scale);
+  }
+};
+
+// One work-item per path; every contribution a global float64 atomic (MODE_ATOMIC) or an
+// int64 fixed-point atomic (MODE_FIXED).
+template <class T, int NDIM, int DEG, int MODE>
+__global__ void __launch_bounds__(256) spline_paths_kernel(const T *__restrict__ st, int64_t ss,
                                                            const T *__restrict__ en, int64_t es,
-                                                           const T *__restrict__ q, int64_t n, Grid g, int degree,
+                                                           const T *__restrict__ q, int64_t n, Grid g,
                                                            double *__restrict__ edges,
                                                            unsigned long long *__restrict__ accum,
                                                            const double *__restrict__ slot) {
-  const double scale = MODE == MODE_FIXED ? slot[0] : 1.0;
-  const double shift = degree % 2 == 0 ? 0.5 : 0.0;
-  const int num_points = (degree - 1 + (NDIM - 1) * degree) / 2 + 1;
-  double point[5], gw[5];
-  gauss_rule(num_points, point, gw);
+  GlobalSink<NDIM, MODE> sink{edges, accum, MODE == MODE_FIXED ? slot[0] : 1.0, &g};
   for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const double charge = (double)q[p];
     if (charge == 0.0) continue;
-    bool finite = true;
-    double r0[3] = {0, 0, 0}, r1[3] = {0, 0, 0}, extent = 0.0;
+    double r0[NDIM], r1[NDIM];
+    if (!path_rel<T, NDIM>(st, ss, en, es, p, g, r0, r1)) continue;
+    spline_path<NDIM, DEG>(r0, r1, charge, g, sink);
+  }
+}
+
+// ---- tiled spline-edge deposit ----------------------------------------------------------
+//
+// Paths binned by the cell of their start point (2D tiles of 32 x 32 cells, 3D tiles of
+// 8 x 8 x 16).  One workgroup per tile accumulates the contributions of the tile's paths in
+// int64 fixed point in LDS over the tile's edges plus a halo of PATH_HALO cells on every
+// side (the quanta are those of the one-path-per-work-item fixed-point deposit: the same
+// integers, summed exactly); a contribution outside the footprint (a long path) goes to the
+// global accumulator directly.  The tile then adds its nonzero LDS words into the global
+// int64 accumulator (integer atomics: exact in any order), and finalize_fixed converts.
+// Each work-item walks a contiguous run of the tile's paths, so that the lanes of a wave
+// work on different cells at any time (paths of one cell would hit the same LDS words).
+constexpr int PATH_HALO_2D = 4, PATH_HALO_3D = 3;
+
+template <int NDIM>
+struct PathTile {
+  static constexpr int TX = NDIM == 3 ? 8 : 32, TY = NDIM == 3 ? 8 : 32, TZ = NDIM == 3 ? 16 : 1;
+  static constexpr int H = NDIM == 3 ? PATH_HALO_3D : PATH_HALO_2D;
+  static constexpr int LX = TX + 2 * H, LY = TY + 2 * H, LZ = NDIM == 3 ? TZ + 2 * H : 1;
+  static constexpr int CELLS = TX * TY * TZ;
+  static constexpr int WORDS = LX * LY * LZ * NDIM;
+};
+
+
+
+// ---- one path into a tile's LDS accumulator, with int32 local indices --------------------
+//
+// The same arithmetic as spline_path with spline_axis / along_stencil (the rel
+// wrap of a periodic axis, the clamp of a bounded one, int(r + 1/2) or int(r), the weights,
+// the cut times), but node and edge indices stay unwrapped int32 numbers relative to the
+// tile's footprint, folded once by +-num, and a cut time is divided out only when its
+// plane or wall exists.
+
+// The LDS accumulator as an LDS pointer (address space 3), so that its adds are LDS atomics.
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+
+struct AxisTile {
+  int num, code, base, len;  // nodes, code, first footprint node (unwrapped), footprint length
+};
+
+// The footprint slot of unwrapped node k (-1 when outside) and its wrapped or clamped node.
+__device__ __forceinline__ void fold_node(int k, const AxisTile &ax, int &slot, int &node) {
+  if (ax.code == 1) {
+    node = k < 0 ? k + ax.num : (k >= ax.num ? k - ax.num : k);
+    int l = k - ax.base;
+    l = l < 0 ? l + ax.num : (l >= ax.num ? l - ax.num : l);
+    slot = l < ax.len ? l : -1;
+  } else {
+    node = k < 0 ? 0 : (k > ax.num - 1 ? ax.num - 1 : k);
+    const int l = node - ax.base;
+    slot = l >= 0 && l < ax.len ? l : -1;
+  }
+}
+
+__device__ __forceinline__ double wrap_rel(double r, int num) {
+  return (r >= 0.0 && r < (double)num) ? r : r - (double)num * floor(r / (double)num);
+}
+
+// Node stencil of degree DEG at rel on one axis (spline_axis).
+template <int DEG>
+__device__ __forceinline__ void node_axis(double rel, const AxisTile &ax, int *slot, int *node, double *w) {
+  double r = rel;
+  if (ax.code == 1) {
+    r = wrap_rel(r, ax.num);
+  } else {
+    if (r < 0.0) r = 0.0;
+    if (r > (double)(ax.num - 1)) r = (double)ax.num - 1.0;
+  }
+  const int middle = (DEG % 2 == 0) ? (int)(r + 0.5) : (int)r;
+  const double o = r - (double)middle;
+  if (DEG == 1) {
+    w[0] = 1.0 - o;
+    w[1] = o;
+  } else if (DEG == 2) {
+    w[0] = 0.5 * (0.5 - o) * (0.5 - o);
+    w[1] = 0.75 - o * o;
+    w[2] = 0.5 * (0.5 + o) * (0.5 + o);
+  } else {
+    const double cm = 1.0 - o, sq = o * o, cu = sq * o;
+    w[0] = cm * cm * cm / 6.0;
+    w[1] = (3.0 * cu - 6.0 * sq + 4.0) / 6.0;
+    w[2] = (-3.0 * cu + 3.0 * sq + 3.0 * o + 1.0) / 6.0;
+    w[3] = cu / 6.0;
+  }
+#pragma unroll
+  for (int a = 0; a <= DEG; ++a) fold_node(middle - DEG / 2 + a, ax, slot[a], node[a]);
+}
+
+// Edge stencil of degree DEG at rel on one axis (along_stencil): DEG edges, weights
+// B_{DEG-1}(rel - k - 1/2); an edge past a bounded wall does not exist (slot -2).
+template <int DEG>
+__device__ __forceinline__ void edge_axis(double rel, const AxisTile &ax, int *slot, int *edge, double *w) {
+  constexpr int LOWER = DEG - 1;
+  double r = rel;
+  if (ax.code == 1) r = wrap_rel(r, ax.num);
+  const double shifted = r - 0.5;
+  int middle;
+  if (LOWER % 2 == 0) {
+    middle = (int)floor_int(shifted + 0.5);
+    if (ax.code == 0 && middle > ax.num - 2) middle = ax.num - 2;
+  } else {
+    middle = (int)floor_int(shifted);
+  }
+  const double o = shifted - (double)middle;
+  if (LOWER == 0) {
+    w[0] = 1.0;
+  } else if (LOWER == 1) {
+    w[0] = 1.0 - o;
+    w[1] = o;
+  } else {
+    w[0] = 0.5 * (0.5 - o) * (0.5 - o);
+    w[1] = 0.75 - o * o;
+    w[2] = 0.5 * (0.5 + o) * (0.5 + o);
+  }
+#pragma unroll
+  for (int a = 0; a < DEG; ++a) {
+    const int k = middle - LOWER / 2 + a;
+    if (ax.code == 1) {
+      fold_node(k, ax, slot[a], edge[a]);
+    } else if (k < 0 || k > ax.num - 2) {
+      slot[a] = -2;
+      edge[a] = 0;
+    } else {
+      edge[a] = k;
+      const int l = k - ax.base;
+      slot[a] = l >= 0 && l < ax.len ? l : -1;
+    }
+  }
+}
+
+template <int NDIM, int DEG>
+__device__ __forceinline__ void path_into_tile(const double *r0, const double *r1, double charge, const Grid &g,
+                                               const AxisTile *ax, lds_u64 *lds,
+                                               unsigned long long *__restrict__ accum, double scale) {
+  using PT = PathTile<NDIM>;
+  constexpr double SHIFT = DEG % 2 == 0 ? 0.5 : 0.0;
+  constexpr int POINTS = (DEG - 1 + (NDIM - 1) * DEG) / 2 + 1;
+  constexpr int S = DEG + 1;
+  double point[5], gw[5];
+  gauss_rule(POINTS, point, gw);
+  double extent = 0.0;
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    const double span = r1[d] > r0[d] ? r1[d] - r0[d] : r0[d] - r1[d];
+    if (span > extent) extent = span;
+  }
+  const int64_t stretches = (int64_t)(extent / 4.0) + 1;
+  for (int64_t st = 0; st < stretches; ++st) {
+    double a0[NDIM], a1[NDIM];
 #pragma unroll
     for (int d = 0; d < NDIM; ++d) {
-      const double s = (double)st[d * ss + p], e = (double)en[d * es + p];
-      finite = finite && (s - s == 0.0) && (e - e == 0.0);
-      r0[d] = (s - g.lower[d]) * g.inverse[d];
-      r1[d] = (e - g.lower[d]) * g.inverse[d];
-      const double span = r1[d] > r0[d] ? r1[d] - r0[d] : r0[d] - r1[d];
-      if (span > extent) extent = span;
+      double f0 = (r1[d] - r0[d]) * (double)st, f1 = (r1[d] - r0[d]) * (double)(st + 1);
+      if (stretches != 1) {  // a division by 1.0 is exact
+        f0 = f0 / (double)stretches;
+        f1 = f1 / (double)stretches;
+      }
+      a0[d] = r0[d] + f0;
+      a1[d] = r0[d] + f1;
     }
-    if (!finite) continue;
-    const int64_t stretches = (int64_t)(extent / 4.0) + 1;
-    for (int64_t s = 0; s < stretches; ++s) {
-      double a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0}, cuts[32];
+    // the cut times: a slot per candidate plane or wall, divided out only when it exists
+    double cut[NDIM * 6];
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+      const double delta = a1[d] - a0[d];
+      const double low = delta > 0.0 ? a0[d] : a1[d];
+      const double high = delta > 0.0 ? a1[d] : a0[d];
+      const double offset = g.code[d] == 2 ? 0.0 : SHIFT;
+      double plane = (double)floor_int(low - offset) + 1.0 + offset;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        double t = 2.0;
+        if (delta != 0.0 && plane < high) {
+          t = (plane - a0[d]) / delta;
+          t = t > 0.0 && t < 1.0 ? t : 2.0;
+        }
+        cut[d * 6 + k] = t;
+        plane += 1.0;
+      }
+      double tw0 = 2.0, tw1 = 2.0;
+      if (g.code[d] != 1 && delta != 0.0 && (low < 0.0 || high > (double)g.num[d] - 1.0)) {
+        tw0 = (0.0 - a0[d]) / delta;
+        tw1 = ((double)g.num[d] - 1.0 - a0[d]) / delta;
+        tw0 = tw0 > 0.0 && tw0 < 1.0 ? tw0 : 2.0;
+        tw1 = tw1 > 0.0 && tw1 < 1.0 ? tw1 : 2.0;
+      }
+      cut[d * 6 + 4] = tw0;
+      cut[d * 6 + 5] = tw1;
+    }
+    double t0 = 0.0;
+    while (t0 < 1.0) {
+      double t1 = 1.0;
+#pragma unroll
+      for (int i = 0; i < NDIM * 6; ++i) t1 = (cut[i] > t0 && cut[i] < t1) ? cut[i] : t1;
+      double p0[NDIM], stp[NDIM];
 #pragma unroll
       for (int d = 0; d < NDIM; ++d) {
-        a0[d] = r0[d] + (r1[d] - r0[d]) * (double)s / (double)stretches;
-        a1[d] = r0[d] + (r1[d] - r0[d]) * (double)(s + 1) / (double)stretches;
+        double begin = a0[d] + t0 * (a1[d] - a0[d]);
+        double finish = a0[d] + t1 * (a1[d] - a0[d]);
+        if (g.code[d] != 1) {
+          const double top = (double)(g.num[d] - 1);
+          if (begin < 0.0) begin = 0.0;
+          if (begin > top) begin = (double)g.num[d] - 1.0;
+          if (finish < 0.0) finish = 0.0;
+          if (finish > top) finish = (double)g.num[d] - 1.0;
+        }
+        p0[d] = begin;
+        stp[d] = finish - begin;
       }
-      const int num_cuts = path_cuts<NDIM>(a0, a1, shift, g, cuts);
-      for (int piece = 0; piece < num_cuts - 1; ++piece) {
-        const double t0 = cuts[piece], t1 = cuts[piece + 1];
-        if (t1 <= t0) continue;
-        double p0[3] = {0, 0, 0}, stp[3] = {0, 0, 0};
+#pragma unroll
+      for (int gp = 0; gp < POINTS; ++gp) {
+        int ns[NDIM][S], nn[NDIM][S], es[NDIM][DEG], en[NDIM][DEG];
+        double wn[NDIM][S], we[NDIM][DEG];
 #pragma unroll
         for (int d = 0; d < NDIM; ++d) {
-          double begin = a0[d] + t0 * (a1[d] - a0[d]);
-          double finish = a0[d] + t1 * (a1[d] - a0[d]);
-          if (g.code[d] != 1) {
-            const double top = (double)(g.num[d] - 1);
-            if (begin < 0.0) begin = 0.0;
-            if (begin > top) begin = (double)g.num[d] - 1.0;
-            if (finish < 0.0) finish = 0.0;
-            if (finish > top) finish = (double)g.num[d] - 1.0;
-          }
-          p0[d] = begin;
-          stp[d] = finish - begin;
+          const double rel = p0[d] + point[gp] * stp[d];
+          node_axis<DEG>(rel, ax[d], ns[d], nn[d], wn[d]);
+          edge_axis<DEG>(rel, ax[d], es[d], en[d], we[d]);
         }
-        for (int gp = 0; gp < num_points; ++gp) {
-          double rel[3];
 #pragma unroll
-          for (int d = 0; d < NDIM; ++d) rel[d] = p0[d] + point[gp] * stp[d];
+        for (int axis = 0; axis < NDIM; ++axis) {
+          const double amount = charge * gw[gp] * stp[axis];
+          if (amount == 0.0) continue;
 #pragma unroll
-          for (int axis = 0; axis < NDIM; ++axis) {
-            const double amount = charge * gw[gp] * stp[axis];
-            if (amount == 0.0) continue;
-            int64_t idx[3][4];
-            double w[3][4];
-            int ex[4] = {1, 1, 1, 1}, size[3] = {1, 1, 1};
-            idx[1][0] = idx[2][0] = 0;
-            w[1][0] = w[2][0] = 1.0;
+          for (int a = 0; a < (axis == 0 ? DEG : S); ++a) {
+            const int sa = axis == 0 ? es[0][a] : ns[0][a];
+            if (sa == -2) continue;
+            const double va = amount * (axis == 0 ? we[0][a] : wn[0][a]);
+            const int ia = axis == 0 ? en[0][a] : nn[0][a];
 #pragma unroll
-            for (int d = 0; d < NDIM; ++d) {
-              if (d == axis) {
-                along_stencil(rel[d], g.num[d], g.code[d], degree, idx[d], w[d], ex);
-                size[d] = degree;
-              } else {
-                spline_axis(rel[d], g.num[d], g.code[d], degree, idx[d], w[d]);
-                size[d] = degree + 1;
-              }
-            }
-            for (int a = 0; a < size[0]; ++a) {
-              if (axis == 0 && !ex[a]) continue;
-              for (int b = 0; b < size[1]; ++b) {
-                if (axis == 1 && !ex[b]) continue;
-                for (int e = 0; e < size[2]; ++e) {
-                  if (axis == 2 && !ex[e]) continue;
-                  double v = amount * w[0][a];
-                  if (NDIM >= 2) v = v * w[1][b];
-                  if (NDIM == 3) v = v * w[2][e];
-                  const int64_t at =
-                      (idx[0][a] * g.stride[0] + idx[1][b] * g.stride[1] + idx[2][e] * g.stride[2]) * NDIM + axis;
-                  deposit<MODE>(edges, accum, at, v, scale);
+            for (int b = 0; b < (NDIM < 2 ? 1 : (axis == 1 ? DEG : S)); ++b) {
+              const int sb = NDIM < 2 ? 0 : (axis == 1 ? es[NDIM - 1 > 0 ? 1 : 0][b] : ns[NDIM - 1 > 0 ? 1 : 0][b]);
+              if (sb == -2) continue;
+              const double vb = NDIM < 2 ? va : va * (axis == 1 ? we[NDIM - 1 > 0 ? 1 : 0][b] : wn[NDIM - 1 > 0 ? 1 : 0][b]);
+              const int ib = NDIM < 2 ? 0 : (axis == 1 ? en[NDIM - 1 > 0 ? 1 : 0][b] : nn[NDIM - 1 > 0 ? 1 : 0][b]);
+#pragma unroll
+              for (int e = 0; e < (NDIM < 3 ? 1 : (axis == 2 ? DEG : S)); ++e) {
+                const int se = NDIM < 3 ? 0 : (axis == 2 ? es[NDIM - 1][e] : ns[NDIM - 1][e]);
+                if (se == -2) continue;
+                const double v = NDIM < 3 ? vb : vb * (axis == 2 ? we[NDIM - 1][e] : wn[NDIM - 1][e]);
+                const int ie = NDIM < 3 ? 0 : (axis == 2 ? en[NDIM - 1][e] : nn[NDIM - 1][e]);
+                const unsigned long long f = to_fixed(v * scale);
+                if (sa >= 0 && sb >= 0 && se >= 0) {
+                  __hip_atomic_fetch_add(lds + ((sa * PT::LY + sb) * PT::LZ + se) * NDIM + axis, f,
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {
+                  const int64_t at = ((int64_t)ia * g.stride[0] + (NDIM > 1 ? (int64_t)ib * g.stride[1] : 0) +
+                                      (NDIM > 2 ? (int64_t)ie * g.stride[2] : 0)) * NDIM + axis;
+                  atomicAdd(accum + at, f);
                 }
               }
             }
           }
         }
       }
+      t0 = t1;
     }
+  }
+}
+
+// At most 128 VGPRs (four waves per SIMD): the work-items walk long chains of dependent
+// float64 arithmetic, and more waves hide more of it.
+template <class T, int NDIM, int DEG, int BLOCK>
+__global__ void __launch_bounds__(BLOCK, 4) spline_paths_tiled(const T *__restrict__ st, const T *__restrict__ rows,
+                                                            int64_t n, const uint32_t *__restrict__ start, Grid g,
+                                                            int nty, int ntz, unsigned long long *__restrict__ accum,
+                                                            const double *__restrict__ slot) {
+  using PT = PathTile<NDIM>;
+  __shared__ unsigned long long lds[PT::WORDS];
+  const int tile = blockIdx.x, t = threadIdx.x;
+  const int tz = NDIM == 3 ? tile % ntz : 0, ty = NDIM == 3 ? (tile / ntz) % nty : tile % nty;
+  const int tx = NDIM == 3 ? tile / (ntz * nty) : tile / nty;
+  const int64_t base[3] = {(int64_t)tx * PT::TX - PT::H, (int64_t)ty * PT::TY - PT::H, (int64_t)tz * PT::TZ - PT::H};
+  const double scale = slot[0];
+  AxisTile ax[NDIM];
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d)
+    ax[d] = AxisTile{(int)g.num[d], g.code[d], (int)base[d], d == 0 ? PT::LX : (d == 1 ? PT::LY : PT::LZ)};
+  for (int i = t; i < PT::WORDS; i += BLOCK) lds[i] = 0ull;
+  const uint32_t first = start[(int64_t)tile * PT::CELLS], last = start[(int64_t)(tile + 1) * PT::CELLS];
+  __syncthreads();
+  const uint32_t count = last - first, chunk = (count + BLOCK - 1) / BLOCK;
+  const uint32_t lo = first + t * chunk, hi = lo + chunk < last ? lo + chunk : last;
+  for (uint32_t p = lo; p < hi; ++p) {
+    const double charge = (double)rows[p];
+    if (charge == 0.0) continue;
+    double r0[NDIM], r1[NDIM];
+    bool finite = true;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+      const double s0 = (double)st[d * n + p], e0 = (double)rows[(1 + d) * n + p];
+      finite = finite && (s0 - s0 == 0.0) && (e0 - e0 == 0.0);
+      r0[d] = (s0 - g.lower[d]) * g.inverse[d];
+      r1[d] = (e0 - g.lower[d]) * g.inverse[d];
+    }
+    if (!finite) continue;
+    path_into_tile<NDIM, DEG>(r0, r1, charge, g, ax, (lds_u64 *)lds, accum, scale);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < PT::WORDS; i += BLOCK) {
+    const unsigned long long v = lds[i];
+    if (v == 0ull) continue;
+    const int axis = i % NDIM, node = i / NDIM;
+    const int l[3] = {NDIM == 3 ? node / (PT::LY * PT::LZ) : node / PT::LY,
+                      NDIM == 3 ? (node / PT::LZ) % PT::LY : node % PT::LY, NDIM == 3 ? node % PT::LZ : 0};
+    int64_t at = 0;
+    bool inside = true;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+      int64_t k = base[d] + l[d];
+      if (g.code[d] == 1) k = ((k % g.num[d]) + g.num[d]) % g.num[d];
+      inside = inside && k >= 0 && k < g.extent[d];
+      at += k * g.stride[d];
+    }
+    if (inside) atomicAdd(accum + at * NDIM + axis, v);
   }
 }
 
@@ -390,6 +766,19 @@ __global__ void path_bound_kernel(const T *__restrict__ st, int64_t ss, const T 
   if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 
+template <class T, int NDIM, int DEG>
+static void launch_spline_paths(int mode, const void *st, int64_t ss, const void *en, int64_t es, const void *q,
+                                int64_t n, const Grid &g, double *edges, unsigned long long *accum,
+                                const double *slot, hipStream_t stream) {
+  const int threads = 256, blocks = blocks_for(n, threads, 1 << 16);
+  if (mode == MODE_ATOMIC)
+    hipLaunchKernelGGL((spline_paths_kernel<T, NDIM, DEG, MODE_ATOMIC>), dim3(blocks), dim3(threads), 0, stream,
+                       (const T *)st, ss, (const T *)en, es, (const T *)q, n, g, edges, accum, slot);
+  else
+    hipLaunchKernelGGL((spline_paths_kernel<T, NDIM, DEG, MODE_FIXED>), dim3(blocks), dim3(threads), 0, stream,
+                       (const T *)st, ss, (const T *)en, es, (const T *)q, n, g, edges, accum, slot);
+}
+
 template <class T, int NDIM>
 static void launch_paths(int kind, int mode, const void *st, int64_t ss, const void *en, int64_t es, const void *q,
                          int64_t n, const Grid &g, int degree, double *edges, unsigned long long *accum,
@@ -402,14 +791,12 @@ static void launch_paths(int kind, int mode, const void *st, int64_t ss, const v
     else
       hipLaunchKernelGGL((whitney_kernel<T, NDIM, MODE_FIXED>), dim3(blocks), dim3(threads), 0, stream,
                          (const T *)st, ss, (const T *)en, es, (const T *)q, n, g, edges, accum, slot);
+  } else if (degree == 1) {
+    launch_spline_paths<T, NDIM, 1>(mode, st, ss, en, es, q, n, g, edges, accum, slot, stream);
+  } else if (degree == 2) {
+    launch_spline_paths<T, NDIM, 2>(mode, st, ss, en, es, q, n, g, edges, accum, slot, stream);
   } else {
-    const int threads = 128, blocks = blocks_for(n, threads, 1 << 16);
-    if (mode == MODE_ATOMIC)
-      hipLaunchKernelGGL((spline_paths_kernel<T, NDIM, MODE_ATOMIC>), dim3(blocks), dim3(threads), 0, stream,
-                         (const T *)st, ss, (const T *)en, es, (const T *)q, n, g, degree, edges, accum, slot);
-    else
-      hipLaunchKernelGGL((spline_paths_kernel<T, NDIM, MODE_FIXED>), dim3(blocks), dim3(threads), 0, stream,
-                         (const T *)st, ss, (const T *)en, es, (const T *)q, n, g, degree, edges, accum, slot);
+    launch_spline_paths<T, NDIM, 3>(mode, st, ss, en, es, q, n, g, edges, accum, slot, stream);
   }
 }
 
@@ -420,6 +807,58 @@ static void dispatch_paths(int kind, int ndim, int mode, const void *st, int64_t
   if (ndim == 1) launch_paths<T, 1>(kind, mode, st, ss, en, es, q, n, g, degree, edges, accum, slot, stream);
   if (ndim == 2) launch_paths<T, 2>(kind, mode, st, ss, en, es, q, n, g, degree, edges, accum, slot, stream);
   if (ndim == 3) launch_paths<T, 3>(kind, mode, st, ss, en, es, q, n, g, degree, edges, accum, slot, stream);
+}
+
+template <class T, int NDIM>
+static void launch_paths_tiled(int degree, const void *spos, const void *srows, int64_t n, const uint32_t *bins,
+                               const Grid &g, unsigned long long *accum, const double *slot, hipStream_t stream) {
+  using PT = PathTile<NDIM>;
+  const int ntx = (int)((g.num[0] + PT::TX - 1) / PT::TX), nty = (int)((g.num[1] + PT::TY - 1) / PT::TY);
+  const int ntz = NDIM == 3 ? (int)((g.num[2] + PT::TZ - 1) / PT::TZ) : 1;
+  const dim3 grid(ntx * nty * ntz), block(256);
+  ProfileScope prof("spline_paths_tiled", stream);
+#define TILED(D)                                                                                                   \
+  hipLaunchKernelGGL((spline_paths_tiled<T, NDIM, D, 256>), grid, block, 0, stream, (const T *)spos,              \
+                     (const T *)srows, n, bins, g, nty, ntz, accum, slot)
+  if (degree == 1) TILED(1);
+  else if (degree == 2) TILED(2);
+  else TILED(3);
+#undef TILED
+}
+
+//: paths at least this many go through the tiled deposit (spline edges, fixed point, 2D/3D)
+constexpr int64_t PATHS_TILED_MIN = 1 << 16;
+
+// Bin the paths by start cell for the tiled deposit: positions = start, values = the
+// charge and the end point (so that all of a path moves with it); returns the bins' sorted
+// start points, sorted rows (charge, end) and offsets in ctx->path_sorted.
+static int bin_paths(int degree, int ndim, const void *start, int64_t ss, const void *end, int64_t es,
+                     const void *charges, int dtype, int64_t n, const double *lower, const double *cell,
+                     const int64_t *num, const int64_t *code, mpf_context *ctx, hipStream_t stream, void **spos,
+                     void **srows, uint32_t **bins) {
+  const size_t elem = dtype == F32 ? 4 : 8, nn = (size_t)n;
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  int rc = ensure(ctx->path_in, up(nn * elem * (ndim + 1)), false, stream);
+  if (rc) return rc;
+  char *in = (char *)ctx->path_in.ptr;
+  MPF_CHECK(hipMemcpyAsync(in, charges, nn * elem, hipMemcpyDeviceToDevice, stream));
+  MPF_CHECK(hipMemcpy2DAsync(in + nn * elem, nn * elem, end, (size_t)es * elem, nn * elem, (size_t)ndim,
+                             hipMemcpyDeviceToDevice, stream));
+  int64_t cells = 0;
+  rc = mpf_bin_sizes(ndim, num, &cells);
+  if (rc) return rc;
+  const size_t b_spos = up(nn * elem * ndim), b_rows = up(nn * elem * (ndim + 1)), b_order = up(nn * 8),
+               b_bins = up((size_t)(cells + 2) * 4);
+  rc = ensure(ctx->path_sorted, b_spos + b_rows + b_order + b_bins + 256, false, stream);
+  if (rc) return rc;
+  char *out = (char *)ctx->path_sorted.ptr;
+  *spos = out;
+  *srows = out + b_spos;
+  int64_t *order = (int64_t *)(out + b_spos + b_rows);
+  *bins = (uint32_t *)(out + b_spos + b_rows + b_order);
+  double *scalars = (double *)(out + b_spos + b_rows + b_order + b_bins);
+  return mpf_bin_markers(SPLINE, degree, ndim, start, ss, in, (int64_t)nn, dtype, n, ndim + 1, lower, cell, num, code,
+                         *spos, *srows, order, *bins, scalars, MPF_SCALE_AUTO, ctx, (mpf_stream)stream);
 }
 
 }  // namespace mpf
@@ -462,6 +901,17 @@ extern "C" int mpf_deposit_paths(int kind, int degree, int ndim, const void *sta
   const int64_t count = g.extent[0] * (ndim > 1 ? g.extent[1] : 1) * (ndim > 2 ? g.extent[2] : 1) * ndim;
   unsigned long long *accum = nullptr;
   const double *slot = nullptr;
+  // large spline-edge deposits in fixed point: binned by start cell, tiled (the same sums)
+  bool tiled = kind == 1 && mode == MODE_FIXED && (ndim == 2 || ndim == 3) && n >= PATHS_TILED_MIN;
+  for (int d = 0; d < ndim; ++d) tiled = tiled && code[d] != 2;
+  void *spos = nullptr, *srows = nullptr;
+  uint32_t *bins = nullptr;
+  if (tiled) {
+    // the binning's own scale goes through ctx->scalars first; the path scale follows
+    rc = bin_paths(deg, ndim, start, start_stride, end, end_stride, charges, dtype, n, lower, cell_size, num_nodes,
+                   code, ctx, stream, &spos, &srows, &bins);
+    if (rc) return rc;
+  }
   if (mode == MODE_FIXED) {
     rc = ensure(ctx->accum, (size_t)count * 8, true, stream);
     if (rc) return rc;
@@ -486,12 +936,18 @@ extern "C" int mpf_deposit_paths(int kind, int degree, int ndim, const void *sta
     accum = (unsigned long long *)ctx->accum.ptr;
     slot = scale_slot(ctx);
   }
-  if (dtype == F32)
+  if (tiled) {
+    if (ndim == 2 && dtype == F32) launch_paths_tiled<float, 2>(deg, spos, srows, n, bins, g, accum, slot, stream);
+    if (ndim == 2 && dtype == F64) launch_paths_tiled<double, 2>(deg, spos, srows, n, bins, g, accum, slot, stream);
+    if (ndim == 3 && dtype == F32) launch_paths_tiled<float, 3>(deg, spos, srows, n, bins, g, accum, slot, stream);
+    if (ndim == 3 && dtype == F64) launch_paths_tiled<double, 3>(deg, spos, srows, n, bins, g, accum, slot, stream);
+  } else if (dtype == F32) {
     dispatch_paths<float>(kind, ndim, mode, start, start_stride, end, end_stride, charges, n, g, deg, edges, accum,
                           slot, stream);
-  else
+  } else {
     dispatch_paths<double>(kind, ndim, mode, start, start_stride, end, end_stride, charges, n, g, deg, edges, accum,
                            slot, stream);
+  }
   MPF_LAUNCH_CHECK();
   if (mode == MODE_FIXED) return finalize_fixed(ctx, edges, count, stream);
   return MPF_OK;

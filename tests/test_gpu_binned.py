@@ -119,8 +119,9 @@ def decode_bin(k, num):
 
 @pytest.mark.parametrize("case", ["3d-quadratic", "3d-linear-bounded", "3d-cubic-gather", "2d-large", "1d"])
 def test_bins_hold_each_marker_in_its_stencil_cell(case):
-    """Every binned marker sits in the bin of the cell the oracle's stencil anchors it to; the
-    2D case has enough cells (2^25) that the sort works on groups of 2048 keys."""
+    """Every binned marker sits in the bin of the cell the oracle's stencil anchors it to (3D
+    bins in tiles of 8 x 8 x 16 cells, 2D in tiles of 32 x 32); the 2D case has enough cells
+    (2^25) and few enough markers that the sort works on groups of 4096 keys."""
     print(f"bins {case}", flush=True)
     rng = np.random.default_rng(7)
     if case.startswith("3d"):
@@ -174,8 +175,13 @@ def test_bins_hold_each_marker_in_its_stencil_cell(case):
         cy, cz = np.divmod(rest, 16)
         got = (tx * 8 + cx, ty * 8 + cy, tz * 16 + cz)
         assert decode_bin(bin_of[live // 2], num) == tuple(int(x[live // 2]) for x in got)
+    elif ndim == 2:
+        t, c = np.divmod(bin_of, 32 * 32)  # tiles of 32 x 32 cells, x-major, y fastest
+        tx, ty = np.divmod(t, -(-num[1] // 32))
+        cx, cy = np.divmod(c, 32)
+        got = (tx * 32 + cx, ty * 32 + cy)
     else:
-        got = np.unravel_index(bin_of, num)
+        got = (bin_of,)
     for d in range(ndim):
         bad = np.nonzero(got[d] != anchors[d])[0]
         assert bad.size == 0, (d, bad[:5], got[d][bad[:5]], anchors[d][bad[:5]])

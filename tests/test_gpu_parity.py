@@ -174,6 +174,50 @@ def test_path_deposits_match_oracle(degree, ndim, periodic, mode):
     assert rel_err(got.cpu().numpy(), want) < BAR[mode]
 
 
+def many_paths(ndim, periodic, seed, n=80000):
+    """Enough paths for the binned, tiled deposit: short ones (a third of a cell), every 50th
+    long (about 6 cells, past the tile's halo), zero charges and NaN ends among them; one 3D
+    axis has fewer nodes than a tile's footprint."""
+    rng = np.random.default_rng(seed)
+    shape = (70, 48) if ndim == 2 else (20, 12, 36)
+    lower = np.array([0.0, -1.0, 2.0])[:ndim]
+    step = np.array([0.05, 0.1, 0.025])[:ndim]
+    upper = lower + step * (np.array(shape) - 1)
+    start = rng.uniform(lower - 2 * step, upper + 2 * step, (n, ndim)).T.copy()
+    reach = np.full(n, 0.3)
+    reach[::50] = 6.0
+    end = start + rng.normal(size=start.shape) * reach * step[:, None]
+    q = rng.normal(size=n)
+    q[::11] = 0.0
+    end[:, ::97] = np.nan
+    return geom(lower, step, shape, [periodic] * ndim), shape, start, end, q
+
+
+@pytest.mark.parametrize("periodic", [0, 1])
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("degree", [1, 2, 3])
+def test_tiled_path_deposit_matches_oracle_and_is_reproducible(degree, ndim, periodic):
+    """80000 paths take the binned, tiled fixed-point deposit: it equals the oracle, the
+    float64-atomic deposit, and itself bit for bit under any path order."""
+    print(f"tiled paths p={degree} {ndim}D periodic={periodic}", flush=True)
+    g, shape, start, end, q = many_paths(ndim, periodic, seed=30 + degree)
+    want = np.zeros((*shape, ndim))
+    orc.deposit_spline(start, end, q, want, *g, degree=degree)
+    fixed = torch.zeros((*shape, ndim), dtype=torch.float64, device="cuda")
+    K.deposit_paths(1, degree, cuda(start), cuda(end), cuda(q), fixed, g, mode=_lib.MODE_FIXED)
+    # the fixed-point quantum is 2^-k with n * max|q| (length + 2) * 2^k < 2^61: about 2^-36
+    # here (long paths), and an edge sums up to ~10^4 rounded contributions
+    assert rel_err(fixed.cpu().numpy(), want) < 2e-9
+    atomic = torch.zeros_like(fixed)
+    K.deposit_paths(1, degree, cuda(start), cuda(end), cuda(q), atomic, g, mode=_lib.MODE_ATOMIC)
+    assert rel_err(atomic.cpu().numpy(), want) < BAR["atomic"]
+    perm = np.random.default_rng(degree).permutation(start.shape[1])
+    again = torch.zeros_like(fixed)
+    K.deposit_paths(1, degree, cuda(start[:, perm]), cuda(end[:, perm]), cuda(q[perm]), again, g,
+                    mode=_lib.MODE_FIXED)
+    assert torch.equal(again, fixed)
+
+
 @pytest.mark.parametrize("periodic", [0, 1])
 @pytest.mark.parametrize("ndim", [2, 3])
 @pytest.mark.parametrize("degree", [2, 3])
