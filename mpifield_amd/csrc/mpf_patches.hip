@@ -28,20 +28,164 @@ struct Item {
   int64_t patch, first, count;
 };
 
-// One patch's per-axis stencils (kind/degree of the patch), as the direct kernels compute them.
-__device__ inline int patch_axis(const mpf_patch &p, int d, double x, int64_t *node, double *w) {
-  const double rel = (x - p.lower[d]) * (1.0 / p.cell_size[d]);
-  if (p.kind == LINEAR) return linear_axis(rel, p.num_nodes[d], (int)p.code[d], node, w);
-  if (p.kind == QUADRATIC) return quadratic_axis(rel, p.num_nodes[d], (int)p.code[d], node, w);
-  return spline_axis(rel, p.num_nodes[d], (int)p.code[d], p.degree, node, w);
+// The dynamic LDS copy of a patch as an LDS pointer (address space 3).
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+
+// One axis of a patch weighting with int32 nodes: linear_axis, quadratic_axis and
+// spline_axis's arithmetic with the kind and degree fixed (the rel wrap of a periodic axis
+// or clamp of a bounded one, the cell choice, the weights, the node wrap or clamp).
+template <int KIND, int P>
+struct PatchStencil {
+  static constexpr int S = KIND == LINEAR ? 2 : (KIND == QUADRATIC ? 3 : P + 1);
+};
+
+template <int KIND, int P>
+__device__ __forceinline__ void patch_stencil(double rel, int num, int code, int *node, double *w) {
+  double r = rel;
+  if (KIND == LINEAR) {
+    int l;
+    if (code == 1) r = periodic_position(r, num);
+    // the cast sees r itself for any finite position (and 0 for NaN: no index outside)
+    const int i = (int)fmin(fmax(r, 0.0), (double)num);
+    if (code == 1)
+      l = i >= num ? num - 1 : i;
+    else
+      l = i > num - 2 ? num - 2 : i;
+    node[0] = l;
+    node[1] = (code == 1 && l + 1 == num) ? 0 : l + 1;
+    const double d = r - (double)l;
+    const double f = d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d);
+    w[0] = 1.0 - f;
+    w[1] = f;
+    return;
+  }
+  if (code == 1) {
+    if (!(r >= 0.0 && r < (double)num)) r = r - (double)num * floor(r / (double)num);
+  } else {
+    if (r < 0.0) r = 0.0;
+    if (r > (double)(num - 1)) r = (double)num - 1.0;
+  }
+  constexpr int Q = KIND == QUADRATIC ? 2 : P;
+  const double rc = fmin(fmax(r, 0.0), (double)num);  // r itself for any finite position, 0 for NaN
+  const int middle = (Q % 2 == 0) ? (int)(rc + 0.5) : (int)rc;
+  const double o = r - (double)middle;
+  if (Q == 0) {
+    w[0] = 1.0;
+  } else if (Q == 1) {
+    w[0] = 1.0 - o;
+    w[1] = o;
+  } else if (Q == 2) {
+    w[0] = 0.5 * (0.5 - o) * (0.5 - o);
+    w[1] = 0.75 - o * o;
+    w[2] = 0.5 * (0.5 + o) * (0.5 + o);
+  } else {
+    const double cm = 1.0 - o, sq = o * o, cu = sq * o;
+    w[0] = cm * cm * cm / 6.0;
+    w[1] = (3.0 * cu - 6.0 * sq + 4.0) / 6.0;
+    w[2] = (-3.0 * cu + 3.0 * sq + 3.0 * o + 1.0) / 6.0;
+    w[3] = cu / 6.0;
+  }
+#pragma unroll
+  for (int a = 0; a <= Q; ++a) {
+    const int k = middle - Q / 2 + a;
+    if (code == 1)
+      node[a] = k < 0 ? k + num : (k >= num ? k - num : k);
+    else
+      node[a] = k < 0 ? 0 : (k > num - 1 ? num - 1 : k);
+  }
 }
+
+// The node offsets of one marker of patch p (each axis's stencil, row-major strides), and
+// its weights; false for a dead marker.
+template <int NDIM, int KIND, int P, class T>
+__device__ __forceinline__ void patch_marker(const mpf_patch &p, const T *__restrict__ pos, int64_t ps, int64_t m,
+                                             int (&node)[3][4], double (&w)[3][4]) {
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    const double rel = ((double)pos[d * ps + m] - p.lower[d]) * (1.0 / p.cell_size[d]);
+    patch_stencil<KIND, P>(rel, (int)p.num_nodes[d], (int)p.code[d], node[d], w[d]);
+  }
+}
+
+template <int NDIM, int KIND, int P, class T>
+__device__ __forceinline__ void patch_scatter_item(const mpf_patch &p, const T *__restrict__ pos, int64_t ps,
+                                                   const T *__restrict__ val, int64_t vs, int ncomp, int64_t first,
+                                                   int64_t count, lds_u64 *lds, double scale) {
+  constexpr int S = PatchStencil<KIND, P>::S;
+  const int s1 = NDIM == 3 ? (int)p.num_nodes[2] : 1;
+  const int s0 = NDIM == 3 ? (int)(p.num_nodes[1] * p.num_nodes[2]) : (NDIM == 2 ? (int)p.num_nodes[1] : 1);
+  for (int64_t m = first + threadIdx.x; m < first + count; m += blockDim.x) {
+    bool live = false;
+    for (int c = 0; c < ncomp; ++c) live |= (double)val[c * vs + m] != 0.0;
+    if (!live) continue;
+    int node[3][4];
+    double w[3][4];
+    patch_marker<NDIM, KIND, P, T>(p, pos, ps, m, node, w);
+#pragma unroll
+    for (int a = 0; a < S; ++a)
+#pragma unroll
+      for (int b = 0; b < (NDIM >= 2 ? S : 1); ++b) {
+        const double wab = NDIM >= 2 ? w[0][a] * w[NDIM >= 2 ? 1 : 0][b] : w[0][a];
+#pragma unroll
+        for (int e = 0; e < (NDIM == 3 ? S : 1); ++e) {
+          const double weight = NDIM == 3 ? wab * w[NDIM - 1][e] : wab;
+          const int at = (node[0][a] * s0 + (NDIM >= 2 ? node[NDIM >= 2 ? 1 : 0][b] * s1 : 0) +
+                          (NDIM == 3 ? node[NDIM - 1][e] : 0)) * ncomp;
+          for (int c = 0; c < ncomp; ++c)
+            __hip_atomic_fetch_add(lds + at + c, to_fixed(weight * (double)val[c * vs + m] * scale), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
+  }
+}
+
+template <int NDIM, int KIND, int P, class T>
+__device__ __forceinline__ void patch_gather_item(const mpf_patch &p, const T *__restrict__ pos, int64_t ps,
+                                                  int ncomp, int64_t first, int64_t count,
+                                                  const double *__restrict__ field, T *__restrict__ res, int64_t rs) {
+  constexpr int S = PatchStencil<KIND, P>::S;
+  const int s1 = NDIM == 3 ? (int)p.num_nodes[2] : 1;
+  const int s0 = NDIM == 3 ? (int)(p.num_nodes[1] * p.num_nodes[2]) : (NDIM == 2 ? (int)p.num_nodes[1] : 1);
+  for (int64_t m = first + threadIdx.x; m < first + count; m += blockDim.x) {
+    int node[3][4];
+    double w[3][4];
+    patch_marker<NDIM, KIND, P, T>(p, pos, ps, m, node, w);
+    for (int c = 0; c < ncomp; ++c) {
+      double total = 0.0;
+#pragma unroll
+      for (int a = 0; a < S; ++a)
+#pragma unroll
+        for (int b = 0; b < (NDIM >= 2 ? S : 1); ++b) {
+          const double wab = NDIM >= 2 ? w[0][a] * w[NDIM >= 2 ? 1 : 0][b] : w[0][a];
+#pragma unroll
+          for (int e = 0; e < (NDIM == 3 ? S : 1); ++e) {
+            const double weight = NDIM == 3 ? wab * w[NDIM - 1][e] : wab;
+            total += weight * field[(node[0][a] * s0 + (NDIM >= 2 ? node[NDIM >= 2 ? 1 : 0][b] * s1 : 0) +
+                                     (NDIM == 3 ? node[NDIM - 1][e] : 0)) * ncomp + c];
+          }
+        }
+      res[c * rs + m] = (T)total;
+    }
+  }
+}
+
+// The item's kind and degree, fixed for the call F<NDIM, KIND, P>.
+#define PATCH_DISPATCH(NDIM, CALL)                                   \
+  switch (p.kind == LINEAR ? 10 : (p.kind == QUADRATIC ? 20 : p.degree)) { \
+    case 10: CALL(NDIM, LINEAR, 1); break;                            \
+    case 20: CALL(NDIM, QUADRATIC, 2); break;                         \
+    case 0: CALL(NDIM, SPLINE, 0); break;                             \
+    case 1: CALL(NDIM, SPLINE, 1); break;                             \
+    case 2: CALL(NDIM, SPLINE, 2); break;                             \
+    default: CALL(NDIM, SPLINE, 3); break;                            \
+  }
 
 __global__ void zero_counter(unsigned int *c) { *c = 0u; }
 
-template <class T>
+template <class T, int NDIM>
 __global__ void __launch_bounds__(512) patches_scatter_kernel(const T *__restrict__ pos, int64_t ps,
                                                               const T *__restrict__ val, int64_t vs, int ncomp,
-                                                              int ndim, const mpf_patch *__restrict__ patches,
+                                                              const mpf_patch *__restrict__ patches,
                                                               const Item *__restrict__ items, int64_t n_items,
                                                               unsigned int *__restrict__ counter,
                                                               unsigned long long *__restrict__ accum,
@@ -56,37 +200,18 @@ __global__ void __launch_bounds__(512) patches_scatter_kernel(const T *__restric
     if (it >= n_items) break;  // every wave of the workgroup leaves together
     const Item item = items[it];
     const mpf_patch p = patches[item.patch];
-    int64_t nodes = 1;
-    for (int d = 0; d < ndim; ++d) nodes *= p.num_nodes[d];
-    const int64_t values = nodes * ncomp;
-    for (int64_t i = threadIdx.x; i < values; i += blockDim.x) lds[i] = 0ull;
+    int nodes = 1;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) nodes *= (int)p.num_nodes[d];
+    const int values = nodes * ncomp;
+    for (int i = threadIdx.x; i < values; i += blockDim.x) lds[i] = 0ull;
     __syncthreads();
-    for (int64_t m = item.first + threadIdx.x; m < item.first + item.count; m += blockDim.x) {
-      bool live = false;
-      for (int c = 0; c < ncomp; ++c) live |= (double)val[c * vs + m] != 0.0;
-      if (!live) continue;
-      int64_t node[3][4];
-      double w[3][4];
-      int size[3] = {1, 1, 1};
-      node[1][0] = node[2][0] = 0;
-      w[1][0] = w[2][0] = 1.0;
-      for (int d = 0; d < ndim; ++d) size[d] = patch_axis(p, d, (double)pos[d * ps + m], node[d], w[d]);
-      const int64_t s1 = ndim == 3 ? p.num_nodes[2] : 1;
-      const int64_t s0 = ndim == 3 ? p.num_nodes[1] * p.num_nodes[2] : (ndim == 2 ? p.num_nodes[1] : 1);
-      for (int a = 0; a < size[0]; ++a)
-        for (int b = 0; b < size[1]; ++b) {
-          const double wab = ndim >= 2 ? w[0][a] * w[1][b] : w[0][a];
-          for (int e = 0; e < size[2]; ++e) {
-            const double weight = ndim == 3 ? wab * w[2][e] : wab;
-            const int64_t at =
-                (node[0][a] * s0 + (ndim >= 2 ? node[1][b] * s1 : 0) + (ndim == 3 ? node[2][e] : 0)) * ncomp;
-            for (int c = 0; c < ncomp; ++c)
-              atomicAdd(&lds[at + c], to_fixed(weight * (double)val[c * vs + m] * scale));
-          }
-        }
-    }
+#define SCATTER(D, K, P) \
+  patch_scatter_item<D, K, P, T>(p, pos, ps, val, vs, ncomp, item.first, item.count, (lds_u64 *)lds, scale)
+    PATCH_DISPATCH(NDIM, SCATTER)
+#undef SCATTER
     __syncthreads();
-    for (int64_t i = threadIdx.x; i < values; i += blockDim.x) {
+    for (int i = threadIdx.x; i < values; i += blockDim.x) {
       const unsigned long long q = lds[i];
       if (q != 0ull) atomicAdd(accum + p.field_offset + i, q);
     }
@@ -94,9 +219,9 @@ __global__ void __launch_bounds__(512) patches_scatter_kernel(const T *__restric
   }
 }
 
-template <class T>
+template <class T, int NDIM>
 __global__ void __launch_bounds__(256) patches_gather_kernel(const T *__restrict__ pos, int64_t ps, int ncomp,
-                                                             int ndim, const mpf_patch *__restrict__ patches,
+                                                             const mpf_patch *__restrict__ patches,
                                                              const Item *__restrict__ items, int64_t n_items,
                                                              const double *__restrict__ fields,
                                                              T *__restrict__ res, int64_t rs) {
@@ -105,29 +230,9 @@ __global__ void __launch_bounds__(256) patches_gather_kernel(const T *__restrict
     const Item item = items[it];
     const mpf_patch p = patches[item.patch];
     const double *field = fields + p.field_offset;
-    const int64_t s1 = ndim == 3 ? p.num_nodes[2] : 1;
-    const int64_t s0 = ndim == 3 ? p.num_nodes[1] * p.num_nodes[2] : (ndim == 2 ? p.num_nodes[1] : 1);
-    for (int64_t m = item.first + threadIdx.x; m < item.first + item.count; m += blockDim.x) {
-      int64_t node[3][4];
-      double w[3][4];
-      int size[3] = {1, 1, 1};
-      node[1][0] = node[2][0] = 0;
-      w[1][0] = w[2][0] = 1.0;
-      for (int d = 0; d < ndim; ++d) size[d] = patch_axis(p, d, (double)pos[d * ps + m], node[d], w[d]);
-      for (int c = 0; c < ncomp; ++c) {
-        double total = 0.0;
-        for (int a = 0; a < size[0]; ++a)
-          for (int b = 0; b < size[1]; ++b) {
-            const double wab = ndim >= 2 ? w[0][a] * w[1][b] : w[0][a];
-            for (int e = 0; e < size[2]; ++e) {
-              const double weight = ndim == 3 ? wab * w[2][e] : wab;
-              total += weight * field[(node[0][a] * s0 + (ndim >= 2 ? node[1][b] * s1 : 0) +
-                                       (ndim == 3 ? node[2][e] : 0)) * ncomp + c];
-            }
-          }
-        res[c * rs + m] = (T)total;
-      }
-    }
+#define GATHER(D, K, P) patch_gather_item<D, K, P, T>(p, pos, ps, ncomp, item.first, item.count, field, res, rs)
+    PATCH_DISPATCH(NDIM, GATHER)
+#undef GATHER
   }
 }
 
@@ -238,16 +343,21 @@ extern "C" int mpf_patches_scatter(const mpf_patch_plan *plan, const void *posit
   if (workgroups < 1) workgroups = 256 * 2;
   {
     ProfileScope prof("patches_scatter", stream);
-    if (dtype == F32)
-      hipLaunchKernelGGL(patches_scatter_kernel<float>, dim3(workgroups), dim3(block), lds, stream,
-                         (const float *)positions, pos_stride, (const float *)values, val_stride, (int)ncomp,
-                         plan->ndim, (const mpf_patch *)plan->patches, (const Item *)plan->items, plan->n_items,
-                         plan->counter, (unsigned long long *)ctx->accum.ptr, scale_slot(ctx));
-    else
-      hipLaunchKernelGGL(patches_scatter_kernel<double>, dim3(workgroups), dim3(block), lds, stream,
-                         (const double *)positions, pos_stride, (const double *)values, val_stride, (int)ncomp,
-                         plan->ndim, (const mpf_patch *)plan->patches, (const Item *)plan->items, plan->n_items,
-                         plan->counter, (unsigned long long *)ctx->accum.ptr, scale_slot(ctx));
+#define LAUNCH(T, D)                                                                                              \
+  hipLaunchKernelGGL((patches_scatter_kernel<T, D>), dim3(workgroups), dim3(block), lds, stream,                  \
+                     (const T *)positions, pos_stride, (const T *)values, val_stride, (int)ncomp,                 \
+                     (const mpf_patch *)plan->patches, (const Item *)plan->items, plan->n_items, plan->counter, \
+                     (unsigned long long *)ctx->accum.ptr, scale_slot(ctx))
+    if (dtype == F32) {
+      if (plan->ndim == 1) LAUNCH(float, 1);
+      if (plan->ndim == 2) LAUNCH(float, 2);
+      if (plan->ndim == 3) LAUNCH(float, 3);
+    } else {
+      if (plan->ndim == 1) LAUNCH(double, 1);
+      if (plan->ndim == 2) LAUNCH(double, 2);
+      if (plan->ndim == 3) LAUNCH(double, 3);
+    }
+#undef LAUNCH
   }
   MPF_LAUNCH_CHECK();
   return finalize_fixed(ctx, fields, plan->field_values, stream);
@@ -265,15 +375,20 @@ extern "C" int mpf_patches_gather(const mpf_patch_plan *plan, const void *positi
   hipStream_t stream = (hipStream_t)stream_;
   if (workgroups < 1) workgroups = (int)(plan->n_items < 65536 ? plan->n_items : 65536);
   ProfileScope prof("patches_gather", stream);
-  if (dtype == F32)
-    hipLaunchKernelGGL(patches_gather_kernel<float>, dim3(workgroups), dim3(256), 0, stream, (const float *)positions,
-                       pos_stride, (int)ncomp, plan->ndim, (const mpf_patch *)plan->patches,
-                       (const Item *)plan->items, plan->n_items, fields, (float *)result, result_stride);
-  else
-    hipLaunchKernelGGL(patches_gather_kernel<double>, dim3(workgroups), dim3(256), 0, stream,
-                       (const double *)positions, pos_stride, (int)ncomp, plan->ndim,
-                       (const mpf_patch *)plan->patches, (const Item *)plan->items, plan->n_items, fields,
-                       (double *)result, result_stride);
+#define LAUNCH(T, D)                                                                                               \
+  hipLaunchKernelGGL((patches_gather_kernel<T, D>), dim3(workgroups), dim3(256), 0, stream, (const T *)positions, \
+                     pos_stride, (int)ncomp, (const mpf_patch *)plan->patches, (const Item *)plan->items,         \
+                     plan->n_items, fields, (T *)result, result_stride)
+  if (dtype == F32) {
+    if (plan->ndim == 1) LAUNCH(float, 1);
+    if (plan->ndim == 2) LAUNCH(float, 2);
+    if (plan->ndim == 3) LAUNCH(float, 3);
+  } else {
+    if (plan->ndim == 1) LAUNCH(double, 1);
+    if (plan->ndim == 2) LAUNCH(double, 2);
+    if (plan->ndim == 3) LAUNCH(double, 3);
+  }
+#undef LAUNCH
   MPF_LAUNCH_CHECK();
   return MPF_OK;
 }

@@ -33,7 +33,7 @@ from mpifield_amd.binning import kind_of
 
 #: the tuned (chunk, block, workgroups) per (ndim, degrees, ncomp), filled by autotune()
 TUNED: dict[tuple, tuple[int, int, int]] = {}
-DEFAULT = (8192, 256, 0)
+DEFAULT = (8192, 512, 0)
 
 
 @dataclass
