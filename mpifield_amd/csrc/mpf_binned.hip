@@ -1176,29 +1176,140 @@ __global__ void __launch_bounds__(BLOCK) gather_items(const float *__restrict__ 
 // The dead bin (markers whose values were all zero at binning) is in no cell; a gather
 // still interpolates at those positions, one marker per work-item, its range read from the
 // offsets on the device.
-template <class T, int KIND, int C>
+template <class T, int KIND, int C, int NDIM = 3>
 __global__ void __launch_bounds__(256) gather_dead_kernel(const T *__restrict__ pos, int64_t n, int degree,
                                                           const uint32_t *__restrict__ start, int64_t cells, Grid g,
                                                           const double *__restrict__ field, T *__restrict__ res) {
   const int64_t first = start[cells], last = start[cells + 1];
   for (int64_t m = first + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < last;
        m += (int64_t)gridDim.x * blockDim.x) {
-    int64_t node[3][4];
-    double w[3][4];
-    int size[3];
-    for (int d = 0; d < 3; ++d)
+    int64_t node[3][4] = {{0}, {0}, {0}};
+    double w[3][4] = {{1.0}, {1.0}, {1.0}};
+    int size[3] = {1, 1, 1};
+    for (int d = 0; d < NDIM; ++d)
       size[d] = axis_stencil<KIND>(((double)pos[d * n + m] - g.lower[d]) * g.inverse[d], g.num[d], g.code[d], degree,
                                    node[d], w[d]);
     for (int c = 0; c < C; ++c) {
       double total = 0.0;
       for (int a = 0; a < size[0]; ++a)
         for (int b = 0; b < size[1]; ++b) {
-          const double wab = w[0][a] * w[1][b];
-          for (int e = 0; e < size[2]; ++e)
-            total += wab * w[2][e] * field[((node[0][a] * g.num[1] + node[1][b]) * g.num[2] + node[2][e]) * C + c];
+          const double wab = NDIM >= 2 ? w[0][a] * w[1][b] : w[0][a];
+          for (int e = 0; e < size[2]; ++e) {
+            const double weight = NDIM == 3 ? wab * w[2][e] : wab;
+            total += weight * field[(node[0][a] * g.stride[0] + node[1][b] * g.stride[1] + node[2][e] * g.stride[2]) * C + c];
+          }
         }
       res[c * n + m] = (T)total;
     }
+  }
+}
+
+// ---- 2D tiles ----------------------------------------------------------------------------
+//
+// 2D markers binned in tiles of TILE2 x TILE2 cells.  Deposit: one workgroup per tile, an
+// int64 fixed-point accumulator over the tile's footprint (TILE2 + S - 1 nodes per axis) in
+// LDS; each work-item walks a contiguous run of the tile's markers (the lanes of a wave on
+// different cells) and adds to_fixed(wx * wy * v * 2^k) per node and component -- the
+// integers of the one-marker-per-work-item fixed-point deposit (scatter_direct), so the
+// result is the same bit for bit; the tile's nonzero words go into the global int64
+// accumulator (integer atomics) and a finalize pass converts.  Gather: the tile's float64
+// footprint in LDS, one marker per lane, the direct gather's sums in its order.
+
+// The footprint slot of the unwrapped stencil node k of an axis whose footprint starts at
+// node `base`: a bounded axis clamps the node (as the stencils do), a periodic one folds.
+__device__ __forceinline__ int slot_2d(int k, int base, int num, int code) {
+  if (code == 1) {
+    const int s = k - base;
+    return s < 0 ? s + num : (s >= num ? s - num : s);
+  }
+  return (k < 0 ? 0 : (k > num - 1 ? num - 1 : k)) - base;
+}
+
+template <class T, int KIND, int P, int C>
+__global__ void __launch_bounds__(256) scatter_tiled_2d(const T *__restrict__ pos, const T *__restrict__ val,
+                                                        int64_t n, const uint32_t *__restrict__ start, Grid g,
+                                                        int nty, unsigned long long *__restrict__ accum,
+                                                        const double *__restrict__ scalars) {
+  using Sh = StencilShape<KIND, P>;
+  constexpr int S = Sh::S, OFF = Sh::OFF, L = TILE2 + S - 1, BLOCK = 256;
+  __shared__ unsigned long long acc[L * L * C];
+  const int tile = blockIdx.x, t = threadIdx.x;
+  const int num0 = (int)g.num[0], num1 = (int)g.num[1];
+  const int bx = (tile / nty) * TILE2 - OFF, by = (tile % nty) * TILE2 - OFF;
+  for (int i = t; i < L * L * C; i += BLOCK) acc[i] = 0ull;
+  const uint32_t first = start[(int64_t)tile * TILE2_CELLS], last = start[(int64_t)(tile + 1) * TILE2_CELLS];
+  const double scale = scalars[0];
+  __syncthreads();
+  const uint32_t chunk = (last - first + BLOCK - 1) / BLOCK;
+  const uint32_t lo = first + t * chunk, hi = lo + chunk < last ? lo + chunk : last;
+  for (uint32_t m = lo; m < hi; ++m) {
+    double wx[4], wy[4], v[C];
+    const int ax = anchor_axis<KIND, P>(((double)pos[m] - g.lower[0]) * g.inverse[0], num0, g.code[0], wx);
+    const int ay = anchor_axis<KIND, P>(((double)pos[n + m] - g.lower[1]) * g.inverse[1], num1, g.code[1], wy);
+#pragma unroll
+    for (int c = 0; c < C; ++c) v[c] = (double)val[c * n + m];
+#pragma unroll
+    for (int a = 0; a < S; ++a) {
+      const int sx = slot_2d(ax - OFF + a, bx, num0, g.code[0]);
+#pragma unroll
+      for (int b = 0; b < S; ++b) {
+        const int sy = slot_2d(ay - OFF + b, by, num1, g.code[1]);
+        const double weight = wx[a] * wy[b];
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+          atomicAdd(acc + (sx * L + sy) * C + c, to_fixed(weight * v[c] * scale));
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < L * L * C; i += BLOCK) {
+    const unsigned long long q = acc[i];
+    if (q == 0ull) continue;
+    const int c = i % C, sl = i / C, sx = sl / L, sy = sl % L;
+    int gx = bx + sx, gy = by + sy;
+    if (g.code[0] == 1) gx = gx < 0 ? gx + num0 : (gx >= num0 ? gx - num0 : gx);
+    if (g.code[1] == 1) gy = gy < 0 ? gy + num1 : (gy >= num1 ? gy - num1 : gy);
+    atomicAdd(accum + ((int64_t)gx * num1 + gy) * C + c, q);
+  }
+}
+
+template <class T, int KIND, int P, int C>
+__global__ void __launch_bounds__(256) gather_tiled_2d(const T *__restrict__ pos, int64_t n,
+                                                       const uint32_t *__restrict__ start, Grid g, int nty,
+                                                       const double *__restrict__ field, T *__restrict__ res) {
+  using Sh = StencilShape<KIND, P>;
+  constexpr int S = Sh::S, OFF = Sh::OFF, L = TILE2 + S - 1, BLOCK = 256;
+  __shared__ double tile_f[L * L * C];
+  const int tile = blockIdx.x, t = threadIdx.x;
+  const int num0 = (int)g.num[0], num1 = (int)g.num[1];
+  const int bx = (tile / nty) * TILE2 - OFF, by = (tile % nty) * TILE2 - OFF;
+  for (int i = t; i < L * L; i += BLOCK) {
+    const int64_t gx = wrap_index(bx + i / L, num0, g.code[0]), gy = wrap_index(by + i % L, num1, g.code[1]);
+#pragma unroll
+    for (int c = 0; c < C; ++c) tile_f[i * C + c] = field[(gx * num1 + gy) * C + c];
+  }
+  const uint32_t first = start[(int64_t)tile * TILE2_CELLS], last = start[(int64_t)(tile + 1) * TILE2_CELLS];
+  __syncthreads();
+  for (uint32_t m = first + t; m < last; m += BLOCK) {
+    double wx[4], wy[4];
+    const int ax = anchor_axis<KIND, P>(((double)pos[m] - g.lower[0]) * g.inverse[0], num0, g.code[0], wx);
+    const int ay = anchor_axis<KIND, P>(((double)pos[n + m] - g.lower[1]) * g.inverse[1], num1, g.code[1], wy);
+    double total[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) total[c] = 0.0;
+#pragma unroll
+    for (int a = 0; a < S; ++a) {
+      const int sx = slot_2d(ax - OFF + a, bx, num0, g.code[0]);
+#pragma unroll
+      for (int b = 0; b < S; ++b) {
+        const int sy = slot_2d(ay - OFF + b, by, num1, g.code[1]);
+        const double weight = wx[a] * wy[b];
+#pragma unroll
+        for (int c = 0; c < C; ++c) total[c] += weight * tile_f[(sx * L + sy) * C + c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < C; ++c) res[c * n + m] = (T)total[c];
   }
 }
 
@@ -1405,6 +1516,60 @@ static bool tiled_supported(int kind, int degree, int ndim, int64_t ncomp, const
   }
   return true;
 }
+
+// 2D tiles take every node weighting (linear, quadratic, spline 0..3) and 1 to 3
+// components on bounded or periodic axes.
+static bool tiled_2d_supported(int kind, int degree, int ndim, int64_t ncomp, const Grid &g) {
+  if (ndim != 2 || ncomp < 1 || ncomp > 3) return false;
+  if (kind == SPLINE && (degree < 0 || degree > 3)) return false;
+  return g.code[0] != 2 && g.code[1] != 2;
+}
+
+template <class T, int KIND, int P, int C>
+static int run_scatter_2d(const void *pos, const void *val, int64_t n, const uint32_t *start, const Grid &g,
+                          double *field, const double *scalars, mpf_context *ctx, hipStream_t stream) {
+  const int ntx = (int)((g.num[0] + TILE2 - 1) / TILE2), nty = (int)((g.num[1] + TILE2 - 1) / TILE2);
+  const int64_t count = g.num[0] * g.num[1] * C;
+  int rc = ensure(ctx->accum, (size_t)count * 8, true, stream);
+  if (rc) return rc;
+  {
+    ProfileScope prof("scatter_tiled_2d", stream);
+    hipLaunchKernelGGL((scatter_tiled_2d<T, KIND, P, C>), dim3(ntx * nty), dim3(256), 0, stream, (const T *)pos,
+                       (const T *)val, n, start, g, nty, (unsigned long long *)ctx->accum.ptr, scalars);
+  }
+  MPF_LAUNCH_CHECK();
+  return finalize_fixed_with(ctx, field, count, scalars, stream);
+}
+
+template <class T, int KIND, int P, int C>
+static int run_gather_2d(const void *pos, int64_t n, const uint32_t *start, const Grid &g, const double *field,
+                         void *res, hipStream_t stream) {
+  const int ntx = (int)((g.num[0] + TILE2 - 1) / TILE2), nty = (int)((g.num[1] + TILE2 - 1) / TILE2);
+  {
+    ProfileScope prof("gather_tiled_2d", stream);
+    hipLaunchKernelGGL((gather_tiled_2d<T, KIND, P, C>), dim3(ntx * nty), dim3(256), 0, stream, (const T *)pos, n,
+                       start, g, nty, field, (T *)res);
+  }
+  MPF_LAUNCH_CHECK();
+  const int64_t cells = (int64_t)ntx * nty * TILE2_CELLS;
+  hipLaunchKernelGGL((gather_dead_kernel<T, KIND, C, 2>), dim3(64), dim3(256), 0, stream, (const T *)pos, n, P, start,
+                     cells, g, field, (T *)res);
+  MPF_LAUNCH_CHECK();
+  return MPF_OK;
+}
+
+// Every 2D case, F(T, KIND, P, C) called with the weighting and component count fixed.
+#define CASES_2D_C(F, T, K, P)        \
+  if (ncomp == 1) return F(T, K, P, 1); \
+  if (ncomp == 2) return F(T, K, P, 2); \
+  return F(T, K, P, 3);
+#define CASES_2D(F, T)                                          \
+  if (kind == LINEAR) { CASES_2D_C(F, T, LINEAR, 1) }             \
+  if (kind == QUADRATIC) { CASES_2D_C(F, T, QUADRATIC, 2) }       \
+  if (degree == 0) { CASES_2D_C(F, T, SPLINE, 0) }                \
+  if (degree == 1) { CASES_2D_C(F, T, SPLINE, 1) }                \
+  if (degree == 2) { CASES_2D_C(F, T, SPLINE, 2) }                \
+  CASES_2D_C(F, T, SPLINE, 3)
 
 }  // namespace mpf
 
@@ -1696,12 +1861,23 @@ extern "C" int mpf_scatter_binned(int kind, int degree, int ndim, const void *so
                      num_nodes, code);
   if (rc) return rc;
   const TileShape ts = tile_shape(kind, degree, ncomp);
-  if (!tiled_supported(kind, degree, ndim, ncomp, g, ts)) {
+  if (!tiled_supported(kind, degree, ndim, ncomp, g, ts) && !tiled_2d_supported(kind, degree, ndim, ncomp, g)) {
     set_error("no tiled kernel for kind %d degree %d, %dD, %lld components on this grid", kind, degree, ndim,
               (long long)ncomp);
     return MPF_ERR_UNSUPPORTED;
   }
   hipStream_t stream = (hipStream_t)stream_;
+  if (tiled_2d_supported(kind, degree, ndim, ncomp, g)) {
+    if (!ctx) ctx = default_context();
+    if (!ctx) return MPF_ERR_HIP;
+#define SC2(T, K, P, C) run_scatter_2d<T, K, P, C>(sorted_positions, sorted_values, n, start, g, field, scalars, ctx, stream)
+    if (dtype == F32) {
+      CASES_2D(SC2, float)
+    } else {
+      CASES_2D(SC2, double)
+    }
+#undef SC2
+  }
 #define SC(T, K, P, C) run_scatter_tiled<T, K, P, C>(sorted_positions, sorted_values, n, start, g, field, scalars, stream)
   if (dtype == F32) {
     TILED_CASES(SC, float)
@@ -1724,12 +1900,21 @@ extern "C" int mpf_gather_binned(int kind, int degree, int ndim, const void *sor
                      num_nodes, code);
   if (rc) return rc;
   const TileShape ts = tile_shape(kind, degree, ncomp);
-  if (!tiled_supported(kind, degree, ndim, ncomp, g, ts)) {
+  if (!tiled_supported(kind, degree, ndim, ncomp, g, ts) && !tiled_2d_supported(kind, degree, ndim, ncomp, g)) {
     set_error("no tiled kernel for kind %d degree %d, %dD, %lld components on this grid", kind, degree, ndim,
               (long long)ncomp);
     return MPF_ERR_UNSUPPORTED;
   }
   hipStream_t stream = (hipStream_t)stream_;
+  if (tiled_2d_supported(kind, degree, ndim, ncomp, g)) {
+#define GA2(T, K, P, C) run_gather_2d<T, K, P, C>(sorted_positions, n, start, g, field, result, stream)
+    if (dtype == F32) {
+      CASES_2D(GA2, float)
+    } else {
+      CASES_2D(GA2, double)
+    }
+#undef GA2
+  }
 #define GA(T, K, P, C) run_gather_tiled<T, K, P, C>(sorted_positions, n, start, g, field, result, stream)
   if (dtype == F32) {
     TILED_CASES(GA, float)

@@ -205,6 +205,15 @@ __global__ void finalize_kernel(double *__restrict__ field, unsigned long long *
   }
 }
 
+int finalize_fixed_with(mpf_context *ctx, double *field, int64_t count, const double *slot, hipStream_t stream) {
+  if (count == 0) return MPF_OK;
+  const int threads = 256;
+  hipLaunchKernelGGL(finalize_kernel, dim3(blocks_for(count, threads, 1 << 16)), dim3(threads), 0, stream, field,
+                     (unsigned long long *)ctx->accum.ptr, count, slot);
+  MPF_LAUNCH_CHECK();
+  return MPF_OK;
+}
+
 int finalize_fixed(mpf_context *ctx, double *field, int64_t count, hipStream_t stream) {
   if (count == 0) return MPF_OK;
   const int threads = 256;

@@ -65,6 +65,8 @@ double *scale_slot(mpf_context *ctx);  // device pointer: [0] scale, [1] 1/scale
 
 // field[i] += accum[i] / scale; accum[i] = 0
 int finalize_fixed(mpf_context *ctx, double *field, int64_t count, hipStream_t stream);
+// the same with the scale read from slot ([scale, 1/scale]) instead of the context's
+int finalize_fixed_with(mpf_context *ctx, double *field, int64_t count, const double *slot, hipStream_t stream);
 
 // Kernel timing: when enabled (mpf_profile_enable), the library records a pair of HIP
 // events on the launch stream around the launches it names, for mpf_profile_query.

@@ -139,9 +139,12 @@ _TILE = (8, 8, 16)
 
 
 def tiled_ok(kind: int, degree: int, ncomp: int, num, code) -> bool:
-    """Whether the tiled kernels take this case (the rule of mpf_binned.hip's tiled_supported)."""
-    if len(num) != 3 or any(int(c) == 2 for c in code):
+    """Whether the tiled kernels take this case (the rules of mpf_binned.hip's tiled_supported
+    and tiled_2d_supported)."""
+    if len(num) not in (2, 3) or any(int(c) == 2 for c in code):
         return False
+    if len(num) == 2:
+        return 1 <= ncomp <= 3 and (kind in (0, 1) or 0 <= degree <= 3)
     p = {0: 1, 1: 2}.get(kind, degree)
     if kind == 2 and degree != 3:
         return False
