@@ -1503,8 +1503,10 @@ static int colors_for(int nt, int code) {
 static bool tiled_supported(int kind, int degree, int ndim, int64_t ncomp, const Grid &g, const TileShape &ts) {
   if (ndim != 3) return false;
   const int p = kind == LINEAR ? 1 : (kind == QUADRATIC ? 2 : degree);
-  if (!((ncomp == 1 || ncomp == 3) && (p == 1 || p == 2 || (p == 3 && ncomp == 1)))) return false;
-  if (kind == SPLINE && degree != 3) return false;  // linear / quadratic have their own kinds
+  // 1 or 3 components up to degree 2; the cubic's 64-node stencil holds one component in
+  // registers
+  if (!((ncomp == 1 || ncomp == 3) && (p <= 2 || (p == 3 && ncomp == 1)))) return false;
+  if (kind == SPLINE && (degree < 0 || degree > 3)) return false;
   const int t[3] = {ts.tx, ts.ty, ts.tz};
   for (int d = 0; d < 3; ++d) {
     if (g.code[d] == 2) return false;
@@ -1844,7 +1846,13 @@ static int run_gather_tiled(const void *pos, int64_t n, const uint32_t *start, c
   if (kind == LINEAR && ncomp == 3) return X(T, LINEAR, 1, 3);       \
   if (kind == QUADRATIC && ncomp == 1) return X(T, QUADRATIC, 2, 1); \
   if (kind == QUADRATIC && ncomp == 3) return X(T, QUADRATIC, 2, 3); \
-  if (kind == SPLINE && degree == 3 && ncomp == 1) return X(T, SPLINE, 3, 1);
+  if (kind == SPLINE && degree == 3 && ncomp == 1) return X(T, SPLINE, 3, 1);  \
+  if (kind == SPLINE && degree == 2 && ncomp == 1) return X(T, SPLINE, 2, 1);  \
+  if (kind == SPLINE && degree == 2 && ncomp == 3) return X(T, SPLINE, 2, 3);  \
+  if (kind == SPLINE && degree == 1 && ncomp == 1) return X(T, SPLINE, 1, 1);  \
+  if (kind == SPLINE && degree == 1 && ncomp == 3) return X(T, SPLINE, 1, 3);  \
+  if (kind == SPLINE && degree == 0 && ncomp == 1) return X(T, SPLINE, 0, 1);  \
+  if (kind == SPLINE && degree == 0 && ncomp == 3) return X(T, SPLINE, 0, 3);
 
 /* Deposit binned markers into field with the tiled kernels (int64 fixed point through
  * float64 exact accumulators; bit-reproducible).  sorted_* and start come from

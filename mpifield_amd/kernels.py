@@ -146,9 +146,7 @@ def tiled_ok(kind: int, degree: int, ncomp: int, num, code) -> bool:
     if len(num) == 2:
         return 1 <= ncomp <= 3 and (kind in (0, 1) or 0 <= degree <= 3)
     p = {0: 1, 1: 2}.get(kind, degree)
-    if kind == 2 and degree != 3:
-        return False
-    if not ((ncomp in (1, 3) and p in (1, 2)) or (p == 3 and ncomp == 1)):
+    if not ((ncomp in (1, 3) and p in (0, 1, 2)) or (p == 3 and ncomp == 1)):
         return False
     for n, c, t in zip(num, code, _TILE):
         if n < t or (int(c) == 1 and (n % t != 0 or n // t < 2)):

@@ -17,8 +17,9 @@ if torch.cuda.is_available():
     from mpifield_amd import binning
     from mpifield_amd import kernels as K
 
-NAMES = {(0, 1): "linear", (1, 2): "quadratic", (2, 3): "spline"}
-TILED = [(0, 1, 1), (0, 1, 3), (1, 2, 1), (1, 2, 3), (2, 3, 1)]  # (kind, degree, ncomp)
+NAMES = {(0, 1): "linear", (1, 2): "quadratic", (2, 0): "spline", (2, 1): "spline", (2, 2): "spline", (2, 3): "spline"}
+TILED = [(0, 1, 1), (0, 1, 3), (1, 2, 1), (1, 2, 3), (2, 3, 1),  # (kind, degree, ncomp)
+         (2, 0, 1), (2, 0, 3), (2, 1, 1), (2, 1, 3), (2, 2, 1), (2, 2, 3)]
 GRIDS = [
     ((32, 16, 32), (1, 1, 1)),
     ((24, 16, 32), (0, 1, 0)),   # bounded x and z: folds at the walls
