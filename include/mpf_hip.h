@@ -224,19 +224,28 @@ int mpf_bin_markers(int kind, int degree, int ndim, const void *positions, int64
 
 /* Deposit binned markers (scatter_quadratic_3d etc., the tiled form): one workgroup per
  * tile of cells accumulates int64 fixed point in LDS and adds the tile into field once;
- * bit-reproducible.  3D, codes 0/1, (linear | quadratic) x (1 | 3 components) and cubic x 1
- * component; MPF_ERR_UNSUPPORTED otherwise (use mpf_scatter). */
+ * bit-reproducible.  3D, codes 0/1: linear, quadratic and spline 0..2 with 1 or 3
+ * components, cubic with 1; 2D, codes 0/1: every node weighting with 1 to 3 components;
+ * MPF_ERR_UNSUPPORTED otherwise (use mpf_scatter).  item_tables / item_counts: the 3D bins'
+ * tables from mpf_bin_items, or NULL (each launch builds them). */
 int mpf_scatter_binned(int kind, int degree, int ndim, const void *sorted_positions, const void *sorted_values,
-                       int dtype, int64_t n, int64_t ncomp, const uint32_t *start, const double *scalars,
-                       double *field, const double *lower, const double *cell_size, const int64_t *num_nodes,
-                       const int64_t *code, mpf_context *ctx, mpf_stream stream);
+                       int dtype, int64_t n, int64_t ncomp, const uint32_t *start, const uint16_t *item_tables,
+                       const uint32_t *item_counts, const double *scalars, double *field, const double *lower,
+                       const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
+                       mpf_stream stream);
 
 /* Gather at binned markers (the tiled form of gather_quadratic_3d etc.); result (ncomp, n)
- * in the sorted order, contiguous rows.  Same cases as mpf_scatter_binned. */
+ * in the sorted order, contiguous rows.  Same cases and tables as mpf_scatter_binned. */
 int mpf_gather_binned(int kind, int degree, int ndim, const void *sorted_positions, int dtype, int64_t n,
-                      const uint32_t *start, const double *field, int64_t ncomp, void *result, const double *lower,
-                      const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
-                      mpf_stream stream);
+                      const uint32_t *start, const uint16_t *item_tables, const uint32_t *item_counts,
+                      const double *field, int64_t ncomp, void *result, const double *lower, const double *cell_size,
+                      const int64_t *num_nodes, const int64_t *code, mpf_context *ctx, mpf_stream stream);
+
+/* The work lists of 3D bins, made once for bins kept across deposits and gathers:
+ * tables[tile * 4096 + i] (uint16: cell | sweep << 12) and counts[tile], tiles = bins / 1024
+ * (mpf_bin_sizes).  No counterpart in the reference (whose CPU kernels need no work lists). */
+int mpf_bin_items(int ndim, const int64_t *num_nodes, const uint32_t *start, uint16_t *tables, uint32_t *counts,
+                  mpf_context *ctx, mpf_stream stream);
 
 /* The gradient of the interpolant at binned markers, contracted with per-marker
  * coefficients: out[d * out_stride + order[m]] = sum_c coef[c * coef_stride + order[m]] *

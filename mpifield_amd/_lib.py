@@ -64,8 +64,10 @@ SIGNATURES: dict[str, list] = {
     "mpf_patches_gather": [_P, _P, _I64, _INT, _P, _I64, _P, _I64, _INT, _P, _P],
     "mpf_bin_markers": [_INT, _INT, _INT, _P, _I64, _P, _I64, _INT, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                         _INT, _P, _P],
-    "mpf_scatter_binned": [_INT, _INT, _INT, _P, _P, _INT, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P],
-    "mpf_gather_binned": [_INT, _INT, _INT, _P, _INT, _I64, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P],
+    "mpf_scatter_binned": [_INT, _INT, _INT, _P, _P, _INT, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                           _P],
+    "mpf_gather_binned": [_INT, _INT, _INT, _P, _INT, _I64, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P],
+    "mpf_bin_items": [_INT, _P, _P, _P, _P, _P, _P],
     "mpf_gather_derivative_binned": [_INT, _INT, _INT, _P, _INT, _I64, _P, _P, _P, _I64, _P, _I64, _P, _I64, _P, _P,
                                      _P, _P, _P, _P],
     "mpf_fixed_scale": [_P, _I64, _INT, _I64, _I64, _INT, _P, _P, _P],

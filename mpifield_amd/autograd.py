@@ -43,7 +43,7 @@ def positions_bins(positions: torch.Tensor, kind: int, degree: int, geometry, nc
     for ref, version, k, bins in _BINS:
         if ref() is positions and version == positions._version and k == key:
             return bins
-    bins = binning.bin_markers(positions.detach(), None, (kind, degree), geometry)
+    bins = binning.bin_markers(positions.detach(), None, (kind, degree), geometry).prepare_items()
     _BINS.append((weakref.ref(positions), positions._version, key, bins))
     del _BINS[:-_BINS_KEPT]
     return bins

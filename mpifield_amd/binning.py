@@ -49,6 +49,8 @@ class MarkerBins:
     order: torch.Tensor  # (N,) int64: sorted index -> input index
     start: torch.Tensor  # (bins + 2,) int32 holding uint32 offsets (3D bins in tile order)
     scalars: torch.Tensor  # (3,) float64: [2^k, 2^-k, markers per register pass]
+    tables: torch.Tensor | None = None  # (tiles, 4096) int16: the 3D tiles' work lists (prepare_items)
+    table_counts: torch.Tensor | None = None  # (tiles,) int32
 
     @property
     def n(self) -> int:
@@ -57,6 +59,26 @@ class MarkerBins:
     @property
     def ncomp(self) -> int:
         return 0 if self.values is None else int(self.values.shape[0])
+
+    def prepare_items(self) -> "MarkerBins":
+        """Make the 3D tiles' work lists once (mpf_bin_items), for bins that are deposited more
+        than once: each deposit launch then loads a tile's list instead of building it."""
+        ndim, _, _, num, _ = _geometry_arrays(self.geometry)
+        if ndim != 3 or self.tables is not None:
+            return self
+        tiles = (self.start.numel() - 2) // 1024
+        dev = self.positions.device
+        self.tables = torch.empty((max(tiles, 1), 4096), dtype=torch.int16, device=dev)
+        self.table_counts = torch.empty(max(tiles, 1), dtype=torch.int32, device=dev)
+        with torch.cuda.device(dev):
+            _lib.call("mpf_bin_items", ndim, num.ctypes.data, self.start.data_ptr(), self.tables.data_ptr(),
+                      self.table_counts.data_ptr(), None, K.stream_handle(dev))
+        return self
+
+    def _table_ptrs(self):
+        if self.tables is None:
+            return None, None
+        return self.tables.data_ptr(), self.table_counts.data_ptr()
 
     def unsort(self, sorted_rows: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """Rows (k, N) in bins order back to the input order."""
@@ -125,8 +147,8 @@ def scatter_binned(bins: MarkerBins, field: torch.Tensor) -> None:
     with torch.cuda.device(field.device):
         _lib.call("mpf_scatter_binned", bins.kind, bins.degree, ndim, bins.positions.data_ptr(),
                   bins.values.data_ptr(), K._dtype_code(bins.positions), bins.n, bins.ncomp, bins.start.data_ptr(),
-                  bins.scalars.data_ptr(), field.data_ptr(), lower.ctypes.data, cell.ctypes.data, num.ctypes.data,
-                  code.ctypes.data, None, K.stream_handle(field.device))
+                  *bins._table_ptrs(), bins.scalars.data_ptr(), field.data_ptr(), lower.ctypes.data,
+                  cell.ctypes.data, num.ctypes.data, code.ctypes.data, None, K.stream_handle(field.device))
 
 
 def gather_binned(bins: MarkerBins, field: torch.Tensor, result: torch.Tensor) -> None:
@@ -137,10 +159,12 @@ def gather_binned(bins: MarkerBins, field: torch.Tensor, result: torch.Tensor) -
     if tuple(result.shape) != (ncomp, bins.n) or not result.is_contiguous() or result.dtype != bins.positions.dtype:
         raise ValueError(f"result must be a contiguous ({ncomp}, {bins.n}) {bins.positions.dtype} tensor")
     with torch.cuda.device(field.device):
+        # the gather builds its tiles' work lists itself: the build hides behind its footprint
+        # load, and loading a list costs it registers (measured slower with the lists)
         _lib.call("mpf_gather_binned", bins.kind, bins.degree, ndim, bins.positions.data_ptr(),
-                  K._dtype_code(bins.positions), bins.n, bins.start.data_ptr(), field.data_ptr(), ncomp,
-                  result.data_ptr(), lower.ctypes.data, cell.ctypes.data, num.ctypes.data, code.ctypes.data, None,
-                  K.stream_handle(field.device))
+                  K._dtype_code(bins.positions), bins.n, bins.start.data_ptr(), None, None,
+                  field.data_ptr(), ncomp, result.data_ptr(), lower.ctypes.data, cell.ctypes.data, num.ctypes.data,
+                  code.ctypes.data, None, K.stream_handle(field.device))
 
 
 def with_values(bins: MarkerBins, values, scale_exp: int = _lib.SCALE_AUTO) -> MarkerBins:
@@ -157,7 +181,8 @@ def with_values(bins: MarkerBins, values, scale_exp: int = _lib.SCALE_AUTO) -> M
     with torch.cuda.device(val.device):
         _lib.call("mpf_fixed_scale", val.data_ptr(), vs, K._dtype_code(val), bins.n, val.shape[0], int(scale_exp),
                   scalars.data_ptr(), None, K.stream_handle(val.device))
-    return MarkerBins(bins.kind, bins.degree, bins.geometry, bins.positions, sval, bins.order, bins.start, scalars)
+    return MarkerBins(bins.kind, bins.degree, bins.geometry, bins.positions, sval, bins.order, bins.start, scalars,
+                      bins.tables, bins.table_counts)
 
 
 def gather_derivative_binned(bins: MarkerBins, field: torch.Tensor, coef: torch.Tensor) -> torch.Tensor:

@@ -547,6 +547,52 @@ __device__ __forceinline__ void load_offsets(const uint32_t *__restrict__ start,
   }
 }
 
+// The tile's item table made once per binning (item_tables_kernel) into LDS: its
+// SWEEPS * TC entries as 16-byte loads, issued together with the offsets' loads.
+template <int BLOCK, int TC>
+__device__ __forceinline__ void load_offsets_and_items(const uint32_t *__restrict__ start,
+                                                       const uint16_t *__restrict__ table, uint32_t *cfirst,
+                                                       uint16_t *items) {
+  constexpr int E = (TC + 1 + BLOCK - 1) / BLOCK, Q = SWEEPS * TC * 2 / 16 / BLOCK;
+  static_assert(SWEEPS * TC * 2 % (16 * BLOCK) == 0, "table copy");
+  uint32_t v[E];
+  uint4 q[Q];
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
+    const int i = threadIdx.x + u * BLOCK;
+    v[u] = start[i <= TC ? i : TC];
+  }
+#pragma unroll
+  for (int u = 0; u < Q; ++u) q[u] = reinterpret_cast<const uint4 *>(table)[threadIdx.x + u * BLOCK];
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
+    const int i = threadIdx.x + u * BLOCK;
+    if (i <= TC) cfirst[i] = v[u];
+  }
+#pragma unroll
+  for (int u = 0; u < Q; ++u) reinterpret_cast<uint4 *>(items)[threadIdx.x + u * BLOCK] = q[u];
+}
+
+// Item tables of every 3D tile, in the order build_items lists them for workgroups of
+// BLOCK work-items: table[tile * SWEEPS * TC + i] and counts[tile].  For bins kept across
+// deposits and gathers, which then load the table instead of building it.
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) item_tables_kernel(const uint32_t *__restrict__ start,
+                                                            uint16_t *__restrict__ table,
+                                                            uint32_t *__restrict__ counts) {
+  constexpr int TC = TILE_CELLS, PARTS = TC / BLOCK;
+  __shared__ uint32_t cfirst[TC + 1];
+  __shared__ __align__(16) uint16_t items[SWEEPS * TC];
+  __shared__ unsigned long long wave_sums[(BLOCK / 64) * PARTS];
+  const int64_t tile = blockIdx.x;
+  load_offsets<BLOCK, TC>(start + tile * TC, cfirst);
+  __syncthreads();
+  const uint32_t total = build_items<BLOCK, TC>(cfirst, items, wave_sums);
+  for (uint32_t i = threadIdx.x; i < (uint32_t)(SWEEPS * TC); i += BLOCK)
+    table[tile * SWEEPS * TC + i] = i < total ? items[i] : (uint16_t)0;
+  if (threadIdx.x == 0) counts[tile] = total;
+}
+
 struct Item {
   int64_t begin;  // first marker (binned index)
   uint32_t size;  // markers, at least one
@@ -697,7 +743,9 @@ template <class T, int KIND, int P, int C, int BLOCK>
 __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ pos, const T *__restrict__ val,
                                                           int64_t n, const uint32_t *__restrict__ start, Grid g,
                                                           Tiling tl, double *__restrict__ field,
-                                                          const double *__restrict__ slot) {
+                                                          const double *__restrict__ slot,
+                                                          const uint16_t *__restrict__ tables,
+                                                          const uint32_t *__restrict__ table_counts) {
   using Sh = StencilShape<KIND, P>;
   constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, TC = TILE_CELLS;
   constexpr int S = Sh::S, OFF = Sh::OFF;
@@ -705,7 +753,7 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
   constexpr int PARTS = TC / BLOCK;
   __shared__ unsigned long long acc[C * L];
   __shared__ uint32_t cfirst[TC + 1];
-  __shared__ uint16_t items[SWEEPS * TC];
+  __shared__ __align__(16) uint16_t items[SWEEPS * TC];
   __shared__ unsigned long long wave_sums[(BLOCK / 64) * PARTS];
   __shared__ long long rows[LX * LY];
 
@@ -720,11 +768,15 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
   const int64_t tile_key = (((int64_t)tx * tl.nty + ty) * tl.ntz + tz) * TC;
   const int t = threadIdx.x;
 
-  load_offsets<BLOCK, TC>(start + tile_key, cfirst);
+  const int64_t tile_id = tile_key / TC;
+  if (tables)
+    load_offsets_and_items<BLOCK, TC>(start + tile_key, tables + tile_id * SWEEPS * TC, cfirst, items);
+  else
+    load_offsets<BLOCK, TC>(start + tile_key, cfirst);
   for (int i = t; i < C * L; i += BLOCK) acc[i] = 0ull;
   footprint_rows<LX, LY, OFF, C, BLOCK>(g, x0, y0, z0, false, rows);
   __syncthreads();
-  const uint32_t total = build_items<BLOCK, TC>(cfirst, items, wave_sums);
+  const uint32_t total = tables ? table_counts[tile_id] : build_items<BLOCK, TC>(cfirst, items, wave_sums);
 
   const double scale = slot[0];
   // markers per register pass: the exact sums stay below 2^51 quanta
@@ -1064,7 +1116,8 @@ template <int KIND, int P, int C, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) gather_items(const float *__restrict__ pos, int64_t n,
                                                       const uint32_t *__restrict__ start, Grid g, int ntx, int nty,
                                                       int ntz, const double *__restrict__ field,
-                                                      float *__restrict__ res) {
+                                                      float *__restrict__ res, const uint16_t *__restrict__ tables,
+                                                      const uint32_t *__restrict__ table_counts) {
   using Sh = StencilShape<KIND, P>;
   using Slot = typename NodeSlot<C>::type;
   constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, TC = TILE_CELLS;
@@ -1074,7 +1127,7 @@ __global__ void __launch_bounds__(BLOCK) gather_items(const float *__restrict__ 
   static_assert(C == 1 || C == 3, "float32 gather tiles hold 1 or 3 components");
   __shared__ Slot tile[L];
   __shared__ uint32_t cfirst[TC + 1];
-  __shared__ uint16_t items[SWEEPS * TC];
+  __shared__ __align__(16) uint16_t items[SWEEPS * TC];
   __shared__ unsigned long long wave_sums[(BLOCK / 64) * PARTS];
   __shared__ long long rows[LX * LY];
 
@@ -1095,11 +1148,17 @@ __global__ void __launch_bounds__(BLOCK) gather_items(const float *__restrict__ 
     constexpr int EO = (TC + 1 + BLOCK - 1) / BLOCK;
     constexpr int E = (L * C + BLOCK - 1) / BLOCK, CV = C == 1 ? 1 : 4, ROW = LZ * C;
     const bool plain_z = g.code[2] != 2 && z0 - OFF >= 0 && z0 - OFF + LZ <= num2;
+    constexpr int Q = SWEEPS * TC * 2 / 16 / BLOCK;
     uint32_t o[EO];
 #pragma unroll
     for (int u = 0; u < EO; ++u) {
       const int i = t + u * BLOCK;
       o[u] = start[tile_key + (i <= TC ? i : TC)];
+    }
+    uint4 tq[Q];
+    if (tables) {
+#pragma unroll
+      for (int u = 0; u < Q; ++u) tq[u] = reinterpret_cast<const uint4 *>(tables + (int64_t)id * SWEEPS * TC)[t + u * BLOCK];
     }
     double v[E];
 #pragma unroll
@@ -1123,9 +1182,13 @@ __global__ void __launch_bounds__(BLOCK) gather_items(const float *__restrict__ 
         flat[i * CV + q] = (float)v[u];
       }
     }
+    if (tables) {
+#pragma unroll
+      for (int u = 0; u < Q; ++u) reinterpret_cast<uint4 *>(items)[t + u * BLOCK] = tq[u];
+    }
   }
   __syncthreads();
-  const uint32_t total = build_items<BLOCK, TC>(cfirst, items, wave_sums);
+  const uint32_t total = tables ? table_counts[id] : build_items<BLOCK, TC>(cfirst, items, wave_sums);
 
   // lookahead as in scatter_tiled: unconditional loads, the first ones drained before the loop
   uint32_t it = t;
@@ -1781,7 +1844,8 @@ extern "C" int mpf_bin_markers(int kind, int degree, int ndim, const void *posit
 
 template <class T, int KIND, int P, int C>
 static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const uint32_t *start, const Grid &g,
-                             double *field, const double *scalars, hipStream_t stream) {
+                             double *field, const double *scalars, const uint16_t *tables,
+                             const uint32_t *table_counts, hipStream_t stream) {
   constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, BLOCK = 256;
   Tiling tl;
   tl.tx = TX;
@@ -1812,7 +1876,7 @@ static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const 
         ProfileScope prof("scatter_tiled", stream);
         hipLaunchKernelGGL((scatter_tiled<T, KIND, P, C, BLOCK>), dim3(blocks),
                            dim3(BLOCK), 0, stream,
-                           (const T *)pos, (const T *)val, n, start, g, tl, field, scalars);
+                           (const T *)pos, (const T *)val, n, start, g, tl, field, scalars, tables, table_counts);
         MPF_LAUNCH_CHECK();
       }
   return MPF_OK;
@@ -1820,7 +1884,7 @@ static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const 
 
 template <class T, int KIND, int P, int C>
 static int run_gather_tiled(const void *pos, int64_t n, const uint32_t *start, const Grid &g, const double *field,
-                            void *res, hipStream_t stream) {
+                            void *res, const uint16_t *tables, const uint32_t *table_counts, hipStream_t stream) {
   constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, BLOCK = 256;
   const int ntx = (int)((g.num[0] + TX - 1) / TX), nty = (int)((g.num[1] + TY - 1) / TY),
             ntz = (int)((g.num[2] + TZ - 1) / TZ);
@@ -1828,7 +1892,7 @@ static int run_gather_tiled(const void *pos, int64_t n, const uint32_t *start, c
     ProfileScope prof("gather_tiled", stream);
     if constexpr (sizeof(T) == 4)
       hipLaunchKernelGGL((gather_items<KIND, P, C, BLOCK>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
-                         (const float *)pos, n, start, g, ntx, nty, ntz, field, (float *)res);
+                         (const float *)pos, n, start, g, ntx, nty, ntz, field, (float *)res, tables, table_counts);
     else
       hipLaunchKernelGGL((gather_tiled<T, KIND, P, C, BLOCK>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
                          (const T *)pos, n, start, g, ntx, nty, ntz, field, (T *)res);
@@ -1860,7 +1924,8 @@ static int run_gather_tiled(const void *pos, int64_t n, const uint32_t *start, c
  * Returns MPF_ERR_UNSUPPORTED for shapes the tiled kernels do not cover. */
 extern "C" int mpf_scatter_binned(int kind, int degree, int ndim, const void *sorted_positions,
                                   const void *sorted_values, int dtype, int64_t n, int64_t ncomp,
-                                  const uint32_t *start, const double *scalars, double *field, const double *lower,
+                                  const uint32_t *start, const uint16_t *item_tables, const uint32_t *item_counts,
+                                  const double *scalars, double *field, const double *lower,
                                   const double *cell_size, const int64_t *num_nodes, const int64_t *code,
                                   mpf_context *ctx, mpf_stream stream_) {
   (void)ctx;
@@ -1886,7 +1951,8 @@ extern "C" int mpf_scatter_binned(int kind, int degree, int ndim, const void *so
     }
 #undef SC2
   }
-#define SC(T, K, P, C) run_scatter_tiled<T, K, P, C>(sorted_positions, sorted_values, n, start, g, field, scalars, stream)
+#define SC(T, K, P, C) \
+  run_scatter_tiled<T, K, P, C>(sorted_positions, sorted_values, n, start, g, field, scalars, item_tables, item_counts, stream)
   if (dtype == F32) {
     TILED_CASES(SC, float)
   } else {
@@ -1899,7 +1965,8 @@ extern "C" int mpf_scatter_binned(int kind, int degree, int ndim, const void *so
 
 /* Gather at binned markers with the tiled kernels: result (ncomp, n) in the sorted order. */
 extern "C" int mpf_gather_binned(int kind, int degree, int ndim, const void *sorted_positions, int dtype, int64_t n,
-                                 const uint32_t *start, const double *field, int64_t ncomp, void *result,
+                                 const uint32_t *start, const uint16_t *item_tables, const uint32_t *item_counts,
+                                 const double *field, int64_t ncomp, void *result,
                                  const double *lower, const double *cell_size, const int64_t *num_nodes,
                                  const int64_t *code, mpf_context *ctx, mpf_stream stream_) {
   (void)ctx;
@@ -1923,7 +1990,8 @@ extern "C" int mpf_gather_binned(int kind, int degree, int ndim, const void *sor
     }
 #undef GA2
   }
-#define GA(T, K, P, C) run_gather_tiled<T, K, P, C>(sorted_positions, n, start, g, field, result, stream)
+#define GA(T, K, P, C) \
+  run_gather_tiled<T, K, P, C>(sorted_positions, n, start, g, field, result, item_tables, item_counts, stream)
   if (dtype == F32) {
     TILED_CASES(GA, float)
   } else {
@@ -2007,5 +2075,24 @@ extern "C" int mpf_fixed_scale(const void *values, int64_t val_stride, int dtype
   int rc = prepare_scale(ctx, values, val_stride, dtype, ncomp, n, 1.0, scale_exp, stream);
   if (rc) return rc;
   MPF_CHECK(hipMemcpyAsync(scalars, scale_slot(ctx), 24, hipMemcpyDeviceToDevice, stream));
+  return MPF_OK;
+}
+
+/* The item tables of 3D bins (tile order): tables[tile * 4096 + i] (uint16) and
+ * counts[tile], made once so that mpf_scatter_binned / mpf_gather_binned load a tile's
+ * work list instead of building it in every launch.  tiles = bins / 1024. */
+extern "C" int mpf_bin_items(int ndim, const int64_t *num_nodes, const uint32_t *start, uint16_t *tables,
+                             uint32_t *counts, mpf_context *ctx, mpf_stream stream_) {
+  (void)ctx;
+  if (ndim != 3) {
+    set_error("bin_items: item tables are made for 3D bins");
+    return MPF_ERR_UNSUPPORTED;
+  }
+  const int64_t tiles = bins_of(3, num_nodes) / TILE_CELLS;
+  hipStream_t stream = (hipStream_t)stream_;
+  if (tiles > 0) {
+    hipLaunchKernelGGL(item_tables_kernel<256>, dim3((unsigned)tiles), dim3(256), 0, stream, start, tables, counts);
+    MPF_LAUNCH_CHECK();
+  }
   return MPF_OK;
 }

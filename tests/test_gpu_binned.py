@@ -71,6 +71,29 @@ def test_binned_scatter_and_gather_match_oracle(kind, degree, ncomp, num, code):
     binning.scatter_binned(bins, got)
     err = np.abs(got.cpu().numpy() - want).max() / np.abs(want).max()
     assert err < 1e-12, err
+    # the tiles' work lists made once give the same deposit and gather, bit for bit
+    tabled = binning.bin_markers(cuda(pos), cuda(val), (kind, degree), g)
+    tabled.values.copy_(bins.values)
+    tabled.positions.copy_(bins.positions)
+    tabled.start.copy_(bins.start)
+    tabled.prepare_items()
+    again = cuda(field0)
+    binning.scatter_binned(tabled, again)
+    assert torch.equal(again, got)
+    # and the float32 gather kernel through the entry point with the lists
+    from mpifield_amd import _lib
+
+    f32_bins = binning.bin_markers(cuda(pos.astype(np.float32)), None, (kind, degree), g)
+    r1 = torch.empty((ncomp, pos.shape[1]), dtype=torch.float32, device="cuda")
+    r2 = torch.empty_like(r1)
+    binning.gather_binned(f32_bins, got, r1)
+    f32_bins.prepare_items()
+    ndim_, lw, cs, nm, cd, _ = K._split(g)
+    _lib.call("mpf_gather_binned", kind, degree, 3, f32_bins.positions.data_ptr(), _lib.F32, f32_bins.n,
+              f32_bins.start.data_ptr(), f32_bins.tables.data_ptr(), f32_bins.table_counts.data_ptr(), got.data_ptr(),
+              ncomp, r2.data_ptr(), lw.ctypes.data, cs.ctypes.data, nm.ctypes.data, cd.ctypes.data, None,
+              K.stream_handle(got.device))
+    torch.testing.assert_close(r1, r2, rtol=0, atol=0, equal_nan=True)  # NaN positions give NaN
 
     live = ~dead
     want_g = np.zeros((ncomp, int(live.sum())))
