@@ -51,10 +51,11 @@ def parse() -> argparse.Namespace:
     p.add_argument("--only-headline", action="store_true", help="time the headline step only (for rocprofv3)")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only copy MIB MiB device-to-device (the counter calibration for rocprofv3 --pmc)")
-    p.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5),
+    p.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6, 7),
                    help="2: the headline (default); 3: 2D quadratic-edge path deposit, 1024^2, 2^25 paths; "
                    "4: 4096 batched 2D patches, ~2e8 markers; 5: differentiable 3D cubic gather of scatter, "
-                   "128^3, 2^24 markers (one JSON line each)")
+                   "128^3, 2^24 markers; 6: 2D Whitney path deposit, same paths as 3; 7: halo fold + fill, "
+                   "256^3 x 3, halo 2 (one JSON line each)")
     return p.parse_args()
 
 
@@ -160,6 +161,29 @@ def time_events(torch, fn, steps: int, warmup: int) -> float:
     return start.elapsed_time(stop) / steps
 
 
+def cpu_cores() -> int:
+    """Host threads for the oracle's OpenMP loops: the box's share (16) at most."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_seconds(fn) -> float:
+    t0 = time.perf_counter()
+    fn()
+    return time.perf_counter() - t0
+
+
+def paths_2d(torch, n: int, nodes: int):
+    """The 1024^2 path workload: start uniform in [0,1)^2, per-axis displacement
+    clip(N(0, h/4), +-h/2), end = start + displacement (not wrapped), unit charges."""
+    h = 1.0 / nodes
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    start = torch.rand((2, n), generator=gen, device="cuda", dtype=torch.float32)
+    d = (torch.randn((2, n), generator=gen, device="cuda") * (0.25 * h)).clamp(-0.5 * h, 0.5 * h)
+    end = (start.double() + d.double()).float()
+    q = torch.ones(n, device="cuda", dtype=torch.float32)
+    return (0.0, 0.0, h, h, nodes, nodes, 1, 1), start, end, q
+
+
 def config3(torch, args) -> dict:
     """2D charge-conserving path deposit (QUADRATIC_EDGES, spline degree 2) on a 1024^2 periodic
     grid: 2^25 float32 paths, start uniform, per-axis displacement clip(N(0, h/4), +-h/2),
@@ -169,13 +193,7 @@ def config3(torch, args) -> dict:
     from mpifield_amd import kernels as K
 
     nodes, n = 1024, 1 << 25
-    h = 1.0 / nodes
-    g = (0.0, 0.0, h, h, nodes, nodes, 1, 1)
-    gen = torch.Generator(device="cuda").manual_seed(3)
-    start = torch.rand((2, n), generator=gen, device="cuda", dtype=torch.float32)
-    d = (torch.randn((2, n), generator=gen, device="cuda") * (0.25 * h)).clamp(-0.5 * h, 0.5 * h)
-    end = (start.double() + d.double()).float()
-    q = torch.ones(n, device="cuda", dtype=torch.float32)
+    g, start, end, q = paths_2d(torch, n, nodes)
     j = torch.zeros((nodes, nodes, 2), dtype=torch.float64, device="cuda")
     out = {}
     for label, mode in (("fixed", _lib.MODE_FIXED), ("atomic", _lib.MODE_ATOMIC)):
@@ -189,6 +207,15 @@ def config3(torch, args) -> dict:
     K.scatter(2, 2, end, q[None], rho1, g, mode=_lib.MODE_FIXED)
     residual = (K.divergence(j, (1, 1)) - (rho1 - rho0)[..., 0]).abs().max().item()
     alg = n * (16 + 4) + 2 * nodes * nodes * 2 * 8
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        s_h, e_h, q_h = start.double().cpu().numpy(), end.double().cpu().numpy(), q.double().cpu().numpy()
+        edges = np.zeros((nodes, nodes, 2))
+        sec = cpu_seconds(lambda: orc.deposit_spline(s_h, e_h, q_h, edges, *g, degree=2))
+        cpu = {"value": n / sec, "unit": "paths/s", "cores": 1, "kind": "port",
+               "sample": f"all {n} paths: {sec:.2f} s (the reference's spline deposit is serial)"}
     return {
         "metric": "paths/s, 2D QUADRATIC_EDGES path deposit 1024^2, 2^25 paths", "value": n / (out["fixed"] * 1e-3),
         "unit": "paths/s", "n_gpus": 1, "steps": args.steps, "ms_per_step": out["fixed"], "higher_is_better": True,
@@ -198,7 +225,86 @@ def config3(torch, args) -> dict:
         "roofline": {"bound": "hbm", "achieved": alg / (out["fixed"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg / (out["fixed"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
         "variants": {"fp64_atomic_ms": out["atomic"], "fp64_atomic_paths_per_s": n / (out["atomic"] * 1e-3)},
-        "continuity_max_abs_residual": residual, "total_charge": float(n),
+        "continuity_max_abs_residual": residual, "total_charge": float(n), "cpu_baseline": cpu,
+    }
+
+
+def config6(torch, args) -> dict:
+    """The WHITNEY (degree-1 edge) path deposit of the same 1024^2 workload: fixed point and
+    float64 atomics, the continuity G^T j = rho(end) - rho(start) with the linear node charge."""
+    from mpifield_amd import _lib
+    from mpifield_amd import kernels as K
+
+    nodes, n = 1024, 1 << 25
+    g, start, end, q = paths_2d(torch, n, nodes)
+    j = torch.zeros((nodes, nodes, 2), dtype=torch.float64, device="cuda")
+    out = {}
+    for label, mode in (("fixed", _lib.MODE_FIXED), ("atomic", _lib.MODE_ATOMIC)):
+        out[label] = time_events(torch, lambda: K.deposit_paths(0, 1, start, end, q, j, g, mode=mode),
+                                 max(1, args.steps // 2), 1)
+    j.zero_()
+    K.deposit_paths(0, 1, start, end, q, j, g, mode=_lib.MODE_FIXED)
+    rho0 = torch.zeros((nodes, nodes, 1), dtype=torch.float64, device="cuda")
+    rho1 = torch.zeros_like(rho0)
+    K.scatter(0, 1, start, q[None], rho0, g, mode=_lib.MODE_FIXED)
+    K.scatter(0, 1, end, q[None], rho1, g, mode=_lib.MODE_FIXED)
+    residual = (K.divergence(j, (1, 1)) - (rho1 - rho0)[..., 0]).abs().max().item()
+    alg = n * (16 + 4) + 2 * nodes * nodes * 2 * 8
+    best = min(out.values())
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        s_h, e_h, q_h = start.double().cpu().numpy(), end.double().cpu().numpy(), q.double().cpu().numpy()
+        edges = np.zeros((nodes, nodes, 2))
+        cores = cpu_cores()
+        sec = cpu_seconds(lambda: orc.deposit_whitney(s_h, e_h, q_h, edges, *g, threads=cores))
+        cpu = {"value": n / sec, "unit": "paths/s", "cores": cores, "kind": "port",
+               "sample": f"all {n} paths: {sec:.2f} s (OpenMP with private copies, as the reference)"}
+    return {
+        "metric": "paths/s, 2D WHITNEY path deposit 1024^2, 2^25 paths", "value": n / (best * 1e-3),
+        "unit": "paths/s", "n_gpus": 1, "ms_per_step": best, "higher_is_better": True, "dtype": "int64",
+        "data": "synthetic",
+        "config": {"workload": f"2D Whitney charge-conserving path deposit, 1024^2 periodic, {n} float32 paths, "
+                   "displacements clip(N(0,h/4), h/2); one thread per path"},
+        "roofline": {"bound": "hbm", "achieved": alg / (best * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / (best * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "variants": {"fixed_ms": out["fixed"], "fp64_atomic_ms": out["atomic"]},
+        "continuity_max_abs_residual": residual, "cpu_baseline": cpu,
+    }
+
+
+def config7(torch, args) -> dict:
+    """Halo fold then fill of one block's storage: 256^3 interior nodes, halo 2 on every side,
+    3 components, periodic (the exchange a single block does locally)."""
+    from mpifield_amd import kernels as K
+
+    inner, h, C = 256, 2, 3
+    side = inner + 2 * h
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    data = torch.randn((side, side, side, C), generator=gen, device="cuda", dtype=torch.float64)
+    per = (True, True, True)
+    fold = time_events(torch, lambda: K.halo("fold", data, h, per), args.steps, 2)
+    fill = time_events(torch, lambda: K.halo("fill", data, h, per), args.steps, 2)
+    slab = h * side * side * C * 8  # one halo slab of one axis side
+    alg_fold, alg_fill = 3 * 2 * slab * 4, 3 * 2 * slab * 2  # per axis and side: fold r/w halo + r/w interior
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        host = data.cpu().numpy()
+        reps = 20
+        sec = cpu_seconds(lambda: [(orc.halo_fold(host, h, per), orc.halo_fill(host, h, per)) for _ in range(reps)])
+        cpu = {"value": reps * (alg_fold + alg_fill) / sec / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": f"the same storage, fold then fill {reps} times: {sec:.2f} s"}
+    total = fold + fill
+    return {
+        "metric": "GB/s, halo fold + fill, 256^3 x 3 float64 storage, halo 2", "value": (alg_fold + alg_fill) / (total * 1e-3) / 1e9,
+        "unit": "GB/s", "n_gpus": 1, "ms_per_step": total, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{side}^3 x {C} float64 storage ({inner}^3 interior, halo {h}), periodic, fold then fill"},
+        "roofline": {"bound": "hbm", "achieved": (alg_fold + alg_fill) / (total * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": (alg_fold + alg_fill) / (total * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "fold_ms": fold, "fill_ms": fill, "cpu_baseline": cpu,
     }
 
 
@@ -240,6 +346,27 @@ def config4(torch, args) -> dict:
     sc = time_events(torch, lambda: batch.scatter(pos, val, fields), max(1, args.steps // 4), 1)
     ga = time_events(torch, lambda: batch.gather(pos, fields, result), max(1, args.steps // 4), 1)
     alg = total * 12 * 2 + int((3 * sides.astype(np.int64) ** 2 * 8).sum())
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        cores, names = cpu_cores(), {1: "linear", 2: "quadratic", 3: "spline"}
+        done, sec, first = 0, 0.0, 0
+        pos_h = pos.cpu().numpy()
+        for i in range(count):
+            m_i = int(markers[i])
+            p_h = pos_h[:, first:first + m_i].astype(np.float64)
+            v_h = np.ones((1, m_i))
+            geo = (*lows[i], cells[i], cells[i], int(sides[i]), int(sides[i]), 0, 0)
+            f_h = np.zeros((int(sides[i]), int(sides[i]), 1))
+            r_h = np.zeros((1, m_i))
+            kind, deg = names[int(degrees[i])], int(degrees[i])
+            sec += cpu_seconds(lambda: (orc.scatter(kind, p_h, v_h, f_h, *geo, degree=deg, threads=cores),
+                                        orc.gather(kind, p_h, f_h, r_h, *geo, degree=deg, threads=cores)))
+            first += m_i
+            done += m_i
+        cpu = {"value": done / sec, "unit": "markers/s", "cores": cores, "kind": "port",
+               "sample": f"all {count} patches ({done} markers) one by one, scatter then gather: {sec:.2f} s"}
     return {
         "metric": "markers/s, 4096 batched 2D patches scatter+gather, ~2e8 markers", "value": total / ((sc + ga) * 1e-3),
         "unit": "markers/s", "n_gpus": 1, "ms_per_step": sc + ga, "higher_is_better": True, "dtype": "int64",
@@ -248,7 +375,7 @@ def config4(torch, args) -> dict:
                    "(log-uniform per patch, rescaled to ~2e8), one component, one persistent launch each way"},
         "roofline": {"bound": "hbm", "achieved": alg / ((sc + ga) * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg / ((sc + ga) * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
-        "scatter_ms": sc, "gather_ms": ga, "markers": total,
+        "scatter_ms": sc, "gather_ms": ga, "markers": total, "cpu_baseline": cpu,
     }
 
 
@@ -282,12 +409,27 @@ def config5(torch, args) -> dict:
         g.double().sum().backward()
 
     ms = time_events(torch, step, max(1, args.steps // 4), 1)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        sample, cores = n, cpu_cores()
+        p_h = pos[:, :sample].detach().double().cpu().numpy()
+        f_h = field.detach().cpu().numpy().copy()
+        r_h = np.zeros((1, sample))
+        geo = (0.0, 0.0, 0.0, cell, cell, cell, nodes, nodes, nodes, 1, 1, 1)
+        sec = cpu_seconds(lambda: (orc.scatter("spline", p_h, np.ones((1, sample)), f_h, *geo, degree=3, threads=cores),
+                                   orc.gather("spline", p_h, f_h, r_h, *geo, degree=3, threads=cores)))
+        cpu = {"value": sample / sec, "unit": "markers/s", "cores": cores, "kind": "port",
+               "sample": f"all {sample} markers, forward only (scatter then gather; the reference has no "
+                         f"differentiation): {sec:.2f} s"}
     return {
         "metric": "markers/s, differentiable 3D cubic gather(scatter) 128^3, 2^24 markers, forward+backward",
         "value": n / (ms * 1e-3), "unit": "markers/s", "n_gpus": 1, "ms_per_step": ms, "higher_is_better": True,
         "dtype": "int64", "data": "synthetic",
         "config": {"workload": f"128^3 periodic, {n} float32 markers, CUBIC, field = 8 Gaussians; scatter, gather, "
                    "backward to positions, weights and field"},
+        "cpu_baseline": cpu,
     }
 
 
@@ -309,7 +451,7 @@ def main() -> None:
         return
 
     if args.config != 2:
-        line = {3: config3, 4: config4, 5: config5}[args.config](torch, args)
+        line = {3: config3, 4: config4, 5: config5, 6: config6, 7: config7}[args.config](torch, args)
         print(json.dumps(line), flush=True)
         return
 
