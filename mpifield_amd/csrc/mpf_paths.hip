@@ -36,107 +36,139 @@ __device__ __forceinline__ void deposit(double *edges, unsigned long long *accum
     atomicAdd(accum + at, to_fixed(v * scale));
 }
 
+// One path's Whitney deposit (deposit_paths_{1,2,3}d_pyccel.py): the path, clamped to the
+// grid on bounded axes, cut at every grid plane it crosses; sink(idx, axis, v) receives every
+// contribution, idx[d] the edge's node index along axis d (wrapped by linear_axis).
+template <int NDIM, class Sink>
+__device__ __forceinline__ void whitney_path(const double *a, const double *b, double charge, const Grid &g,
+                                             Sink &sink) {
+  double delta[3] = {0, 0, 0}, step[3] = {0, 0, 0}, plane[3] = {0, 0, 0}, x0[3] = {0, 0, 0}, x1[3] = {0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    delta[d] = b[d] - a[d];
+    step[d] = 1.0;
+    plane[d] = floor(a[d]) + 1.0;
+    if (delta[d] < 0.0) {
+      step[d] = -1.0;
+      plane[d] = ceil(a[d]) - 1.0;
+    }
+    x0[d] = clamp_to_axis(a[d], g.num[d], g.code[d]);
+  }
+  double time = 0.0;
+  while (time < 1.0) {
+    double following = 1.0;
+    double times[3] = {2.0, 2.0, 2.0};
+    if (NDIM == 1) {
+      following = plane_time(plane[0], a[0], delta[0]);
+      if (following < 1.0) {
+        x1[0] = clamp_to_axis(a[0] + following * delta[0], g.num[0], g.code[0]);
+        plane[0] += step[0];
+      } else {
+        following = 1.0;
+        x1[0] = clamp_to_axis(b[0], g.num[0], g.code[0]);
+      }
+    } else {
+#pragma unroll
+      for (int d = 0; d < NDIM; ++d) {
+        times[d] = plane_time(plane[d], a[d], delta[d]);
+        if (times[d] < following) following = times[d];
+      }
+#pragma unroll
+      for (int d = 0; d < NDIM; ++d)
+        x1[d] = following < 1.0 ? clamp_to_axis(a[d] + following * delta[d], g.num[d], g.code[d])
+                                : clamp_to_axis(b[d], g.num[d], g.code[d]);
+#pragma unroll
+      for (int d = 0; d < NDIM; ++d)
+        if (times[d] <= following) plane[d] += step[d];
+    }
+    int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    double f[3] = {0, 0, 0}, dd[3] = {0, 0, 0}, along[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+      double mid = 0.5 * (x0[d] + x1[d]);
+      int64_t nd[2];
+      double wd[2];
+      linear_axis(mid, g.num[d], g.code[d], nd, wd);  // periodic_position + left/right node + fraction
+      lo[d] = nd[0];
+      hi[d] = nd[1];
+      f[d] = wd[1];
+      dd[d] = x1[d] - x0[d];
+      along[d] = charge * dd[d];
+    }
+#define PUT(i, j, k, c, v)                     \
+  {                                            \
+    const int64_t idx_[3] = {(i), (j), (k)};   \
+    sink(idx_, (c), (v));                      \
+  }
+    if (NDIM == 1) {
+      PUT(lo[0], 0, 0, 0, charge * (x1[0] - x0[0]));
+    } else if (NDIM == 2) {
+      PUT(lo[0], lo[1], 0, 0, (1.0 - f[1]) * along[0]);
+      PUT(lo[0], hi[1], 0, 0, f[1] * along[0]);
+      PUT(lo[0], lo[1], 0, 1, (1.0 - f[0]) * along[1]);
+      PUT(hi[0], lo[1], 0, 1, f[0] * along[1]);
+    } else {
+      PUT(lo[0], lo[1], lo[2], 0, along[0] * edge_path_weight(f[1], f[2], dd[1], dd[2], 0, 0));
+      PUT(lo[0], hi[1], lo[2], 0, along[0] * edge_path_weight(f[1], f[2], dd[1], dd[2], 1, 0));
+      PUT(lo[0], lo[1], hi[2], 0, along[0] * edge_path_weight(f[1], f[2], dd[1], dd[2], 0, 1));
+      PUT(lo[0], hi[1], hi[2], 0, along[0] * edge_path_weight(f[1], f[2], dd[1], dd[2], 1, 1));
+      PUT(lo[0], lo[1], lo[2], 1, along[1] * edge_path_weight(f[0], f[2], dd[0], dd[2], 0, 0));
+      PUT(hi[0], lo[1], lo[2], 1, along[1] * edge_path_weight(f[0], f[2], dd[0], dd[2], 1, 0));
+      PUT(lo[0], lo[1], hi[2], 1, along[1] * edge_path_weight(f[0], f[2], dd[0], dd[2], 0, 1));
+      PUT(hi[0], lo[1], hi[2], 1, along[1] * edge_path_weight(f[0], f[2], dd[0], dd[2], 1, 1));
+      PUT(lo[0], lo[1], lo[2], 2, along[2] * edge_path_weight(f[0], f[1], dd[0], dd[1], 0, 0));
+      PUT(hi[0], lo[1], lo[2], 2, along[2] * edge_path_weight(f[0], f[1], dd[0], dd[1], 1, 0));
+      PUT(lo[0], hi[1], lo[2], 2, along[2] * edge_path_weight(f[0], f[1], dd[0], dd[1], 0, 1));
+      PUT(hi[0], hi[1], lo[2], 2, along[2] * edge_path_weight(f[0], f[1], dd[0], dd[1], 1, 1));
+    }
+#undef PUT
+    time = following;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) x0[d] = x1[d];
+  }
+}
+
+// Whether a Whitney path's rel coordinates are usable (every one finite and below 1e300).
+template <int NDIM>
+__device__ __forceinline__ bool whitney_finite(const double *a, const double *b) {
+  bool finite = true;
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d)
+    finite = finite && a[d] > -1.0e300 && a[d] < 1.0e300 && b[d] > -1.0e300 && b[d] < 1.0e300;
+  return finite;
+}
+
+template <int NDIM, int MODE>
+struct GlobalSink {
+  double *edges;
+  unsigned long long *accum;
+  double scale;
+  const Grid *g;
+  __device__ __forceinline__ void operator()(const int64_t *idx, int axis, double v) {
+    const int64_t at = (idx[0] * g->stride[0] + (NDIM > 1 ? idx[1] * g->stride[1] : 0) +
+                        (NDIM > 2 ? idx[2] * g->stride[2] : 0)) * NDIM + axis;
+    deposit<MODE>(edges, accum, at, v, scale);
+  }
+};
+
 template <class T, int NDIM, int MODE>
 __global__ void __launch_bounds__(256) whitney_kernel(const T *__restrict__ st, int64_t ss, const T *__restrict__ en,
                                                       int64_t es, const T *__restrict__ q, int64_t n, Grid g,
                                                       double *__restrict__ edges,
                                                       unsigned long long *__restrict__ accum,
                                                       const double *__restrict__ slot) {
-  const double scale = MODE == MODE_FIXED ? slot[0] : 1.0;
+  GlobalSink<NDIM, MODE> sink{edges, accum, MODE == MODE_FIXED ? slot[0] : 1.0, &g};
   for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
     const double charge = (double)q[p];
     if (charge == 0.0) continue;
     double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
-    bool finite = true;
 #pragma unroll
     for (int d = 0; d < NDIM; ++d) {
       a[d] = ((double)st[d * ss + p] - g.lower[d]) * g.inverse[d];
       b[d] = ((double)en[d * es + p] - g.lower[d]) * g.inverse[d];
-      finite = finite && a[d] > -1.0e300 && a[d] < 1.0e300 && b[d] > -1.0e300 && b[d] < 1.0e300;
     }
-    if (!finite) continue;
-    double delta[3] = {0, 0, 0}, step[3] = {0, 0, 0}, plane[3] = {0, 0, 0}, x0[3] = {0, 0, 0}, x1[3] = {0, 0, 0};
-#pragma unroll
-    for (int d = 0; d < NDIM; ++d) {
-      delta[d] = b[d] - a[d];
-      step[d] = 1.0;
-      plane[d] = floor(a[d]) + 1.0;
-      if (delta[d] < 0.0) {
-        step[d] = -1.0;
-        plane[d] = ceil(a[d]) - 1.0;
-      }
-      x0[d] = clamp_to_axis(a[d], g.num[d], g.code[d]);
-    }
-    double time = 0.0;
-    while (time < 1.0) {
-      double following = 1.0;
-      double times[3] = {2.0, 2.0, 2.0};
-      if (NDIM == 1) {
-        following = plane_time(plane[0], a[0], delta[0]);
-        if (following < 1.0) {
-          x1[0] = clamp_to_axis(a[0] + following * delta[0], g.num[0], g.code[0]);
-          plane[0] += step[0];
-        } else {
-          following = 1.0;
-          x1[0] = clamp_to_axis(b[0], g.num[0], g.code[0]);
-        }
-      } else {
-#pragma unroll
-        for (int d = 0; d < NDIM; ++d) {
-          times[d] = plane_time(plane[d], a[d], delta[d]);
-          if (times[d] < following) following = times[d];
-        }
-#pragma unroll
-        for (int d = 0; d < NDIM; ++d)
-          x1[d] = following < 1.0 ? clamp_to_axis(a[d] + following * delta[d], g.num[d], g.code[d])
-                                  : clamp_to_axis(b[d], g.num[d], g.code[d]);
-#pragma unroll
-        for (int d = 0; d < NDIM; ++d)
-          if (times[d] <= following) plane[d] += step[d];
-      }
-      int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-      double f[3] = {0, 0, 0}, dd[3] = {0, 0, 0}, along[3] = {0, 0, 0};
-#pragma unroll
-      for (int d = 0; d < NDIM; ++d) {
-        double mid = 0.5 * (x0[d] + x1[d]);
-        int64_t nd[2];
-        double wd[2];
-        linear_axis(mid, g.num[d], g.code[d], nd, wd);  // periodic_position + left/right node + fraction
-        lo[d] = nd[0];
-        hi[d] = nd[1];
-        f[d] = wd[1];
-        dd[d] = x1[d] - x0[d];
-        along[d] = charge * dd[d];
-      }
-      if (NDIM == 1) {
-        deposit<MODE>(edges, accum, lo[0], charge * (x1[0] - x0[0]), scale);
-      } else if (NDIM == 2) {
-        const int64_t s = g.stride[0];
-        deposit<MODE>(edges, accum, (lo[0] * s + lo[1]) * 2 + 0, (1.0 - f[1]) * along[0], scale);
-        deposit<MODE>(edges, accum, (lo[0] * s + hi[1]) * 2 + 0, f[1] * along[0], scale);
-        deposit<MODE>(edges, accum, (lo[0] * s + lo[1]) * 2 + 1, (1.0 - f[0]) * along[1], scale);
-        deposit<MODE>(edges, accum, (hi[0] * s + lo[1]) * 2 + 1, f[0] * along[1], scale);
-      } else {
-        const int64_t s0 = g.stride[0], s1 = g.stride[1];
-#define AT(i, j, k, c) (((i)*s0 + (j)*s1 + (k)) * 3 + (c))
-        deposit<MODE>(edges, accum, AT(lo[0], lo[1], lo[2], 0), along[0] * edge_path_weight(f[1], f[2], dd[1], dd[2], 0, 0), scale);
-        deposit<MODE>(edges, accum, AT(lo[0], hi[1], lo[2], 0), along[0] * edge_path_weight(f[1], f[2], dd[1], dd[2], 1, 0), scale);
-        deposit<MODE>(edges, accum, AT(lo[0], lo[1], hi[2], 0), along[0] * edge_path_weight(f[1], f[2], dd[1], dd[2], 0, 1), scale);
-        deposit<MODE>(edges, accum, AT(lo[0], hi[1], hi[2], 0), along[0] * edge_path_weight(f[1], f[2], dd[1], dd[2], 1, 1), scale);
-        deposit<MODE>(edges, accum, AT(lo[0], lo[1], lo[2], 1), along[1] * edge_path_weight(f[0], f[2], dd[0], dd[2], 0, 0), scale);
-        deposit<MODE>(edges, accum, AT(hi[0], lo[1], lo[2], 1), along[1] * edge_path_weight(f[0], f[2], dd[0], dd[2], 1, 0), scale);
-        deposit<MODE>(edges, accum, AT(lo[0], lo[1], hi[2], 1), along[1] * edge_path_weight(f[0], f[2], dd[0], dd[2], 0, 1), scale);
-        deposit<MODE>(edges, accum, AT(hi[0], lo[1], hi[2], 1), along[1] * edge_path_weight(f[0], f[2], dd[0], dd[2], 1, 1), scale);
-        deposit<MODE>(edges, accum, AT(lo[0], lo[1], lo[2], 2), along[2] * edge_path_weight(f[0], f[1], dd[0], dd[1], 0, 0), scale);
-        deposit<MODE>(edges, accum, AT(hi[0], lo[1], lo[2], 2), along[2] * edge_path_weight(f[0], f[1], dd[0], dd[1], 1, 0), scale);
-        deposit<MODE>(edges, accum, AT(lo[0], hi[1], lo[2], 2), along[2] * edge_path_weight(f[0], f[1], dd[0], dd[1], 0, 1), scale);
-        deposit<MODE>(edges, accum, AT(hi[0], hi[1], lo[2], 2), along[2] * edge_path_weight(f[0], f[1], dd[0], dd[1], 1, 1), scale);
-#undef AT
-      }
-      time = following;
-#pragma unroll
-      for (int d = 0; d < NDIM; ++d) x0[d] = x1[d];
-    }
+    if (!whitney_finite<NDIM>(a, b)) continue;
+    whitney_path<NDIM>(a, b, charge, g, sink);
   }
 }
 
@@ -399,18 +431,6 @@ __device__ __forceinline__ bool path_rel(const T *st, int64_t ss, const T *en, i
   return finite;
 }
 
-template <int NDIM, int MODE>
-struct GlobalSink {
-  double *edges;
-  unsigned long long *accum;
-  double scale;
-  const Grid *g;
-  __device__ __forceinline__ void operator()(const int64_t *idx, int axis, double v) {
-    const int64_t at = (idx[0] * g->stride[0] + (NDIM > 1 ? idx[1] * g->stride[1] : 0) +
-                        (NDIM > 2 ? idx[2] * g->stride[2] : 0)) * NDIM + axis;
-    deposit<MODE>(edges, accum, at, v, scale);
-  }
-};
 
 // One work-item per path; every contribution a global float64 atomic (MODE_ATOMIC) or an
 // int64 fixed-point atomic (MODE_FIXED).
@@ -689,8 +709,41 @@ __device__ __forceinline__ void path_into_tile(const double *r0, const double *r
   }
 }
 
-// At most 128 VGPRs (four waves per SIMD): the work-items walk long chains of dependent
-// float64 arithmetic, and more waves hide more of it.
+// Whitney contributions into a tile's LDS accumulator: the wrapped node index of each axis
+// folded into the footprint (one +-num step), a contribution outside it to the global
+// accumulator.
+template <int NDIM>
+struct WhitneyTileSink {
+  lds_u64 *lds;
+  unsigned long long *accum;
+  double scale;
+  const Grid *g;
+  const AxisTile *ax;
+  __device__ __forceinline__ void operator()(const int64_t *idx, int axis, double v) {
+    using PT = PathTile<NDIM>;
+    int l[3] = {0, 0, 0};
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+      int k = (int)idx[d] - ax[d].base;
+      if (ax[d].code == 1) k = k < 0 ? k + ax[d].num : (k >= ax[d].num ? k - ax[d].num : k);
+      in = in && k >= 0 && k < ax[d].len;
+      l[d] = k;
+    }
+    const unsigned long long f = to_fixed(v * scale);
+    if (in) {
+      __hip_atomic_fetch_add(lds + ((l[0] * PT::LY + l[1]) * PT::LZ + l[2]) * NDIM + axis, f, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+      const int64_t at = (idx[0] * g->stride[0] + (NDIM > 1 ? idx[1] * g->stride[1] : 0) +
+                          (NDIM > 2 ? idx[2] * g->stride[2] : 0)) * NDIM + axis;
+      atomicAdd(accum + at, f);
+    }
+  }
+};
+
+// DEG 1..3: spline edges; DEG 0: Whitney.  At most 128 VGPRs (four waves per SIMD): the
+// work-items walk long chains of dependent float64 arithmetic, and more waves hide more of it.
 template <class T, int NDIM, int DEG, int BLOCK>
 __global__ void __launch_bounds__(BLOCK, 4) spline_paths_tiled(const T *__restrict__ st, const T *__restrict__ rows,
                                                             int64_t n, const uint32_t *__restrict__ start, Grid g,
@@ -712,6 +765,7 @@ __global__ void __launch_bounds__(BLOCK, 4) spline_paths_tiled(const T *__restri
   __syncthreads();
   const uint32_t count = last - first, chunk = (count + BLOCK - 1) / BLOCK;
   const uint32_t lo = first + t * chunk, hi = lo + chunk < last ? lo + chunk : last;
+  WhitneyTileSink<NDIM> wsink{(lds_u64 *)lds, accum, scale, &g, ax};
   for (uint32_t p = lo; p < hi; ++p) {
     const double charge = (double)rows[p];
     if (charge == 0.0) continue;
@@ -724,8 +778,13 @@ __global__ void __launch_bounds__(BLOCK, 4) spline_paths_tiled(const T *__restri
       r0[d] = (s0 - g.lower[d]) * g.inverse[d];
       r1[d] = (e0 - g.lower[d]) * g.inverse[d];
     }
-    if (!finite) continue;
-    path_into_tile<NDIM, DEG>(r0, r1, charge, g, ax, (lds_u64 *)lds, accum, scale);
+    if constexpr (DEG == 0) {
+      if (!whitney_finite<NDIM>(r0, r1)) continue;
+      whitney_path<NDIM>(r0, r1, charge, g, wsink);
+    } else {
+      if (!finite) continue;
+      path_into_tile<NDIM, DEG>(r0, r1, charge, g, ax, (lds_u64 *)lds, accum, scale);
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < PT::WORDS; i += BLOCK) {
@@ -820,7 +879,8 @@ static void launch_paths_tiled(int degree, const void *spos, const void *srows, 
 #define TILED(D)                                                                                                   \
   hipLaunchKernelGGL((spline_paths_tiled<T, NDIM, D, 256>), grid, block, 0, stream, (const T *)spos,              \
                      (const T *)srows, n, bins, g, nty, ntz, accum, slot)
-  if (degree == 1) TILED(1);
+  if (degree == 0) TILED(0);  // Whitney
+  else if (degree == 1) TILED(1);
   else if (degree == 2) TILED(2);
   else TILED(3);
 #undef TILED
@@ -832,7 +892,7 @@ constexpr int64_t PATHS_TILED_MIN = 1 << 16;
 // Bin the paths by start cell for the tiled deposit: positions = start, values = the
 // charge and the end point (so that all of a path moves with it); returns the bins' sorted
 // start points, sorted rows (charge, end) and offsets in ctx->path_sorted.
-static int bin_paths(int degree, int ndim, const void *start, int64_t ss, const void *end, int64_t es,
+static int bin_paths(int kind, int degree, int ndim, const void *start, int64_t ss, const void *end, int64_t es,
                      const void *charges, int dtype, int64_t n, const double *lower, const double *cell,
                      const int64_t *num, const int64_t *code, mpf_context *ctx, hipStream_t stream, void **spos,
                      void **srows, uint32_t **bins) {
@@ -857,7 +917,7 @@ static int bin_paths(int degree, int ndim, const void *start, int64_t ss, const 
   int64_t *order = (int64_t *)(out + b_spos + b_rows);
   *bins = (uint32_t *)(out + b_spos + b_rows + b_order);
   double *scalars = (double *)(out + b_spos + b_rows + b_order + b_bins);
-  return mpf_bin_markers(SPLINE, degree, ndim, start, ss, in, (int64_t)nn, dtype, n, ndim + 1, lower, cell, num, code,
+  return mpf_bin_markers(kind, degree, ndim, start, ss, in, (int64_t)nn, dtype, n, ndim + 1, lower, cell, num, code,
                          *spos, *srows, order, *bins, scalars, MPF_SCALE_AUTO, ctx, (mpf_stream)stream);
 }
 
@@ -902,14 +962,15 @@ extern "C" int mpf_deposit_paths(int kind, int degree, int ndim, const void *sta
   unsigned long long *accum = nullptr;
   const double *slot = nullptr;
   // large spline-edge deposits in fixed point: binned by start cell, tiled (the same sums)
-  bool tiled = kind == 1 && mode == MODE_FIXED && (ndim == 2 || ndim == 3) && n >= PATHS_TILED_MIN;
+  bool tiled = mode == MODE_FIXED && (ndim == 2 || ndim == 3) && n >= PATHS_TILED_MIN;
   for (int d = 0; d < ndim; ++d) tiled = tiled && code[d] != 2;
   void *spos = nullptr, *srows = nullptr;
   uint32_t *bins = nullptr;
   if (tiled) {
     // the binning's own scale goes through ctx->scalars first; the path scale follows
-    rc = bin_paths(deg, ndim, start, start_stride, end, end_stride, charges, dtype, n, lower, cell_size, num_nodes,
-                   code, ctx, stream, &spos, &srows, &bins);
+    // Whitney paths by the linear cell of their start, spline-edge paths by the spline cell
+    rc = bin_paths(kind == 0 ? LINEAR : SPLINE, deg, ndim, start, start_stride, end, end_stride, charges, dtype, n,
+                   lower, cell_size, num_nodes, code, ctx, stream, &spos, &srows, &bins);
     if (rc) return rc;
   }
   if (mode == MODE_FIXED) {
@@ -937,10 +998,11 @@ extern "C" int mpf_deposit_paths(int kind, int degree, int ndim, const void *sta
     slot = scale_slot(ctx);
   }
   if (tiled) {
-    if (ndim == 2 && dtype == F32) launch_paths_tiled<float, 2>(deg, spos, srows, n, bins, g, accum, slot, stream);
-    if (ndim == 2 && dtype == F64) launch_paths_tiled<double, 2>(deg, spos, srows, n, bins, g, accum, slot, stream);
-    if (ndim == 3 && dtype == F32) launch_paths_tiled<float, 3>(deg, spos, srows, n, bins, g, accum, slot, stream);
-    if (ndim == 3 && dtype == F64) launch_paths_tiled<double, 3>(deg, spos, srows, n, bins, g, accum, slot, stream);
+    const int tdeg = kind == 0 ? 0 : deg;  // degree 0 selects the Whitney tile kernel
+    if (ndim == 2 && dtype == F32) launch_paths_tiled<float, 2>(tdeg, spos, srows, n, bins, g, accum, slot, stream);
+    if (ndim == 2 && dtype == F64) launch_paths_tiled<double, 2>(tdeg, spos, srows, n, bins, g, accum, slot, stream);
+    if (ndim == 3 && dtype == F32) launch_paths_tiled<float, 3>(tdeg, spos, srows, n, bins, g, accum, slot, stream);
+    if (ndim == 3 && dtype == F64) launch_paths_tiled<double, 3>(tdeg, spos, srows, n, bins, g, accum, slot, stream);
   } else if (dtype == F32) {
     dispatch_paths<float>(kind, ndim, mode, start, start_stride, end, end_stride, charges, n, g, deg, edges, accum,
                           slot, stream);

@@ -195,25 +195,30 @@ def many_paths(ndim, periodic, seed, n=80000):
 
 @pytest.mark.parametrize("periodic", [0, 1])
 @pytest.mark.parametrize("ndim", [2, 3])
-@pytest.mark.parametrize("degree", [1, 2, 3])
+@pytest.mark.parametrize("degree", [0, 1, 2, 3])
 def test_tiled_path_deposit_matches_oracle_and_is_reproducible(degree, ndim, periodic):
-    """80000 paths take the binned, tiled fixed-point deposit: it equals the oracle, the
-    float64-atomic deposit, and itself bit for bit under any path order."""
+    """80000 paths take the binned, tiled fixed-point deposit (degree 0 here means WHITNEY,
+    1..3 the spline edges): it equals the oracle, the float64-atomic deposit, and itself bit
+    for bit under any path order."""
     print(f"tiled paths p={degree} {ndim}D periodic={periodic}", flush=True)
     g, shape, start, end, q = many_paths(ndim, periodic, seed=30 + degree)
     want = np.zeros((*shape, ndim))
-    orc.deposit_spline(start, end, q, want, *g, degree=degree)
+    if degree == 0:
+        orc.deposit_whitney(start, end, q, want, *g)
+    else:
+        orc.deposit_spline(start, end, q, want, *g, degree=degree)
+    kind, deg = (0, 1) if degree == 0 else (1, degree)
     fixed = torch.zeros((*shape, ndim), dtype=torch.float64, device="cuda")
-    K.deposit_paths(1, degree, cuda(start), cuda(end), cuda(q), fixed, g, mode=_lib.MODE_FIXED)
+    K.deposit_paths(kind, deg, cuda(start), cuda(end), cuda(q), fixed, g, mode=_lib.MODE_FIXED)
     # the fixed-point quantum is 2^-k with n * max|q| (length + 2) * 2^k < 2^61: about 2^-36
     # here (long paths), and an edge sums up to ~10^4 rounded contributions
     assert rel_err(fixed.cpu().numpy(), want) < 2e-9
     atomic = torch.zeros_like(fixed)
-    K.deposit_paths(1, degree, cuda(start), cuda(end), cuda(q), atomic, g, mode=_lib.MODE_ATOMIC)
+    K.deposit_paths(kind, deg, cuda(start), cuda(end), cuda(q), atomic, g, mode=_lib.MODE_ATOMIC)
     assert rel_err(atomic.cpu().numpy(), want) < BAR["atomic"]
     perm = np.random.default_rng(degree).permutation(start.shape[1])
     again = torch.zeros_like(fixed)
-    K.deposit_paths(1, degree, cuda(start[:, perm]), cuda(end[:, perm]), cuda(q[perm]), again, g,
+    K.deposit_paths(kind, deg, cuda(start[:, perm]), cuda(end[:, perm]), cuda(q[perm]), again, g,
                     mode=_lib.MODE_FIXED)
     assert torch.equal(again, fixed)
 
