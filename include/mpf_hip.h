@@ -169,6 +169,29 @@ int mpf_deposit_paths_spline_1d(MPF_PATHS_IN, MPF_GEOM_1D, int degree, int mode,
 int mpf_deposit_paths_spline_2d(MPF_PATHS_IN, MPF_GEOM_2D, int degree, int mode, MPF_TAIL);
 int mpf_deposit_paths_spline_3d(MPF_PATHS_IN, MPF_GEOM_3D, int degree, int mode, MPF_TAIL);
 
+/* ---- edge gathers (edges.py:169-185), the transpose of the path deposits ---------- */
+
+#define MPF_EDGES_GATHER_IN \
+  const void *positions, int64_t pos_stride, int dtype, int64_t n, const double *edges, void *result, \
+      int64_t result_stride
+
+/* WHITNEY: kernels/gather_edges_{1,2,3}d (gather_edges_2d_pyccel.py:37-63); result (ndim, n) */
+int mpf_gather_edges_1d(MPF_EDGES_GATHER_IN, MPF_GEOM_1D, MPF_TAIL);
+int mpf_gather_edges_2d(MPF_EDGES_GATHER_IN, MPF_GEOM_2D, MPF_TAIL);
+int mpf_gather_edges_3d(MPF_EDGES_GATHER_IN, MPF_GEOM_3D, MPF_TAIL);
+/* QUADRATIC_EDGES / CUBIC_EDGES: kernels/gather_edges_spline_{1,2,3}d, degree 1..3
+ * (gather_edges_spline_2d_pyccel.py:52-82) */
+int mpf_gather_edges_spline_1d(MPF_EDGES_GATHER_IN, MPF_GEOM_1D, int degree, MPF_TAIL);
+int mpf_gather_edges_spline_2d(MPF_EDGES_GATHER_IN, MPF_GEOM_2D, int degree, MPF_TAIL);
+int mpf_gather_edges_spline_3d(MPF_EDGES_GATHER_IN, MPF_GEOM_3D, int degree, MPF_TAIL);
+
+/* Generic edge gather: kind 0 Whitney, 1 spline edges of `degree`; result (ndim, n) in
+ * the marker dtype with row stride result_stride, overwritten. */
+int mpf_gather_edges(int kind, int degree, int ndim, const void *positions, int64_t pos_stride, int dtype, int64_t n,
+                     const double *edges, void *result, int64_t result_stride, const double *lower,
+                     const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
+                     mpf_stream stream);
+
 /* Generic form: kind 0 Whitney, 1 spline edges of `degree`. */
 int mpf_deposit_paths(int kind, int degree, int ndim, const void *start, int64_t start_stride, const void *end,
                       int64_t end_stride, const void *charges, int dtype, int64_t n, double *edges,

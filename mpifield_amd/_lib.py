@@ -36,6 +36,7 @@ def _geom(ndim: int) -> list:
 _SCATTER_IN = [_P, _I64, _P, _I64, _INT, _I64, _I64, _P]
 _GATHER_IN = [_P, _I64, _INT, _I64, _P, _I64, _P, _I64]
 _PATHS_IN = [_P, _I64, _P, _I64, _P, _INT, _I64, _P]
+_EDGES_GATHER_IN = [_P, _I64, _INT, _I64, _P, _P, _I64]
 _TAIL = [_P, _P]
 
 #: every exported entry point of include/mpf_hip.h with its argument types
@@ -71,6 +72,7 @@ SIGNATURES: dict[str, list] = {
     "mpf_gather_derivative_binned": [_INT, _INT, _INT, _P, _INT, _I64, _P, _P, _P, _I64, _P, _I64, _P, _I64, _P, _P,
                                      _P, _P, _P, _P],
     "mpf_fixed_scale": [_P, _I64, _INT, _I64, _I64, _INT, _P, _P, _P],
+    "mpf_gather_edges": [_INT, _INT, _INT, _P, _I64, _INT, _I64, _P, _P, _I64, _P, _P, _P, _P, _P, _P],
 }
 for _d in (1, 2, 3):
     for _kind in ("linear", "quadratic"):
@@ -80,6 +82,8 @@ for _d in (1, 2, 3):
     SIGNATURES[f"mpf_gather_spline_{_d}d"] = _GATHER_IN + _geom(_d) + [_INT] + _TAIL
     SIGNATURES[f"mpf_deposit_paths_{_d}d"] = _PATHS_IN + _geom(_d) + [_INT] + _TAIL
     SIGNATURES[f"mpf_deposit_paths_spline_{_d}d"] = _PATHS_IN + _geom(_d) + [_INT, _INT] + _TAIL
+    SIGNATURES[f"mpf_gather_edges_{_d}d"] = _EDGES_GATHER_IN + _geom(_d) + _TAIL
+    SIGNATURES[f"mpf_gather_edges_spline_{_d}d"] = _EDGES_GATHER_IN + _geom(_d) + [_INT] + _TAIL
 
 _RESTYPES = {"mpf_last_error": ctypes.c_char_p, "mpf_context_create": _P, "mpf_context_destroy": None,
              "mpf_patch_plan_create": _P, "mpf_patch_plan_destroy": None, "mpf_patch_plan_items": _I64}

@@ -921,6 +921,115 @@ static int bin_paths(int kind, int degree, int ndim, const void *start, int64_t 
                          *spos, *srows, order, *bins, scalars, MPF_SCALE_AUTO, ctx, (mpf_stream)stream);
 }
 
+// ---- edge gathers (the transpose of the path deposits) ----------------------------------
+//
+// One marker per work-item.  DEG 0: WHITNEY (gather_edges_{1,2,3}d_pyccel.py): in LINEAR's
+// cell of the marker the edges along axis a weighted by the linear weights across, the
+// products and sums in the reference's order, over the cell size along a.  DEG 1..3: the
+// spline edges (gather_edges_spline_{1,2,3}d_pyccel.py): the position clamped to the grid
+// on a non-periodic axis; along axis a the along_stencil edges that exist times the
+// spline_axis nodes across.
+template <class T, int NDIM, int DEG>
+__global__ void __launch_bounds__(256) gather_edges_kernel(const T *__restrict__ pos, int64_t ps, int64_t n, Grid g,
+                                                           const double *__restrict__ edges, T *__restrict__ res,
+                                                           int64_t rs) {
+  const int64_t s0 = NDIM == 3 ? g.extent[1] * g.extent[2] : (NDIM == 2 ? g.extent[1] : 1);
+  const int64_t s1 = NDIM == 3 ? g.extent[2] : 1;
+  for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x) {
+    if constexpr (DEG == 0) {
+      int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+      double f[3] = {0, 0, 0};
+#pragma unroll
+      for (int d = 0; d < NDIM; ++d) {
+        int64_t nd[2];
+        double wd[2];
+        linear_axis(((double)pos[d * ps + m] - g.lower[d]) * g.inverse[d], g.num[d], g.code[d], nd, wd);
+        lo[d] = nd[0];
+        hi[d] = nd[1];
+        f[d] = wd[1];
+      }
+      if constexpr (NDIM == 1) {
+        res[m] = (T)(edges[lo[0]] * g.inverse[0]);
+      } else if constexpr (NDIM == 2) {
+#define E2(i, j, a) edges[((i) * s0 + (j)) * 2 + (a)]
+        res[m] = (T)(((1.0 - f[1]) * E2(lo[0], lo[1], 0) + f[1] * E2(lo[0], hi[1], 0)) * g.inverse[0]);
+        res[rs + m] = (T)(((1.0 - f[0]) * E2(lo[0], lo[1], 1) + f[0] * E2(hi[0], lo[1], 1)) * g.inverse[1]);
+#undef E2
+      } else {
+#define E3(i, j, k, a) edges[((i) * s0 + (j) * s1 + (k)) * 3 + (a)]
+        const double wx0 = 1.0 - f[0], wx1 = f[0], wy0 = 1.0 - f[1], wy1 = f[1], wz0 = 1.0 - f[2], wz1 = f[2];
+        res[m] = (T)((wy0 * wz0 * E3(lo[0], lo[1], lo[2], 0) + wy1 * wz0 * E3(lo[0], hi[1], lo[2], 0) +
+                      wy0 * wz1 * E3(lo[0], lo[1], hi[2], 0) + wy1 * wz1 * E3(lo[0], hi[1], hi[2], 0)) * g.inverse[0]);
+        res[rs + m] = (T)((wx0 * wz0 * E3(lo[0], lo[1], lo[2], 1) + wx1 * wz0 * E3(hi[0], lo[1], lo[2], 1) +
+                           wx0 * wz1 * E3(lo[0], lo[1], hi[2], 1) + wx1 * wz1 * E3(hi[0], lo[1], hi[2], 1)) *
+                          g.inverse[1]);
+        res[2 * rs + m] = (T)((wx0 * wy0 * E3(lo[0], lo[1], lo[2], 2) + wx1 * wy0 * E3(hi[0], lo[1], lo[2], 2) +
+                               wx0 * wy1 * E3(lo[0], hi[1], lo[2], 2) + wx1 * wy1 * E3(hi[0], hi[1], lo[2], 2)) *
+                              g.inverse[2]);
+#undef E3
+      }
+    } else {
+      constexpr int S = DEG + 1;
+      double rel[NDIM];
+#pragma unroll
+      for (int d = 0; d < NDIM; ++d) {
+        double r = ((double)pos[d * ps + m] - g.lower[d]) * g.inverse[d];
+        if (g.code[d] != 1) {
+          if (r < 0.0) r = 0.0;
+          if (r > (double)(g.num[d] - 1)) r = (double)g.num[d] - 1.0;
+        }
+        rel[d] = r;
+      }
+      int64_t node[NDIM][S], edge[NDIM][DEG];
+      double wn[NDIM][S], we[NDIM][DEG];
+      int ex[NDIM][DEG];
+#pragma unroll
+      for (int d = 0; d < NDIM; ++d) {
+        spline_axis(rel[d], g.num[d], g.code[d], DEG, node[d], wn[d]);
+        along_stencil(rel[d], g.num[d], g.code[d], DEG, edge[d], we[d], ex[d]);
+      }
+#pragma unroll
+      for (int axis = 0; axis < NDIM; ++axis) {
+        double total = 0.0;
+#pragma unroll
+        for (int a = 0; a < (axis == 0 ? DEG : S); ++a) {
+          if (axis == 0 && !ex[0][a]) continue;
+          const double va = axis == 0 ? we[0][a] : wn[0][a];
+          const int64_t ia = axis == 0 ? edge[0][a] : node[0][a];
+#pragma unroll
+          for (int b = 0; b < (NDIM < 2 ? 1 : (axis == 1 ? DEG : S)); ++b) {
+            if (NDIM >= 2 && axis == 1 && !ex[NDIM > 1 ? 1 : 0][b]) continue;
+            const double vb = NDIM < 2 ? va : va * (axis == 1 ? we[NDIM > 1 ? 1 : 0][b] : wn[NDIM > 1 ? 1 : 0][b]);
+            const int64_t ib = NDIM < 2 ? 0 : (axis == 1 ? edge[NDIM > 1 ? 1 : 0][b] : node[NDIM > 1 ? 1 : 0][b]);
+#pragma unroll
+            for (int e = 0; e < (NDIM < 3 ? 1 : (axis == 2 ? DEG : S)); ++e) {
+              if (NDIM == 3 && axis == 2 && !ex[NDIM - 1][e]) continue;
+              const double v = NDIM < 3 ? vb : vb * (axis == 2 ? we[NDIM - 1][e] : wn[NDIM - 1][e]);
+              const int64_t ie = NDIM < 3 ? 0 : (axis == 2 ? edge[NDIM - 1][e] : node[NDIM - 1][e]);
+              total += v * edges[(ia * s0 + ib * s1 + ie) * NDIM + axis];
+            }
+          }
+        }
+        res[axis * rs + m] = (T)(total * g.inverse[axis]);
+      }
+    }
+  }
+}
+
+template <class T, int NDIM>
+static void launch_gather_edges(int degree, const void *pos, int64_t ps, int64_t n, const Grid &g,
+                                const double *edges, void *res, int64_t rs, hipStream_t stream) {
+  const int threads = 256, blocks = blocks_for(n, threads, 1 << 16);
+#define GE(D)                                                                                                    \
+  hipLaunchKernelGGL((gather_edges_kernel<T, NDIM, D>), dim3(blocks), dim3(threads), 0, stream, (const T *)pos, ps, \
+                     n, g, edges, (T *)res, rs)
+  if (degree == 0) GE(0);
+  else if (degree == 1) GE(1);
+  else if (degree == 2) GE(2);
+  else GE(3);
+#undef GE
+}
+
 }  // namespace mpf
 
 using namespace mpf;
@@ -1039,3 +1148,61 @@ extern "C" int mpf_deposit_paths(int kind, int degree, int ndim, const void *sta
 PATH_ENTRY(1)
 PATH_ENTRY(2)
 PATH_ENTRY(3)
+
+/* The edge field at the markers (the transpose of the path deposits): kind 0 WHITNEY,
+ * kind 1 spline edges of `degree`; result (ndim, n) in the marker dtype, overwritten. */
+extern "C" int mpf_gather_edges(int kind, int degree, int ndim, const void *positions, int64_t pos_stride, int dtype,
+                                int64_t n, const double *edges, void *result, int64_t result_stride,
+                                const double *lower, const double *cell_size, const int64_t *num_nodes,
+                                const int64_t *code, mpf_context *ctx, mpf_stream stream_) {
+  (void)ctx;
+  if (kind != 0 && kind != 1) {
+    set_error("edge kind must be 0 (Whitney) or 1 (spline edges), got %d", kind);
+    return MPF_ERR_ARGUMENT;
+  }
+  if (kind == 1 && (degree < 1 || degree > 3)) {
+    set_error("spline edge degrees are 1 to 3, got %d", degree);
+    return MPF_ERR_ARGUMENT;
+  }
+  if (dtype != F32 && dtype != F64) {
+    set_error("dtype must be MPF_F32 or MPF_F64, got %d", dtype);
+    return MPF_ERR_ARGUMENT;
+  }
+  if (n < 0) {
+    set_error("negative marker count");
+    return MPF_ERR_ARGUMENT;
+  }
+  Grid g;
+  const int deg = kind == 0 ? 1 : degree;
+  int rc = make_grid(g, ndim, kind == 0 ? LINEAR : SPLINE, deg, lower, cell_size, num_nodes, code);
+  if (rc) return rc;
+  if (n == 0) return MPF_OK;
+  hipStream_t stream = (hipStream_t)stream_;
+  const int d = kind == 0 ? 0 : degree;
+#define GE_ND(T)                                                                                                  \
+  if (ndim == 1) launch_gather_edges<T, 1>(d, positions, pos_stride, n, g, edges, result, result_stride, stream); \
+  if (ndim == 2) launch_gather_edges<T, 2>(d, positions, pos_stride, n, g, edges, result, result_stride, stream); \
+  if (ndim == 3) launch_gather_edges<T, 3>(d, positions, pos_stride, n, g, edges, result, result_stride, stream);
+  if (dtype == F32) {
+    GE_ND(float)
+  } else {
+    GE_ND(double)
+  }
+#undef GE_ND
+  MPF_LAUNCH_CHECK();
+  return MPF_OK;
+}
+
+#define GATHER_EDGES_ENTRY(D)                                                                                       \
+  extern "C" int mpf_gather_edges_##D##d(MPF_EDGES_GATHER_IN, GEOM##D, MPF_TAIL) {                                  \
+    P##D return mpf_gather_edges(0, 1, D, positions, pos_stride, dtype, n, edges, result, result_stride, lo, cs, nn, \
+                                 pc, ctx, stream);                                                                \
+  }                                                                                                                \
+  extern "C" int mpf_gather_edges_spline_##D##d(MPF_EDGES_GATHER_IN, GEOM##D, int degree, MPF_TAIL) {               \
+    P##D return mpf_gather_edges(1, degree, D, positions, pos_stride, dtype, n, edges, result, result_stride, lo, cs, \
+                                 nn, pc, ctx, stream);                                                            \
+  }
+
+GATHER_EDGES_ENTRY(1)
+GATHER_EDGES_ENTRY(2)
+GATHER_EDGES_ENTRY(3)

@@ -6,9 +6,8 @@ axis): ``edges[..., a]`` is the line integral along the edge from a node to its
 neighbour along axis ``a``. ``deposit_paths_block`` adds the current of charges moving
 along straight paths (not wrapped on a periodic axis, clamped on a bounded one), so that
 ``divergence_block`` of it is the change of the node weighting's charge of the same degree.
-
-The edge gathers (the transpose of the deposit) are not part of this library; an
-EdgeWeighting here maps no gather kernel and says so when one is asked for.
+``gather_block`` writes the field of an edge field at markers (the transpose of the deposit:
+the current times an edge field is the work of the field along the paths).
 """
 
 from __future__ import annotations
@@ -119,7 +118,7 @@ class EdgeWeighting:
 #: The lowest-order (Whitney) edge weighting: its deposit conserves LINEAR's charge.
 WHITNEY = EdgeWeighting(
     "whitney",
-    gather={},
+    gather={1: K.gather_edges_1d, 2: K.gather_edges_2d, 3: K.gather_edges_3d},
     deposit_paths={1: K.deposit_paths_1d, 2: K.deposit_paths_2d, 3: K.deposit_paths_3d},
     halo=1,
     degree=1,
@@ -130,7 +129,8 @@ def spline_edges(name: str, degree: int) -> EdgeWeighting:
     """The edge weighting of B-spline degree ``degree`` on the spline path kernels."""
     return EdgeWeighting(
         name,
-        gather={},
+        gather={n: _WithDegree(k, degree) for n, k in
+                ((1, K.gather_edges_spline_1d), (2, K.gather_edges_spline_2d), (3, K.gather_edges_spline_3d))},
         deposit_paths={n: _WithDegree(k, degree) for n, k in
                        ((1, K.deposit_paths_spline_1d), (2, K.deposit_paths_spline_2d),
                         (3, K.deposit_paths_spline_3d))},

@@ -30,6 +30,8 @@ KERNEL_NAMES = (
     "scatter_spline_1d", "scatter_spline_2d", "scatter_spline_3d",
     "deposit_paths_1d", "deposit_paths_2d", "deposit_paths_3d",
     "deposit_paths_spline_1d", "deposit_paths_spline_2d", "deposit_paths_spline_3d",
+    "gather_edges_1d", "gather_edges_2d", "gather_edges_3d",
+    "gather_edges_spline_1d", "gather_edges_spline_2d", "gather_edges_spline_3d",
 )  # fmt: skip
 
 _state = threading.local()
@@ -243,6 +245,32 @@ def deposit_paths(kind: int, degree: int, start, end, charges, edges: torch.Tens
                   None, stream_handle(edges.device))
 
 
+def gather_edges(kind: int, degree: int, positions, edges: torch.Tensor, result: torch.Tensor, geometry) -> None:
+    """The generic edge gather (mpf_gather_edges): overwrites ``result`` (ndim, N) with the field
+    of the edge field ``edges`` at ``positions``; kind 0 WHITNEY, 1 the spline edges of ``degree``."""
+    ndim, lower, cell, num, code, _ = _split(geometry)
+    edges = _field(edges, "edges")
+    if not isinstance(result, torch.Tensor) or not result.is_cuda:
+        raise TypeError("result must be a torch tensor on the GPU")
+    pos, ps = marker_rows(positions, ndim, result.dtype if result.dtype in (torch.float32, torch.float64) else None)
+    deg = 1 if kind == 0 else degree
+    extent = _expected_extent(num, code, deg, _lib.KIND_LINEAR if kind == 0 else _lib.KIND_SPLINE)
+    if tuple(edges.shape) != (*extent, ndim):
+        raise ValueError(f"an edge field on this grid is {(*extent, ndim)}, got {tuple(edges.shape)}")
+    result2 = result.reshape(1, -1) if result.dim() == 1 else result
+    if tuple(result2.shape) != (ndim, pos.shape[1]):
+        raise ValueError(f"result must be ({ndim}, {pos.shape[1]}), got {tuple(result.shape)}")
+    direct = result2.dtype == pos.dtype and (result2.shape[1] <= 1 or result2.stride(1) == 1)
+    out = result2 if direct else torch.empty((ndim, pos.shape[1]), dtype=pos.dtype, device=edges.device)
+    rs = out.stride(0) if out.shape[0] > 1 else out.shape[1]
+    with torch.cuda.device(edges.device):
+        _lib.call("mpf_gather_edges", kind, degree, ndim, pos.data_ptr(), ps, _dtype_code(pos), pos.shape[1],
+                  edges.data_ptr(), out.data_ptr(), rs, lower.ctypes.data, cell.ctypes.data, num.ctypes.data,
+                  code.ctypes.data, None, stream_handle(edges.device))
+    if not direct:
+        result2.copy_(out)
+
+
 # ---- the catalogue ---------------------------------------------------------------------
 
 
@@ -345,7 +373,28 @@ def _path_kernel(ndim: int, spline: bool):
 deposit_paths_1d, deposit_paths_2d, deposit_paths_3d = (_path_kernel(d, False) for d in (1, 2, 3))
 deposit_paths_spline_1d, deposit_paths_spline_2d, deposit_paths_spline_3d = (_path_kernel(d, True) for d in (1, 2, 3))
 
-#: every kernel by its catalogue name (catalog.py:116-124), the edge gathers excepted
+
+def _edge_gather_kernel(ndim: int, spline: bool):
+    if spline:
+        def kernel(positions, edges, result, *geometry_degree):
+            *geometry, degree = geometry_degree
+            gather_edges(1, int(degree), positions, edges, result, geometry)
+
+        kernel.__name__ = f"gather_edges_spline_{ndim}d"
+    else:
+        def kernel(positions, edges, result, *geometry):
+            gather_edges(0, 1, positions, edges, result, geometry)
+
+        kernel.__name__ = f"gather_edges_{ndim}d"
+    kernel.__doc__ = f"kernels/{kernel.__name__}: overwrite result with the field of edges at positions."
+    return kernel
+
+
+gather_edges_1d, gather_edges_2d, gather_edges_3d = (_edge_gather_kernel(d, False) for d in (1, 2, 3))
+gather_edges_spline_1d, gather_edges_spline_2d, gather_edges_spline_3d = (_edge_gather_kernel(d, True)
+                                                                          for d in (1, 2, 3))
+
+#: every kernel by its catalogue name (catalog.py:51-88; the forms kernels are not built)
 KERNELS = {name: globals()[name] for name in KERNEL_NAMES}
 
 

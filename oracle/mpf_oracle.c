@@ -836,6 +836,104 @@ void orc_deposit_spline(int ndim, int degree, const double *start, int64_t start
 }
 
 /* ------------------------------------------------------------------------- */
+/* Edge gathers (the transpose of the path deposits)                          */
+/* ------------------------------------------------------------------------- */
+
+/* gather_edges_{1,2,3}d_pyccel.py (1D :27-35, 2D :37-63, 3D :40-86): the Whitney field of
+ * the edge line integrals at each marker, in LINEAR's cell (wrapped on a periodic axis,
+ * clamped on a bounded one): along axis a the edges of the cell along a, weighted by the
+ * linear weights edge_weight(f, 0/1) = 1 - f, f across the cell (whitney_cell_pyccel.py:
+ * 23-27), summed in the reference's order, divided by the cell size along a. */
+void orc_gather_edges_whitney(int ndim, const double *pos, int64_t ps, int64_t n, const double *edges,
+                              double *result, int64_t rs, const double *lower, const double *cell,
+                              const int64_t *num, const int64_t *code) {
+    grid_t g;
+    make_grid(&g, ndim, KIND_LINEAR, 1, lower, cell, num, code);
+    const int64_t s1 = g.extent[1] * g.extent[2], s2 = g.extent[2];
+    for (int64_t m = 0; m < n; ++m) {
+        int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        double f[3] = {0, 0, 0};
+        for (int d = 0; d < ndim; ++d) {
+            double rel = (pos[d * ps + m] - g.lower[d]) * g.inverse[d];
+            if (g.code[d] == 1) rel = periodic_position(rel, g.num[d]);
+            lo[d] = left_node(rel, g.num[d], g.code[d]);
+            hi[d] = right_node(lo[d], g.num[d], g.code[d]);
+            f[d] = cell_fraction(rel, lo[d]);
+        }
+#define E(i, j, k, a) edges[((i) * (ndim == 3 ? s1 : (ndim == 2 ? g.extent[1] : 1)) + (ndim == 3 ? (j) * s2 : (ndim == 2 ? (j) : 0)) + (ndim == 3 ? (k) : 0)) * ndim + (a)]
+        if (ndim == 1) {
+            result[m] = edges[lo[0]] * g.inverse[0];
+        } else if (ndim == 2) {
+            result[m] = ((1.0 - f[1]) * E(lo[0], lo[1], 0, 0) + f[1] * E(lo[0], hi[1], 0, 0)) * g.inverse[0];
+            result[rs + m] = ((1.0 - f[0]) * E(lo[0], lo[1], 0, 1) + f[0] * E(hi[0], lo[1], 0, 1)) * g.inverse[1];
+        } else {
+            const double wy0 = 1.0 - f[1], wy1 = f[1], wz0 = 1.0 - f[2], wz1 = f[2], wx0 = 1.0 - f[0], wx1 = f[0];
+            result[m] = (wy0 * wz0 * E(lo[0], lo[1], lo[2], 0) + wy1 * wz0 * E(lo[0], hi[1], lo[2], 0) +
+                         wy0 * wz1 * E(lo[0], lo[1], hi[2], 0) + wy1 * wz1 * E(lo[0], hi[1], hi[2], 0)) * g.inverse[0];
+            result[rs + m] = (wx0 * wz0 * E(lo[0], lo[1], lo[2], 1) + wx1 * wz0 * E(hi[0], lo[1], lo[2], 1) +
+                              wx0 * wz1 * E(lo[0], lo[1], hi[2], 1) + wx1 * wz1 * E(hi[0], lo[1], hi[2], 1)) * g.inverse[1];
+            result[2 * rs + m] = (wx0 * wy0 * E(lo[0], lo[1], lo[2], 2) + wx1 * wy0 * E(hi[0], lo[1], lo[2], 2) +
+                                  wx0 * wy1 * E(lo[0], hi[1], lo[2], 2) + wx1 * wy1 * E(hi[0], hi[1], lo[2], 2)) * g.inverse[2];
+        }
+#undef E
+    }
+}
+
+/* gather_edges_spline_{1,2,3}d_pyccel.py (2D :52-82, 3D :70-130): the field of the spline
+ * edge functions B_{p-1}(r_a - k_a - 1/2) prod_{b != a} B_p(r_b - k_b) at each marker,
+ * positions clamped to [0, num - 1] on a non-periodic axis; along axis a the along_stencil
+ * edges that exist times the spline_cell nodes across, the products left to right and the
+ * sums in the reference's loop order, divided by the cell size along a. */
+void orc_gather_edges_spline(int ndim, int degree, const double *pos, int64_t ps, int64_t n, const double *edges,
+                             double *result, int64_t rs, const double *lower, const double *cell,
+                             const int64_t *num, const int64_t *code) {
+    grid_t g;
+    make_grid(&g, ndim, KIND_SPLINE, degree, lower, cell, num, code);
+    const int64_t s0 = ndim == 3 ? g.extent[1] * g.extent[2] : (ndim == 2 ? g.extent[1] : 1);
+    const int64_t s1 = ndim == 3 ? g.extent[2] : 1;
+    for (int64_t m = 0; m < n; ++m) {
+        double rel[3] = {0, 0, 0};
+        for (int d = 0; d < ndim; ++d) {
+            double r = (pos[d * ps + m] - g.lower[d]) * g.inverse[d];
+            if (g.code[d] != 1) {
+                if (r < 0.0) r = 0.0;
+                if (r > (double)(g.num[d] - 1)) r = (double)g.num[d] - 1.0;
+            }
+            rel[d] = r;
+        }
+        for (int axis = 0; axis < ndim; ++axis) {
+            int64_t idx[3][4] = {{0}, {0}, {0}};
+            double w[3][4] = {{1.0}, {1.0}, {1.0}};
+            int ex[4] = {1, 1, 1, 1}, size[3] = {1, 1, 1};
+            for (int d = 0; d < ndim; ++d) {
+                if (d == axis) {
+                    along_stencil(rel[d], g.num[d], g.code[d], degree, idx[d], w[d], ex);
+                    size[d] = degree;
+                } else {
+                    spline_cell(rel[d], g.num[d], g.code[d], degree, idx[d], w[d]);
+                    size[d] = degree + 1;
+                }
+            }
+            double total = 0.0;
+            for (int a = 0; a < size[0]; ++a) {
+                if (axis == 0 && !ex[a]) continue;
+                for (int b = 0; b < size[1]; ++b) {
+                    if (axis == 1 && !ex[b]) continue;
+                    for (int e = 0; e < size[2]; ++e) {
+                        if (axis == 2 && !ex[e]) continue;
+                        double v = w[0][a];
+                        if (ndim >= 2) v = v * w[1][b];
+                        if (ndim == 3) v = v * w[2][e];
+                        total += v * edges[(idx[0][a] * s0 + idx[1][b] * s1 + idx[2][e]) * ndim + axis];
+                    }
+                }
+            }
+            result[axis * rs + m] = total * g.inverse[axis];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
 /* Halo fill and fold, gradient and divergence                                */
 /* ------------------------------------------------------------------------- */
 

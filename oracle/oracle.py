@@ -43,6 +43,8 @@ def _load() -> ctypes.CDLL:
     lib.orc_gather.argtypes = [_INT, _INT, _INT, _P, _I64, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _INT]
     lib.orc_deposit_whitney.argtypes = [_INT, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _INT]
     lib.orc_deposit_spline.argtypes = [_INT, _INT, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P]
+    lib.orc_gather_edges_whitney.argtypes = [_INT, _P, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P]
+    lib.orc_gather_edges_spline.argtypes = [_INT, _INT, _P, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P]
     for name in ("orc_halo_fill", "orc_halo_fold"):
         getattr(lib, name).argtypes = [_INT, _P, _I64, _I64, _P, _P]
     lib.orc_divergence.argtypes = [_INT, _P, _P, _P, _P]
@@ -136,6 +138,24 @@ def deposit_spline(start, end, charges, edges: np.ndarray, *geometry, degree: in
     lower, cell, num, code = _geometry(geometry, ndim)
     lib().orc_deposit_spline(ndim, degree, _ptr(s), ss, _ptr(e), es, _ptr(q), q.size, _ptr(edges), _ptr(lower),
                              _ptr(cell), _ptr(num), _ptr(code))
+
+
+def gather_edges(positions, edges: np.ndarray, result: np.ndarray, *geometry, degree: int = 0) -> None:
+    """Overwrite ``result`` ``(ndim, N)`` with the field of the edge field ``edges`` at
+    ``positions``: Whitney for degree 0 (gather_edges_*d), spline edges for degree 1..3
+    (gather_edges_spline_*d)."""
+    ndim = len(geometry) // 4
+    pos, ps = _rows(positions)
+    edges = np.ascontiguousarray(edges, dtype=np.float64)
+    assert result.dtype == np.float64 and result.strides[1] == 8 and result.shape == (ndim, pos.shape[1])
+    rs = result.strides[0] // 8 if ndim > 1 else result.shape[1]
+    lower, cell, num, code = _geometry(geometry, ndim)
+    if degree == 0:
+        lib().orc_gather_edges_whitney(ndim, _ptr(pos), ps, pos.shape[1], _ptr(edges), _ptr(result), rs, _ptr(lower),
+                                       _ptr(cell), _ptr(num), _ptr(code))
+    else:
+        lib().orc_gather_edges_spline(ndim, degree, _ptr(pos), ps, pos.shape[1], _ptr(edges), _ptr(result), rs,
+                                      _ptr(lower), _ptr(cell), _ptr(num), _ptr(code))
 
 
 def _halo(name: str, data: np.ndarray, halo: int, periodic) -> None:

@@ -173,5 +173,12 @@ def test_edge_weighting_continuity_through_the_api():
         np.testing.assert_allclose(div, (after - before)[..., 0], atol=1e-11)
     with pytest.raises(ValueError, match="an edge field on this grid is"):
         m.WHITNEY.deposit_paths_block(start, end, q, np.zeros((9, 8, 3)), grid)
+    # the edge gather through the API, host arrays in and out
+    field = np.zeros((2, 300))
+    m.WHITNEY.gather_block(start, current, field, grid)
+    assert np.isfinite(field).all() and np.abs(field).max() > 0
+    with pytest.raises(ValueError, match="an edge field on this grid is"):
+        m.WHITNEY.gather_block(start, np.zeros((9, 8, 3)), field, grid)
+    no_gather = m.EdgeWeighting("paths only", gather={}, deposit_paths=dict(m.WHITNEY.deposit_paths))
     with pytest.raises(ValueError, match="has no 2D gather kernel"):
-        m.WHITNEY.gather_block(start, np.zeros((9, 8, 2)), np.zeros((2, 300)), grid)
+        no_gather.gather_block(start, np.zeros((9, 8, 2)), field, grid)

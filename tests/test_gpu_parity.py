@@ -242,6 +242,64 @@ def test_path_deposit_continuity_on_the_device(degree, ndim, periodic):
     np.testing.assert_allclose(div.cpu().numpy(), (rho1 - rho0)[..., 0].cpu().numpy(), atol=1e-11)
 
 
+# ---- edge gathers ----------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("code", [0, 1, 2])
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+@pytest.mark.parametrize("degree", [0, 1, 2, 3])
+def test_edge_gathers_match_oracle(degree, ndim, code):
+    """degree 0 here means WHITNEY (gather_edges_*d); 1..3 the spline edges; markers partly
+    outside the grid; float64 markers at 1e-12, float32 ones at float32 rounding."""
+    if code == 2 and degree == 0:
+        pytest.skip("WHITNEY does not take clamped axes")
+    print(f"edge gather p={degree} {ndim}D code={code}", flush=True)
+    g, shape, start, _, _ = paths(ndim, code, seed=20 + degree, n=3000)
+    deg = max(degree, 1)
+    extent = tuple(n + (deg - 1 if code == 2 else 0) for n in shape)
+    edges = np.random.default_rng(degree).normal(size=(*extent, ndim))
+    want = np.zeros((ndim, start.shape[1]))
+    orc.gather_edges(start, edges, want, *g, degree=degree)
+    kind = 0 if degree == 0 else 1
+    got = torch.empty((ndim, start.shape[1]), dtype=torch.float64, device="cuda")
+    K.gather_edges(kind, deg, cuda(start), cuda(edges), got, g)
+    assert rel_err(got.cpu().numpy(), want) < 1e-12
+    p32 = start.astype(np.float32)
+    want32 = np.zeros_like(want)
+    orc.gather_edges(p32.astype(np.float64), edges, want32, *g, degree=degree)
+    got32 = torch.empty((ndim, start.shape[1]), dtype=torch.float32, device="cuda")
+    K.gather_edges(kind, deg, cuda(p32), cuda(edges), got32, g)
+    np.testing.assert_allclose(got32.cpu().numpy(), want32, rtol=1e-6, atol=1e-6 * np.abs(want32).max())
+
+
+def test_edge_weightings_gather_through_the_api():
+    """EdgeWeighting.gather_block (edges.py:169-185) on device tensors, and the current times
+    an edge field is the work of the field along the paths (test_spline_edges.py:88-115)."""
+    import mpifield_amd as m
+
+    for weighting, degree in ((m.WHITNEY, 0), (m.QUADRATIC_EDGES, 2), (m.CUBIC_EDGES, 3)):
+        g, shape, start, end, q = paths(2, 1, seed=40 + degree, n=2000)
+        ok = np.isfinite(end).all(axis=0)
+        start, end, q = start[:, ok], end[:, ok], q[ok]
+        grid = m.BlockGrid.from_geometry(g[0:2], g[2:4], g[4:6], (True, True))
+        edges = np.random.default_rng(degree).normal(size=(*shape, 2))
+        field = torch.empty((2, start.shape[1]), dtype=torch.float64, device="cuda")
+        weighting.gather_block(cuda(start), cuda(edges), field, grid)
+        want = np.zeros((2, start.shape[1]))
+        orc.gather_edges(start, edges, want, *g, degree=degree)
+        assert rel_err(field.cpu().numpy(), want) < 1e-12
+        current = torch.zeros((*shape, 2), dtype=torch.float64, device="cuda")
+        with K.deposit_mode("atomic"):
+            weighting.deposit_paths_block(cuda(start), cuda(end), cuda(q), current, grid)
+        oracle_current = np.zeros((*shape, 2))
+        if degree == 0:
+            orc.deposit_whitney(start, end, q, oracle_current, *g)
+        else:
+            orc.deposit_spline(start, end, q, oracle_current, *g, degree=degree)
+        work = float((current * cuda(edges)).sum())
+        assert work == pytest.approx(float((oracle_current * edges).sum()), rel=1e-11)
+
+
 # ---- halos, gradient, divergence --------------------------------------------------------
 
 

@@ -490,3 +490,104 @@ def test_gradient_and_divergence_are_transposes():
         # edges.py:218-237: periodic roll, bounded drop
         np.testing.assert_allclose(grad[..., 1], (np.roll(phi, -1, axis=1) - phi) if periodic[1] else
                                    np.concatenate([np.diff(phi, axis=1), np.zeros((5, 1, 3))], axis=1))
+
+
+# ----------------------------------------------------------------------------------------
+# Edge gathers
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+@pytest.mark.parametrize("periodic", [0, 1])
+def test_whitney_field_of_a_gradient_is_the_gradient_of_linear(ndim, periodic):
+    """test_edges.py:124-150: inside a cell, away from its faces, the Whitney field of the
+    edge differences of a node potential is the derivative of the linear interpolant (a
+    central difference of it is exact there)."""
+    rng = np.random.default_rng(9)
+    shape = (9, 8, 7)[:ndim]
+    lower, h = np.array([0.0, -1.0, 2.0])[:ndim], np.array([0.25, 0.5, 0.1])[:ndim]
+    g = geom(lower, h, shape, [periodic] * ndim)
+    potential = rng.normal(size=shape)
+    edges = orc.gradient(potential, [periodic] * ndim)
+    cells = np.array([rng.integers(0, n - 1, 50) for n in shape])
+    points = lower[:, None] + (cells + rng.uniform(0.2, 0.8, cells.shape)) * h[:, None]
+    field = np.zeros_like(points)
+    orc.gather_edges(points, edges, field, *g, degree=0)
+    step = 0.05
+    for axis in range(ndim):
+        shift = np.zeros((ndim, 1))
+        shift[axis] = step * h[axis]
+        values = []
+        for sign in (1.0, -1.0):
+            value = np.zeros((1, points.shape[1]))
+            orc.gather("linear", points + sign * shift, potential[..., None], value, *g)
+            values.append(value[0])
+        np.testing.assert_allclose(field[axis], (values[0] - values[1]) / (2.0 * shift[axis]), rtol=1e-10,
+                                   atol=1e-10)
+
+
+def _work_along(start, end, q, edges, g, degree, periodic):
+    """sum q * integral of E . dl along the paths (clamped to a bounded grid), exact: each
+    path cut where the field's pieces change (the knot planes, integers for WHITNEY and odd
+    degrees, half-integers for even ones, and the walls), 5-point Gauss-Legendre on every
+    piece (E . dl is a polynomial of degree at most 8 in t there)."""
+    ndim = start.shape[0]
+    lower = np.asarray(g[:ndim])
+    h = np.asarray(g[ndim:2 * ndim])
+    num = np.asarray(g[2 * ndim:3 * ndim])
+    shift = 0.5 if degree % 2 == 0 and degree > 0 else 0.0
+    nodes, weights = np.polynomial.legendre.leggauss(5)
+
+    def clamp(points):
+        return points if periodic else np.clip(points, lower[:, None], (lower + h * (num - 1))[:, None])
+
+    work = 0.0
+    for p in range(q.size):
+        r0, r1 = (start[:, p] - lower) / h, (end[:, p] - lower) / h
+        cuts = [0.0, 1.0]
+        for d in range(ndim):
+            if r1[d] == r0[d]:
+                continue
+            lo, hi = min(r0[d], r1[d]), max(r0[d], r1[d])
+            planes = np.arange(np.floor(lo - shift) + shift, hi + 1.0)
+            if not periodic:
+                planes = np.concatenate([planes, [0.0, num[d] - 1.0]])
+            t = (planes - r0[d]) / (r1[d] - r0[d])
+            cuts.extend(t[(t > 0.0) & (t < 1.0)])
+        cuts = np.unique(cuts)
+        for t0, t1 in zip(cuts[:-1], cuts[1:]):
+            a = clamp(start[:, p:p + 1] + t0 * (end - start)[:, p:p + 1])
+            b = clamp(start[:, p:p + 1] + t1 * (end - start)[:, p:p + 1])
+            points = np.ascontiguousarray(a + 0.5 * (nodes + 1.0) * (b - a))
+            field = np.zeros_like(points)
+            orc.gather_edges(points, edges, field, *g, degree=degree)
+            work += q[p] * 0.5 * np.sum(weights * np.sum(field * (b - a), axis=0))
+    return work
+
+
+@pytest.mark.parametrize("degree", [0, 1, 2, 3])
+@pytest.mark.parametrize("ndim,periodic", [(1, 0), (1, 1), (2, 1), (2, 0), (3, 0), (3, 1)])
+def test_the_current_is_the_transpose_of_the_edge_gather(degree, ndim, periodic):
+    """test_spline_edges.py:88-115 and test_edges.py:153-172 (degree 0 here is WHITNEY):
+    j . e = sum q * integral of E . dl along the clamped paths, integrated exactly piece by
+    piece; paths partly outside the grid and some several cells long."""
+    g, shape, start, end, q = _paths(ndim, periodic, seed=5 + degree, n=12, reach=3.0)
+    edges = np.random.default_rng(degree).normal(size=(*shape, ndim))
+    current = np.zeros_like(edges)
+    if degree == 0:
+        orc.deposit_whitney(start, end, q, current, *g)
+    else:
+        orc.deposit_spline(start, end, q, current, *g, degree=degree)
+    work = _work_along(start, end, q, edges, g, degree, periodic)
+    assert float(np.sum(current * edges)) == pytest.approx(work, rel=1e-11, abs=1e-12)
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+@pytest.mark.parametrize("periodic", [0, 1])
+def test_spline_edge_gather_of_degree_one_is_whitney(ndim, periodic):
+    """The degree-1 spline edge functions are the Whitney ones (test_spline_edges.py:46-64)."""
+    g, shape, start, _, _ = _paths(ndim, periodic, seed=3, n=500)
+    edges = np.random.default_rng(4).normal(size=(*shape, ndim))
+    whitney, spline = np.zeros_like(start), np.zeros_like(start)
+    orc.gather_edges(start, edges, whitney, *g, degree=0)
+    orc.gather_edges(start, edges, spline, *g, degree=1)
+    np.testing.assert_allclose(spline, whitney, rtol=1e-13, atol=1e-13)
