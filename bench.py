@@ -51,11 +51,11 @@ def parse() -> argparse.Namespace:
     p.add_argument("--only-headline", action="store_true", help="time the headline step only (for rocprofv3)")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only copy MIB MiB device-to-device (the counter calibration for rocprofv3 --pmc)")
-    p.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6, 7),
+    p.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6, 7, 8),
                    help="2: the headline (default); 3: 2D quadratic-edge path deposit, 1024^2, 2^25 paths; "
                    "4: 4096 batched 2D patches, ~2e8 markers; 5: differentiable 3D cubic gather of scatter, "
                    "128^3, 2^24 markers; 6: 2D Whitney path deposit, same paths as 3; 7: halo fold + fill, "
-                   "256^3 x 3, halo 2 (one JSON line each)")
+                   "256^3 x 3, halo 2; 8: 2D quadratic-edge field at 2^25 markers (one JSON line each)")
     return p.parse_args()
 
 
@@ -274,6 +274,43 @@ def config6(torch, args) -> dict:
     }
 
 
+def config8(torch, args) -> dict:
+    """The QUADRATIC_EDGES field at markers (the transpose of config 3): 2^25 float32 markers
+    (the paths' start points) on the 1024^2 periodic grid, a random edge field."""
+    from mpifield_amd import kernels as K
+
+    nodes, n = 1024, 1 << 25
+    g, start, _, _ = paths_2d(torch, n, nodes)
+    gen = torch.Generator(device="cuda").manual_seed(8)
+    edges = torch.randn((nodes, nodes, 2), generator=gen, device="cuda", dtype=torch.float64)
+    result = torch.empty((2, n), dtype=torch.float32, device="cuda")
+    ms = time_events(torch, lambda: K.gather_edges(1, 2, start, edges, result, g), args.steps, 2)
+    alg = n * (8 + 8) + nodes * nodes * 2 * 8
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        sample = 1 << 22
+        p_h = start[:, :sample].double().cpu().numpy()
+        e_h = edges.cpu().numpy()
+        r_h = np.zeros((2, sample))
+        sec = cpu_seconds(lambda: orc.gather_edges(p_h, e_h, r_h, *g, degree=2))
+        got = result[:, :sample].double().cpu().numpy()
+        err = float(np.abs(got - r_h).max() / np.abs(r_h).max())
+        cpu = {"value": sample / sec, "unit": "markers/s", "cores": 1, "kind": "port",
+               "sample": f"first {sample} markers: {sec:.2f} s; max relative difference of the GPU result {err:.1e}"}
+    return {
+        "metric": "markers/s, 2D QUADRATIC_EDGES edge gather 1024^2, 2^25 markers", "value": n / (ms * 1e-3),
+        "unit": "markers/s", "n_gpus": 1, "ms_per_step": ms, "higher_is_better": True, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2D spline-edge (degree 2) field at {n} float32 markers, 1024^2 periodic, float64 edges; "
+                   "one marker per work-item"},
+        "roofline": {"bound": "hbm", "achieved": alg / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+
+
 def config7(torch, args) -> dict:
     """Halo fold then fill of one block's storage: 256^3 interior nodes, halo 2 on every side,
     3 components, periodic (the exchange a single block does locally)."""
@@ -451,7 +488,7 @@ def main() -> None:
         return
 
     if args.config != 2:
-        line = {3: config3, 4: config4, 5: config5, 6: config6, 7: config7}[args.config](torch, args)
+        line = {3: config3, 4: config4, 5: config5, 6: config6, 7: config7, 8: config8}[args.config](torch, args)
         print(json.dumps(line), flush=True)
         return
 
