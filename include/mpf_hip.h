@@ -198,6 +198,33 @@ int mpf_deposit_paths(int kind, int degree, int ndim, const void *start, int64_t
                       const double *lower, const double *cell_size, const int64_t *num_nodes, const int64_t *code,
                       int mode, int scale_exp, mpf_context *ctx, mpf_stream stream);
 
+/* ---- forms of the spline sequence (splines.py:180-201 faces, cells) ---------------- */
+
+/* kernels/scatter_forms_spline_{1,2,3}d and gather_forms_spline_{1,2,3}d
+ * (scatter_forms_spline_3d_pyccel.py:124-210, gather_forms_spline_2d_pyccel.py:93-150):
+ * the `form`-form (1 .. ndim) of spline degree 1..3; field num_nodes + (ncomp,) with
+ * ncomp = ndim choose form (num + degree - 1 on a clamped axis), values / result
+ * (ncomp, n).  Component c is staggered (B_{p-1} shifted half a cell) along its edge
+ * (form 1), across its normal (form 2 in 3D) or along every axis (form == ndim), B_p
+ * elsewhere; no measure factors, so the two are transposes.  A marker whose values are all
+ * zero deposits nothing. */
+int mpf_scatter_forms_spline_1d(MPF_SCATTER_IN, MPF_GEOM_1D, int degree, int form, int mode, MPF_TAIL);
+int mpf_scatter_forms_spline_2d(MPF_SCATTER_IN, MPF_GEOM_2D, int degree, int form, int mode, MPF_TAIL);
+int mpf_scatter_forms_spline_3d(MPF_SCATTER_IN, MPF_GEOM_3D, int degree, int form, int mode, MPF_TAIL);
+int mpf_gather_forms_spline_1d(MPF_GATHER_IN, MPF_GEOM_1D, int degree, int form, MPF_TAIL);
+int mpf_gather_forms_spline_2d(MPF_GATHER_IN, MPF_GEOM_2D, int degree, int form, MPF_TAIL);
+int mpf_gather_forms_spline_3d(MPF_GATHER_IN, MPF_GEOM_3D, int degree, int form, MPF_TAIL);
+
+/* Generic forms (the arrays of mpf_scatter / mpf_gather). */
+int mpf_scatter_forms(int degree, int form, int ndim, const void *positions, int64_t pos_stride, const void *values,
+                      int64_t val_stride, int dtype, int64_t n, int64_t ncomp, double *field, const double *lower,
+                      const double *cell_size, const int64_t *num_nodes, const int64_t *code, int mode,
+                      int scale_exp, mpf_context *ctx, mpf_stream stream);
+int mpf_gather_forms(int degree, int form, int ndim, const void *positions, int64_t pos_stride, int dtype, int64_t n,
+                     const double *field, int64_t ncomp, void *result, int64_t result_stride, const double *lower,
+                     const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
+                     mpf_stream stream);
+
 /* ---- halos, gradient, divergence -------------------------------------------------- */
 
 /* Halo fill (Field.fill_halos -> mpiarray update_halos, field.py:601-613) and fold

@@ -73,6 +73,9 @@ SIGNATURES: dict[str, list] = {
                                      _P, _P, _P, _P],
     "mpf_fixed_scale": [_P, _I64, _INT, _I64, _I64, _INT, _P, _P, _P],
     "mpf_gather_edges": [_INT, _INT, _INT, _P, _I64, _INT, _I64, _P, _P, _I64, _P, _P, _P, _P, _P, _P],
+    "mpf_scatter_forms": [_INT, _INT, _INT, _P, _I64, _P, _I64, _INT, _I64, _I64, _P, _P, _P, _P, _P, _INT, _INT, _P,
+                          _P],
+    "mpf_gather_forms": [_INT, _INT, _INT, _P, _I64, _INT, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P],
 }
 for _d in (1, 2, 3):
     for _kind in ("linear", "quadratic"):
@@ -84,6 +87,8 @@ for _d in (1, 2, 3):
     SIGNATURES[f"mpf_deposit_paths_spline_{_d}d"] = _PATHS_IN + _geom(_d) + [_INT, _INT] + _TAIL
     SIGNATURES[f"mpf_gather_edges_{_d}d"] = _EDGES_GATHER_IN + _geom(_d) + _TAIL
     SIGNATURES[f"mpf_gather_edges_spline_{_d}d"] = _EDGES_GATHER_IN + _geom(_d) + [_INT] + _TAIL
+    SIGNATURES[f"mpf_scatter_forms_spline_{_d}d"] = _SCATTER_IN + _geom(_d) + [_INT, _INT, _INT] + _TAIL
+    SIGNATURES[f"mpf_gather_forms_spline_{_d}d"] = _GATHER_IN + _geom(_d) + [_INT, _INT] + _TAIL
 
 _RESTYPES = {"mpf_last_error": ctypes.c_char_p, "mpf_context_create": _P, "mpf_context_destroy": None,
              "mpf_patch_plan_create": _P, "mpf_patch_plan_destroy": None, "mpf_patch_plan_items": _I64}

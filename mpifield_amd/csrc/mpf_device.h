@@ -161,6 +161,59 @@ __device__ inline int spline_axis(double rel, int64_t num, int code, int degree,
   return degree + 1;
 }
 
+__device__ __forceinline__ int64_t floor_int(double x) {
+  int64_t whole = (int64_t)x;
+  if (x < (double)whole) whole -= 1;
+  return whole;
+}
+
+// spline_edges_pyccel.py:86-138: the `degree` edges along an axis around rel, weights
+// B_{p-1}(rel - k - 1/2); on a bounded axis an edge past a wall does not exist.
+__device__ __forceinline__ void along_stencil(double rel, int64_t num, int code, int degree, int64_t *edge, double *w,
+                                     int *exists) {
+  if (code == 2) {
+    clamped_cell(rel, num, degree, 1, edge, w);
+    for (int a = 0; a < degree; ++a) exists[a] = 1;
+    return;
+  }
+  const int lower = degree - 1;
+  double r = rel;
+  if (code == 1 && !(r >= 0.0 && r < (double)num)) r = r - (double)num * floor(r / (double)num);
+  const double shifted = r - 0.5;
+  int64_t middle;
+  if (lower % 2 == 0) {
+    middle = floor_int(shifted + 0.5);
+    if (code == 0 && middle > num - 2) middle = num - 2;
+  } else {
+    middle = floor_int(shifted);
+  }
+  const double o = shifted - (double)middle;
+  if (lower == 0) {
+    w[0] = 1.0;
+  } else if (lower == 1) {
+    w[0] = 1.0 - o;
+    w[1] = o;
+  } else {
+    w[0] = 0.5 * (0.5 - o) * (0.5 - o);
+    w[1] = 0.75 - o * o;
+    w[2] = 0.5 * (0.5 + o) * (0.5 + o);
+  }
+  const int64_t first = middle - lower / 2;
+  for (int a = 0; a < degree; ++a) {
+    const int64_t k = first + a;
+    if (code == 1) {
+      edge[a] = (k + 2 * num) % num;
+      exists[a] = 1;
+    } else if (k < 0 || k > num - 2) {
+      edge[a] = 0;
+      exists[a] = 0;
+    } else {
+      edge[a] = k;
+      exists[a] = 1;
+    }
+  }
+}
+
 template <int KIND>
 __device__ __forceinline__ int axis_stencil(double rel, int64_t num, int code, int degree, int64_t *node, double *w) {
   if (KIND == LINEAR) return linear_axis(rel, num, code, node, w);

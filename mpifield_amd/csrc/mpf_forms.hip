@@ -1,0 +1,367 @@
+// mpf_forms.hip -- the k-forms of the spline sequence (faces, cells), one work-item per
+// marker: scatter_forms_spline_{1,2,3}d_pyccel.py and gather_forms_spline_{1,2,3}d_pyccel.py.
+//
+// Component c of a k-form is the tensor product of B_p along the axes it is not staggered
+// on (spline_axis) and B_{p-1} shifted by half a cell along those it is (along_stencil):
+// edges along their axis, faces (k = 2 in 3D) across their normal, cells along every axis
+// (spline_edges_pyccel.py:233-245 staggered).  The values are integrals over the k-cells;
+// nothing is divided by or multiplied with the measures, so the scatter and the gather
+// are each other's transpose.  The position is clamped to [0, num - 1] on a non-periodic
+// axis first; a staggered k-cell past a bounded wall does not exist and is skipped (never
+// multiplied by zero, so a non-finite field value there stays unread).  The weight
+// products run left to right (x, y, z, then the value or the field) and the gather sums in
+// the reference's x-outer, z-inner order, so a gathered value is the oracle's to the bit.
+#include "mpf_internal.h"
+
+namespace mpf {
+
+template <int NDIM, int FORM>
+__host__ __device__ constexpr int form_components() {
+  return FORM == NDIM ? 1 : NDIM;
+}
+
+template <int NDIM, int FORM>
+__host__ __device__ constexpr bool form_staggered(int c, int d) {
+  return d >= NDIM ? false : (FORM == NDIM ? true : (FORM == 1 ? d == c : (FORM == 2 && NDIM == 3 ? d != c : false)));
+}
+
+// The per-axis stencils of one marker: the spline nodes (DEG + 1, all present) where some
+// component is not staggered, the along edges (DEG, some absent) where some component is.
+// Axes past NDIM hold one node of weight 1 at index 0 (a multiply by 1.0 is exact).
+template <int NDIM, int FORM, int DEG>
+struct FormStencil {
+  int64_t node[3][DEG + 1], edge[3][DEG];
+  double wn[3][DEG + 1], we[3][DEG];
+  int ex[3][DEG];
+
+  __device__ __forceinline__ void make(const double *rel, const Grid &g) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      bool need_node = d >= NDIM, need_edge = false;
+#pragma unroll
+      for (int c = 0; c < form_components<NDIM, FORM>(); ++c) {
+        if (form_staggered<NDIM, FORM>(c, d)) need_edge = true;
+        else if (d < NDIM) need_node = true;
+      }
+      if (d >= NDIM) {
+        node[d][0] = 0;
+        wn[d][0] = 1.0;
+        continue;
+      }
+      if (need_node) spline_axis(rel[d], g.num[d], g.code[d], DEG, node[d], wn[d]);
+      if (need_edge) along_stencil(rel[d], g.num[d], g.code[d], DEG, edge[d], we[d], ex[d]);
+    }
+  }
+};
+
+template <int NDIM, int FORM, int DEG, int C, int D>
+struct FormAxis {
+  static constexpr bool stag = form_staggered<NDIM, FORM>(C, D);
+  static constexpr int size = D >= NDIM ? 1 : (stag ? DEG : DEG + 1);
+  static __device__ __forceinline__ bool present(const FormStencil<NDIM, FORM, DEG> &s, int a) {
+    if constexpr (stag) return s.ex[D][a] != 0;
+    else return true;
+  }
+  static __device__ __forceinline__ double weight(const FormStencil<NDIM, FORM, DEG> &s, int a) {
+    if constexpr (stag) return s.we[D][a];
+    else return s.wn[D][a];
+  }
+  static __device__ __forceinline__ int64_t index(const FormStencil<NDIM, FORM, DEG> &s, int a) {
+    if constexpr (stag) return s.edge[D][a];
+    else return s.node[D][a];
+  }
+};
+
+// Calls f(offset, weight) for every present k-cell of component C: the weights multiplied
+// x, y, z left to right (the trailing multiplies by 1.0 of absent axes are exact).
+template <int NDIM, int FORM, int DEG, int C, class F>
+__device__ __forceinline__ void for_each_cell(const FormStencil<NDIM, FORM, DEG> &s, int64_t s0, int64_t s1, F &&f) {
+  using X = FormAxis<NDIM, FORM, DEG, C, 0>;
+  using Y = FormAxis<NDIM, FORM, DEG, C, 1>;
+  using Z = FormAxis<NDIM, FORM, DEG, C, 2>;
+#pragma unroll
+  for (int a = 0; a < X::size; ++a) {
+    if (!X::present(s, a)) continue;
+    const double wa = X::weight(s, a);
+    const int64_t ia = X::index(s, a) * s0;
+#pragma unroll
+    for (int b = 0; b < Y::size; ++b) {
+      if (!Y::present(s, b)) continue;
+      const double wb = NDIM >= 2 ? wa * Y::weight(s, b) : wa;
+      const int64_t ib = ia + Y::index(s, b) * s1;
+#pragma unroll
+      for (int e = 0; e < Z::size; ++e) {
+        if (!Z::present(s, e)) continue;
+        const double w = NDIM == 3 ? wb * Z::weight(s, e) : wb;
+        f(ib + Z::index(s, e), w);
+      }
+    }
+  }
+}
+
+template <int NDIM>
+__device__ __forceinline__ void form_relative(const double *x, const Grid &g, double *rel) {
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    double r = (x[d] - g.lower[d]) * g.inverse[d];
+    if (g.code[d] != 1) {
+      if (r < 0.0) r = 0.0;
+      if (r > (double)(g.num[d] - 1)) r = (double)g.num[d] - 1.0;
+    }
+    rel[d] = r;
+  }
+}
+
+template <int NDIM>
+__device__ __forceinline__ void form_strides(const Grid &g, int64_t &s0, int64_t &s1) {
+  s0 = NDIM == 3 ? g.extent[1] * g.extent[2] : (NDIM == 2 ? g.extent[1] : 1);
+  s1 = NDIM == 3 ? g.extent[2] : 1;
+}
+
+template <class T, int NDIM, int FORM, int DEG, int C>
+__device__ __forceinline__ void gather_component(const FormStencil<NDIM, FORM, DEG> &s, int64_t s0, int64_t s1,
+                                                 const double *__restrict__ field, T *__restrict__ res, int64_t rs,
+                                                 int64_t m) {
+  constexpr int NC = form_components<NDIM, FORM>();
+  if constexpr (C < NC) {
+    double total = 0.0;
+    for_each_cell<NDIM, FORM, DEG, C>(s, s0, s1, [&](int64_t at, double w) { total += w * field[at * NC + C]; });
+    res[C * rs + m] = (T)total;
+  }
+}
+
+template <class T, int NDIM, int FORM, int DEG>
+__global__ void __launch_bounds__(256) gather_forms_kernel(const T *__restrict__ pos, int64_t ps, int64_t n, Grid g,
+                                                           const double *__restrict__ field, T *__restrict__ res,
+                                                           int64_t rs) {
+  int64_t s0, s1;
+  form_strides<NDIM>(g, s0, s1);
+  for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x) {
+    double x[NDIM], rel[NDIM];
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) x[d] = (double)pos[d * ps + m];
+    form_relative<NDIM>(x, g, rel);
+    FormStencil<NDIM, FORM, DEG> s;
+    s.make(rel, g);
+    gather_component<T, NDIM, FORM, DEG, 0>(s, s0, s1, field, res, rs, m);
+    gather_component<T, NDIM, FORM, DEG, 1>(s, s0, s1, field, res, rs, m);
+    gather_component<T, NDIM, FORM, DEG, 2>(s, s0, s1, field, res, rs, m);
+  }
+}
+
+template <int MODE, int NDIM, int FORM, int DEG, int C>
+__device__ __forceinline__ void scatter_component(const FormStencil<NDIM, FORM, DEG> &s, int64_t s0, int64_t s1,
+                                                  double value, double scale, double *__restrict__ field,
+                                                  unsigned long long *__restrict__ accum) {
+  constexpr int NC = form_components<NDIM, FORM>();
+  if constexpr (C < NC) {
+    for_each_cell<NDIM, FORM, DEG, C>(s, s0, s1, [&](int64_t at, double w) {
+      const double contribution = w * value;
+      if (MODE == MODE_ATOMIC)
+        unsafeAtomicAdd(field + at * NC + C, contribution);
+      else
+        atomicAdd(accum + at * NC + C, to_fixed(contribution * scale));
+    });
+  }
+}
+
+template <class T, int NDIM, int FORM, int DEG, int MODE>
+__global__ void __launch_bounds__(256) scatter_forms_kernel(const T *__restrict__ pos, int64_t ps,
+                                                            const T *__restrict__ val, int64_t vs, int64_t n, Grid g,
+                                                            double *__restrict__ field,
+                                                            unsigned long long *__restrict__ accum,
+                                                            const double *__restrict__ scale_slot) {
+  constexpr int NC = form_components<NDIM, FORM>();
+  const double scale = MODE == MODE_FIXED ? scale_slot[0] : 1.0;
+  int64_t s0, s1;
+  form_strides<NDIM>(g, s0, s1);
+  for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x) {
+    double v[NC];
+    bool live = false;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      v[c] = (double)val[c * vs + m];
+      live |= v[c] != 0.0;
+    }
+    if (!live) continue;  // a dead marker: its position is not read
+    double x[NDIM], rel[NDIM];
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) x[d] = (double)pos[d * ps + m];
+    form_relative<NDIM>(x, g, rel);
+    FormStencil<NDIM, FORM, DEG> s;
+    s.make(rel, g);
+    scatter_component<MODE, NDIM, FORM, DEG, 0>(s, s0, s1, v[0], scale, field, accum);
+    if constexpr (NC > 1) scatter_component<MODE, NDIM, FORM, DEG, 1>(s, s0, s1, v[NC > 1 ? 1 : 0], scale, field, accum);
+    if constexpr (NC > 2) scatter_component<MODE, NDIM, FORM, DEG, 2>(s, s0, s1, v[NC > 2 ? 2 : 0], scale, field, accum);
+  }
+}
+
+template <class T, int NDIM, int FORM>
+static void launch_forms(bool gather, int mode, int degree, const void *pos, int64_t ps, const void *val, int64_t vs,
+                         int64_t n, const Grid &g, const double *cfield, double *field, void *res, int64_t rs,
+                         unsigned long long *accum, const double *slot, hipStream_t stream) {
+  const int threads = 256, blocks = blocks_for(n, threads, 1 << 16);
+#define FORM_LAUNCH(DEG)                                                                                           \
+  if (gather)                                                                                                      \
+    hipLaunchKernelGGL((gather_forms_kernel<T, NDIM, FORM, DEG>), dim3(blocks), dim3(threads), 0, stream,          \
+                       (const T *)pos, ps, n, g, cfield, (T *)res, rs);                                            \
+  else if (mode == MODE_ATOMIC)                                                                                    \
+    hipLaunchKernelGGL((scatter_forms_kernel<T, NDIM, FORM, DEG, MODE_ATOMIC>), dim3(blocks), dim3(threads), 0,    \
+                       stream, (const T *)pos, ps, (const T *)val, vs, n, g, field, accum, slot);                  \
+  else                                                                                                             \
+    hipLaunchKernelGGL((scatter_forms_kernel<T, NDIM, FORM, DEG, MODE_FIXED>), dim3(blocks), dim3(threads), 0,     \
+                       stream, (const T *)pos, ps, (const T *)val, vs, n, g, field, accum, slot);
+  if (degree == 1) {
+    FORM_LAUNCH(1)
+  } else if (degree == 2) {
+    FORM_LAUNCH(2)
+  } else {
+    FORM_LAUNCH(3)
+  }
+#undef FORM_LAUNCH
+}
+
+template <class T>
+static void dispatch_forms(bool gather, int mode, int degree, int form, int ndim, const void *pos, int64_t ps,
+                           const void *val, int64_t vs, int64_t n, const Grid &g, const double *cfield, double *field,
+                           void *res, int64_t rs, unsigned long long *accum, const double *slot, hipStream_t stream) {
+#define FORM_ARGS gather, mode, degree, pos, ps, val, vs, n, g, cfield, field, res, rs, accum, slot, stream
+  if (ndim == 1) launch_forms<T, 1, 1>(FORM_ARGS);
+  if (ndim == 2 && form == 1) launch_forms<T, 2, 1>(FORM_ARGS);
+  if (ndim == 2 && form == 2) launch_forms<T, 2, 2>(FORM_ARGS);
+  if (ndim == 3 && form == 1) launch_forms<T, 3, 1>(FORM_ARGS);
+  if (ndim == 3 && form == 2) launch_forms<T, 3, 2>(FORM_ARGS);
+  if (ndim == 3 && form == 3) launch_forms<T, 3, 3>(FORM_ARGS);
+#undef FORM_ARGS
+}
+
+// spline_forms.py:46-55 _check_form and spline_edges.py:31-34 _check.
+static int check_forms(int degree, int form, int ndim, int dtype, int64_t n, int64_t ncomp) {
+  if (degree < 1 || degree > 3) {
+    set_error("spline edge degrees are (1, 2, 3), got %d", degree);
+    return MPF_ERR_ARGUMENT;
+  }
+  if (ndim < 1 || ndim > 3) {
+    set_error("ndim must be 1, 2 or 3, got %d", ndim);
+    return MPF_ERR_ARGUMENT;
+  }
+  if (form < 1 || form > ndim) {
+    set_error("forms of %d dimensions are 1 to %d, got %d", ndim, ndim, form);
+    return MPF_ERR_ARGUMENT;
+  }
+  const int64_t want = form == ndim ? 1 : ndim;
+  if (ncomp != want) {
+    set_error("a %d-form in %dD has %lld components, got %lld", form, ndim, (long long)want, (long long)ncomp);
+    return MPF_ERR_ARGUMENT;
+  }
+  if (dtype != F32 && dtype != F64) {
+    set_error("dtype must be MPF_F32 or MPF_F64, got %d", dtype);
+    return MPF_ERR_ARGUMENT;
+  }
+  if (n < 0) {
+    set_error("negative marker count");
+    return MPF_ERR_ARGUMENT;
+  }
+  return MPF_OK;
+}
+
+}  // namespace mpf
+
+using namespace mpf;
+
+extern "C" int mpf_scatter_forms(int degree, int form, int ndim, const void *positions, int64_t pos_stride,
+                                 const void *values, int64_t val_stride, int dtype, int64_t n, int64_t ncomp,
+                                 double *field, const double *lower, const double *cell_size,
+                                 const int64_t *num_nodes, const int64_t *code, int mode, int scale_exp,
+                                 mpf_context *ctx, mpf_stream stream_) {
+  int rc = check_forms(degree, form, ndim, dtype, n, ncomp);
+  if (rc) return rc;
+  if (mode != MODE_ATOMIC && mode != MODE_FIXED) {
+    set_error("mode must be MPF_MODE_ATOMIC or MPF_MODE_FIXED, got %d", mode);
+    return MPF_ERR_ARGUMENT;
+  }
+  Grid g;
+  rc = make_grid(g, ndim, SPLINE, degree, lower, cell_size, num_nodes, code);
+  if (rc) return rc;
+  if (n == 0) return MPF_OK;
+  hipStream_t stream = (hipStream_t)stream_;
+  if (!ctx) ctx = default_context();
+  if (!ctx) return MPF_ERR_HIP;
+  const int64_t count = g.extent[0] * (ndim > 1 ? g.extent[1] : 1) * (ndim > 2 ? g.extent[2] : 1) * ncomp;
+  unsigned long long *accum = nullptr;
+  const double *slot = nullptr;
+  if (mode == MODE_FIXED) {
+    rc = ensure(ctx->accum, (size_t)count * 8, true, stream);
+    if (rc) return rc;
+    // a staggered weight on a clamped axis is a B-spline rescaled by degree / (knot span),
+    // at most `degree`; elsewhere the weights of a marker sum to at most one
+    double bound = 1.0;
+    for (int d = 0; d < ndim; ++d)
+      if (code[d] == 2) bound *= (double)degree;
+    rc = prepare_scale(ctx, values, val_stride, dtype, ncomp, n, bound, scale_exp, stream);
+    if (rc) return rc;
+    accum = (unsigned long long *)ctx->accum.ptr;
+    slot = scale_slot(ctx);
+  }
+  {
+    ProfileScope prof("scatter_forms", stream);
+    if (dtype == F32)
+      dispatch_forms<float>(false, mode, degree, form, ndim, positions, pos_stride, values, val_stride, n, g, nullptr,
+                            field, nullptr, 0, accum, slot, stream);
+    else
+      dispatch_forms<double>(false, mode, degree, form, ndim, positions, pos_stride, values, val_stride, n, g,
+                             nullptr, field, nullptr, 0, accum, slot, stream);
+  }
+  MPF_LAUNCH_CHECK();
+  if (mode == MODE_FIXED) return finalize_fixed(ctx, field, count, stream);
+  return MPF_OK;
+}
+
+extern "C" int mpf_gather_forms(int degree, int form, int ndim, const void *positions, int64_t pos_stride, int dtype,
+                                int64_t n, const double *field, int64_t ncomp, void *result, int64_t result_stride,
+                                const double *lower, const double *cell_size, const int64_t *num_nodes,
+                                const int64_t *code, mpf_context *ctx, mpf_stream stream_) {
+  (void)ctx;
+  int rc = check_forms(degree, form, ndim, dtype, n, ncomp);
+  if (rc) return rc;
+  Grid g;
+  rc = make_grid(g, ndim, SPLINE, degree, lower, cell_size, num_nodes, code);
+  if (rc) return rc;
+  if (n == 0) return MPF_OK;
+  hipStream_t stream = (hipStream_t)stream_;
+  ProfileScope prof("gather_forms", stream);
+  if (dtype == F32)
+    dispatch_forms<float>(true, MODE_ATOMIC, degree, form, ndim, positions, pos_stride, nullptr, 0, n, g, field,
+                          nullptr, result, result_stride, nullptr, nullptr, stream);
+  else
+    dispatch_forms<double>(true, MODE_ATOMIC, degree, form, ndim, positions, pos_stride, nullptr, 0, n, g, field,
+                           nullptr, result, result_stride, nullptr, nullptr, stream);
+  MPF_LAUNCH_CHECK();
+  return MPF_OK;
+}
+
+#define GEOM1 MPF_GEOM_1D
+#define GEOM2 MPF_GEOM_2D
+#define GEOM3 MPF_GEOM_3D
+#define F1 const double lo[1] = {lower_x}; const double cs[1] = {cell_size_x}; const int64_t nn[1] = {num_x}; \
+  const int64_t pc[1] = {periodic_x};
+#define F2 const double lo[2] = {lower_x, lower_y}; const double cs[2] = {cell_size_x, cell_size_y}; \
+  const int64_t nn[2] = {num_x, num_y}; const int64_t pc[2] = {periodic_x, periodic_y};
+#define F3 const double lo[3] = {lower_x, lower_y, lower_z}; \
+  const double cs[3] = {cell_size_x, cell_size_y, cell_size_z}; const int64_t nn[3] = {num_x, num_y, num_z}; \
+  const int64_t pc[3] = {periodic_x, periodic_y, periodic_z};
+
+#define FORMS_ENTRY(D)                                                                                              \
+  extern "C" int mpf_scatter_forms_spline_##D##d(MPF_SCATTER_IN, GEOM##D, int degree, int form, int mode,           \
+                                                 MPF_TAIL) {                                                      \
+    F##D return mpf_scatter_forms(degree, form, D, positions, pos_stride, values, val_stride, dtype, n, ncomp,    \
+                                  field, lo, cs, nn, pc, mode, MPF_SCALE_AUTO, ctx, stream);                      \
+  }                                                                                                              \
+  extern "C" int mpf_gather_forms_spline_##D##d(MPF_GATHER_IN, GEOM##D, int degree, int form, MPF_TAIL) {          \
+    F##D return mpf_gather_forms(degree, form, D, positions, pos_stride, dtype, n, field, ncomp, result,            \
+                                 result_stride, lo, cs, nn, pc, ctx, stream);                                     \
+  }
+
+FORMS_ENTRY(1)
+FORMS_ENTRY(2)
+FORMS_ENTRY(3)

@@ -32,6 +32,8 @@ KERNEL_NAMES = (
     "deposit_paths_spline_1d", "deposit_paths_spline_2d", "deposit_paths_spline_3d",
     "gather_edges_1d", "gather_edges_2d", "gather_edges_3d",
     "gather_edges_spline_1d", "gather_edges_spline_2d", "gather_edges_spline_3d",
+    "scatter_forms_spline_1d", "scatter_forms_spline_2d", "scatter_forms_spline_3d",
+    "gather_forms_spline_1d", "gather_forms_spline_2d", "gather_forms_spline_3d",
 )  # fmt: skip
 
 _state = threading.local()
@@ -271,6 +273,62 @@ def gather_edges(kind: int, degree: int, positions, edges: torch.Tensor, result:
         result2.copy_(out)
 
 
+def form_components(form: int, ndim: int) -> int:
+    """The components of a ``form``-form in ``ndim`` dimensions, ``ndim`` choose ``form``
+    (spline_forms.py:39-43)."""
+    if not 1 <= form <= ndim <= 3:
+        raise ValueError(f"forms of {ndim} dimensions are 1 to {ndim}, got {form}")
+    return 1 if form == ndim else ndim
+
+
+def _form_field(field, degree: int, form: int, num, code) -> torch.Tensor:
+    field = _field(field)
+    ncomp = form_components(form, len(num))
+    extent = _expected_extent(num, code, degree)
+    if tuple(field.shape) != (*extent, ncomp):
+        raise ValueError(f"a {form}-form field on this grid is {(*extent, ncomp)}, got {tuple(field.shape)}")
+    return field
+
+
+def scatter_forms(degree: int, form: int, positions, values, field: torch.Tensor, geometry, *,
+                  mode: int | None = None) -> None:
+    """The k-form deposit (mpf_scatter_forms): adds ``values`` (ncomp, N) into the
+    ``form``-form ``field`` num_nodes + (ncomp,) (num + degree - 1 on a clamped axis)."""
+    ndim, lower, cell, num, code, _ = _split(geometry)
+    field = _form_field(field, degree, form, num, code)
+    pos, ps = marker_rows(positions, ndim)
+    val, vs = marker_rows(values, field.shape[-1], pos.dtype)
+    if val.shape[1] != pos.shape[1]:
+        raise ValueError(f"{pos.shape[1]} positions for {val.shape[1]} values")
+    with torch.cuda.device(field.device):
+        _lib.call("mpf_scatter_forms", degree, form, ndim, pos.data_ptr(), ps, val.data_ptr(), vs, _dtype_code(pos),
+                  pos.shape[1], val.shape[0], field.data_ptr(), lower.ctypes.data, cell.ctypes.data,
+                  num.ctypes.data, code.ctypes.data, _mode_code() if mode is None else mode, _lib.SCALE_AUTO, None,
+                  stream_handle(field.device))
+
+
+def gather_forms(degree: int, form: int, positions, field: torch.Tensor, result: torch.Tensor, geometry) -> None:
+    """The k-form gather (mpf_gather_forms): overwrites ``result`` (ncomp, N) with sum W f."""
+    ndim, lower, cell, num, code, _ = _split(geometry)
+    field = _form_field(field, degree, form, num, code)
+    if not isinstance(result, torch.Tensor) or not result.is_cuda:
+        raise TypeError("result must be a torch tensor on the GPU")
+    pos, ps = marker_rows(positions, ndim, result.dtype if result.dtype in (torch.float32, torch.float64) else None)
+    ncomp = field.shape[-1]
+    result2 = result.reshape(1, -1) if result.dim() == 1 else result
+    if tuple(result2.shape) != (ncomp, pos.shape[1]):
+        raise ValueError(f"result must be ({ncomp}, {pos.shape[1]}), got {tuple(result.shape)}")
+    direct = result2.dtype == pos.dtype and (result2.shape[1] <= 1 or result2.stride(1) == 1)
+    out = result2 if direct else torch.empty((ncomp, pos.shape[1]), dtype=pos.dtype, device=field.device)
+    rs = out.stride(0) if out.shape[0] > 1 else out.shape[1]
+    with torch.cuda.device(field.device):
+        _lib.call("mpf_gather_forms", degree, form, ndim, pos.data_ptr(), ps, _dtype_code(pos), pos.shape[1],
+                  field.data_ptr(), ncomp, out.data_ptr(), rs, lower.ctypes.data, cell.ctypes.data,
+                  num.ctypes.data, code.ctypes.data, None, stream_handle(field.device))
+    if not direct:
+        result2.copy_(out)
+
+
 # ---- the catalogue ---------------------------------------------------------------------
 
 
@@ -394,7 +452,31 @@ gather_edges_1d, gather_edges_2d, gather_edges_3d = (_edge_gather_kernel(d, Fals
 gather_edges_spline_1d, gather_edges_spline_2d, gather_edges_spline_3d = (_edge_gather_kernel(d, True)
                                                                           for d in (1, 2, 3))
 
-#: every kernel by its catalogue name (catalog.py:51-88; the forms kernels are not built)
+
+def _forms_kernel(ndim: int, scatter: bool):
+    if scatter:
+        def kernel(positions, values, field, *geometry_degree_form):
+            *geometry, degree, form = geometry_degree_form
+            scatter_forms(int(degree), int(form), positions, values, field, geometry)
+
+        kernel.__name__ = f"scatter_forms_spline_{ndim}d"
+        kernel.__doc__ = (f"kernels/{kernel.__name__}: add the deposit of the k-form values at positions into field "
+                          "(degree and form the last two arguments).")
+    else:
+        def kernel(positions, field, result, *geometry_degree_form):
+            *geometry, degree, form = geometry_degree_form
+            gather_forms(int(degree), int(form), positions, field, result, geometry)
+
+        kernel.__name__ = f"gather_forms_spline_{ndim}d"
+        kernel.__doc__ = (f"kernels/{kernel.__name__}: overwrite result with the k-form field at positions "
+                          "(degree and form the last two arguments).")
+    return kernel
+
+
+scatter_forms_spline_1d, scatter_forms_spline_2d, scatter_forms_spline_3d = (_forms_kernel(d, True) for d in (1, 2, 3))
+gather_forms_spline_1d, gather_forms_spline_2d, gather_forms_spline_3d = (_forms_kernel(d, False) for d in (1, 2, 3))
+
+#: every kernel by its catalogue name (catalog.py:51-88)
 KERNELS = {name: globals()[name] for name in KERNEL_NAMES}
 
 

@@ -934,6 +934,163 @@ void orc_gather_edges_spline(int ndim, int degree, const double *pos, int64_t ps
 }
 
 /* ------------------------------------------------------------------------- */
+/* Forms of the spline de Rham sequence (faces, cells)                        */
+/* ------------------------------------------------------------------------- */
+
+/* spline_edges_pyccel.py:233-245 staggered: edges along their axis, faces across their
+ * normal, the top form along every axis. */
+static int form_staggered(int form, int ndim, int component, int axis) {
+    if (form == ndim) return 1;
+    if (form == 1) return axis == component;
+    if (form == 2 && ndim == 3) return axis != component;
+    return 0;
+}
+
+/* spline_forms.py:39-43 num_components: ndim choose form. */
+int orc_form_components(int form, int ndim) {
+    if (form < 1 || form > ndim || ndim > 3) return 0;
+    return form == ndim ? 1 : ndim;
+}
+
+/* One axis of component c: along_stencil (degree edges, some absent on a bounded axis)
+ * where c is staggered, spline_cell (degree + 1 nodes, all present) elsewhere. */
+static int form_axis(int form, int ndim, int c, int d, int degree, double rel, const grid_t *g, int64_t *idx,
+                     double *w, int *ex) {
+    if (form_staggered(form, ndim, c, d)) {
+        along_stencil(rel, g->num[d], g->code[d], degree, idx, w, ex);
+        return degree;
+    }
+    spline_cell(rel, g->num[d], g->code[d], degree, idx, w);
+    for (int a = 0; a <= degree; ++a) ex[a] = 1;
+    return degree + 1;
+}
+
+/* The marker's position in cells, clamped to [0, num - 1] on a non-periodic axis
+ * (gather_forms_spline_2d_pyccel.py:52-63). */
+static void form_relative(const grid_t *g, const double *pos, int64_t ps, int64_t m, double rel[3]) {
+    rel[0] = rel[1] = rel[2] = 0.0;
+    for (int d = 0; d < g->ndim; ++d) {
+        double r = (pos[d * ps + m] - g->lower[d]) * g->inverse[d];
+        if (g->code[d] != 1) {
+            if (r < 0.0) r = 0.0;
+            if (r > (double)(g->num[d] - 1)) r = (double)g->num[d] - 1.0;
+        }
+        rel[d] = r;
+    }
+}
+
+/* gather_forms_spline_{1,2,3}d_pyccel.py _gather_chunk (2D :25-90, 3D :30-130): per
+ * component the sum over the present k-cells of the weight product (left to right) times
+ * the field, loops x outer to z inner; overwrites result (ncomp, n). */
+void orc_gather_forms(int ndim, int degree, int form, const double *pos, int64_t ps, int64_t n, const double *field,
+                      double *result, int64_t rs, const double *lower, const double *cell, const int64_t *num,
+                      const int64_t *code) {
+    grid_t g;
+    make_grid(&g, ndim, KIND_SPLINE, degree, lower, cell, num, code);
+    const int ncomp = orc_form_components(form, ndim);
+    const int64_t s0 = g.extent[1] * g.extent[2], s1 = g.extent[2];
+    for (int64_t m = 0; m < n; ++m) {
+        double rel[3];
+        form_relative(&g, pos, ps, m, rel);
+        for (int c = 0; c < ncomp; ++c) {
+            int64_t idx[3][4] = {{0}, {0}, {0}};
+            double w[3][4] = {{1.0}, {1.0}, {1.0}};
+            int ex[3][4] = {{1, 1, 1, 1}, {1, 1, 1, 1}, {1, 1, 1, 1}}, size[3] = {1, 1, 1};
+            for (int d = 0; d < ndim; ++d) size[d] = form_axis(form, ndim, c, d, degree, rel[d], &g, idx[d], w[d], ex[d]);
+            double total = 0.0;
+            for (int a = 0; a < size[0]; ++a) {
+                if (!ex[0][a]) continue;
+                for (int b = 0; b < size[1]; ++b) {
+                    if (!ex[1][b]) continue;
+                    for (int e = 0; e < size[2]; ++e) {
+                        if (!ex[2][e]) continue;
+                        double v = w[0][a];
+                        if (ndim >= 2) v = v * w[1][b];
+                        if (ndim == 3) v = v * w[2][e];
+                        total += v * field[(idx[0][a] * s0 + idx[1][b] * s1 + idx[2][e]) * ncomp + c];
+                    }
+                }
+            }
+            result[c * rs + m] = total;
+        }
+    }
+}
+
+/* scatter_forms_spline_{1,2,3}d_pyccel.py _scatter_chunk (3D :23-121): a marker whose values
+ * are all zero is skipped before its position is read; each present k-cell adds
+ * w_x w_y w_z value. */
+static void forms_scatter_chunk(int degree, int form, const grid_t *g, const double *pos, int64_t ps,
+                                const double *val, int64_t vs, int ncomp, double *buf, int64_t first, int64_t last) {
+    const int ndim = g->ndim;
+    const int64_t s0 = g->extent[1] * g->extent[2], s1 = g->extent[2];
+    for (int64_t m = first; m < last; ++m) {
+        int live = 0;
+        for (int c = 0; c < ncomp; ++c)
+            if (val[c * vs + m] != 0.0) live = 1;
+        if (!live) continue;
+        double rel[3];
+        form_relative(g, pos, ps, m, rel);
+        for (int c = 0; c < ncomp; ++c) {
+            const double value = val[c * vs + m];
+            int64_t idx[3][4] = {{0}, {0}, {0}};
+            double w[3][4] = {{1.0}, {1.0}, {1.0}};
+            int ex[3][4] = {{1, 1, 1, 1}, {1, 1, 1, 1}, {1, 1, 1, 1}}, size[3] = {1, 1, 1};
+            for (int d = 0; d < ndim; ++d) size[d] = form_axis(form, ndim, c, d, degree, rel[d], g, idx[d], w[d], ex[d]);
+            for (int a = 0; a < size[0]; ++a) {
+                if (!ex[0][a]) continue;
+                for (int b = 0; b < size[1]; ++b) {
+                    if (!ex[1][b]) continue;
+                    for (int e = 0; e < size[2]; ++e) {
+                        if (!ex[2][e]) continue;
+                        double v = w[0][a];
+                        if (ndim >= 2) v = v * w[1][b];
+                        if (ndim == 3) v = v * w[2][e];
+                        buf[(idx[0][a] * s0 + idx[1][b] * s1 + idx[2][e]) * ncomp + c] += v * value;
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* scatter_forms_spline_3d_pyccel.py:150-210: adds; the node deposits' thread policy
+ * (serial for few markers or a large grid, else per-thread copies summed in thread order). */
+void orc_scatter_forms(int ndim, int degree, int form, const double *pos, int64_t ps, const double *val, int64_t vs,
+                       int64_t n, double *field, const double *lower, const double *cell, const int64_t *num,
+                       const int64_t *code, int threads) {
+    grid_t g;
+    make_grid(&g, ndim, KIND_SPLINE, degree, lower, cell, num, code);
+    const int ncomp = orc_form_components(form, ndim);
+    const int64_t values = grid_nodes(&g) * ncomp;
+    if (threads < 1) threads = 1;
+    int serial = threads == 1 || n < MIN_PARALLEL_SCATTER || n < values || (int64_t)threads * values > MAX_BUFFER_VALUES;
+    double *buffer = serial ? NULL : calloc((size_t)threads * (size_t)values, sizeof(double));
+    if (!buffer) {
+        forms_scatter_chunk(degree, form, &g, pos, ps, val, vs, ncomp, field, 0, n);
+        return;
+    }
+    const int64_t chunk = (n + threads - 1) / threads;
+#pragma omp parallel num_threads(threads)
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        int64_t first = (int64_t)t * chunk < n ? (int64_t)t * chunk : n;
+        int64_t last = first + chunk < n ? first + chunk : n;
+        forms_scatter_chunk(degree, form, &g, pos, ps, val, vs, ncomp, buffer + (size_t)t * values, first, last);
+#pragma omp barrier
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < values; ++i) {
+            double total = 0.0;
+            for (int k = 0; k < threads; ++k) total += buffer[(size_t)k * values + i];
+            field[i] += total;
+        }
+    }
+    free(buffer);
+}
+
+/* ------------------------------------------------------------------------- */
 /* Halo fill and fold, gradient and divergence                                */
 /* ------------------------------------------------------------------------- */
 

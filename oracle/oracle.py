@@ -45,6 +45,10 @@ def _load() -> ctypes.CDLL:
     lib.orc_deposit_spline.argtypes = [_INT, _INT, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P]
     lib.orc_gather_edges_whitney.argtypes = [_INT, _P, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P]
     lib.orc_gather_edges_spline.argtypes = [_INT, _INT, _P, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P]
+    lib.orc_gather_forms.argtypes = [_INT, _INT, _INT, _P, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P]
+    lib.orc_scatter_forms.argtypes = [_INT, _INT, _INT, _P, _I64, _P, _I64, _I64, _P, _P, _P, _P, _P, _INT]
+    lib.orc_form_components.argtypes = [_INT, _INT]
+    lib.orc_form_components.restype = _INT
     for name in ("orc_halo_fill", "orc_halo_fold"):
         getattr(lib, name).argtypes = [_INT, _P, _I64, _I64, _P, _P]
     lib.orc_divergence.argtypes = [_INT, _P, _P, _P, _P]
@@ -156,6 +160,46 @@ def gather_edges(positions, edges: np.ndarray, result: np.ndarray, *geometry, de
     else:
         lib().orc_gather_edges_spline(ndim, degree, _ptr(pos), ps, pos.shape[1], _ptr(edges), _ptr(result), rs,
                                       _ptr(lower), _ptr(cell), _ptr(num), _ptr(code))
+
+
+def form_components(form: int, ndim: int) -> int:
+    """The components of a ``form``-form in ``ndim`` dimensions (0 for no such form)."""
+    return int(lib().orc_form_components(form, ndim))
+
+
+def _form_field(field: np.ndarray, form: int, ndim: int) -> None:
+    if not (1 <= form <= ndim <= 3):
+        raise ValueError(f"forms of {ndim} dimensions are 1 to {ndim}, got {form}")
+    if field.shape[-1] != form_components(form, ndim):
+        raise ValueError(f"a {form}-form in {ndim}D has {form_components(form, ndim)} components")
+
+
+def gather_forms(positions, field: np.ndarray, result: np.ndarray, *geometry, degree: int, form: int) -> None:
+    """Overwrite ``result`` ``(ncomp, N)`` with the ``form``-form ``field`` (*extent, ncomp) at
+    ``positions`` (gather_forms_spline_*d)."""
+    ndim = len(geometry) // 4
+    pos, ps = _rows(positions)
+    field = np.ascontiguousarray(field, dtype=np.float64)
+    _form_field(field, form, ndim)
+    assert result.dtype == np.float64 and result.strides[1] == 8 and result.shape == (field.shape[-1], pos.shape[1])
+    rs = result.strides[0] // 8 if result.shape[0] > 1 else result.shape[1]
+    lower, cell, num, code = _geometry(geometry, ndim)
+    lib().orc_gather_forms(ndim, degree, form, _ptr(pos), ps, pos.shape[1], _ptr(field), _ptr(result), rs,
+                           _ptr(lower), _ptr(cell), _ptr(num), _ptr(code))
+
+
+def scatter_forms(positions, values, field: np.ndarray, *geometry, degree: int, form: int, threads: int = 1) -> None:
+    """Add the deposit of ``values`` (ncomp, N) at ``positions`` into the ``form``-form ``field``
+    (scatter_forms_spline_*d)."""
+    ndim = len(geometry) // 4
+    pos, ps = _rows(positions)
+    val, vs = _rows(values)
+    assert field.dtype == np.float64 and field.flags.c_contiguous
+    _form_field(field, form, ndim)
+    assert val.shape == (field.shape[-1], pos.shape[1])
+    lower, cell, num, code = _geometry(geometry, ndim)
+    lib().orc_scatter_forms(ndim, degree, form, _ptr(pos), ps, _ptr(val), vs, pos.shape[1], _ptr(field),
+                            _ptr(lower), _ptr(cell), _ptr(num), _ptr(code), threads)
 
 
 def _halo(name: str, data: np.ndarray, halo: int, periodic) -> None:

@@ -1,0 +1,176 @@
+"""GPU parity of the forms kernels (faces, cells: scatter_forms_spline_*d and
+gather_forms_spline_*d) against the CPU oracle, through the C ABI and the weightings.
+
+Bars: float64 gathers equal the oracle to the bit (the same float64 operations in the same
+order); float32 markers run the same float64 stencils, so the oracle fed the float32
+values agrees as tightly, and the float32 result within float32 rounding. Deposits: float
+atomics within 1e-12 of the field's largest value, fixed point within 1e-10 and
+bit-reproducible under any marker order.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+if torch.cuda.is_available():
+    from mpifield_amd import _lib
+    from mpifield_amd import kernels as K
+
+CASES = [(1, (1,)), (1, (0,)), (1, (2,)), (2, (1, 0)), (2, (2, 1)), (2, (0, 0)), (3, (0, 1, 2)), (3, (1, 1, 1)),
+         (3, (0, 0, 0)), (3, (2, 2, 2))]
+
+
+def geom(lower, cell, num, code):
+    return (*map(float, lower), *map(float, cell), *map(int, num), *map(int, code))
+
+
+def grid_case(ndim, codes, degree):
+    num = np.array([9, 7, 6])[:ndim]
+    h = np.array([0.25, 0.5, 0.1])[:ndim]
+    lower = np.array([0.0, -1.0, 2.0])[:ndim]
+    extent = tuple(int(n) + (degree - 1 if c == 2 else 0) for n, c in zip(num, codes))
+    return geom(lower, h, num, codes), lower, h, num, extent
+
+
+def markers(rng, lower, h, num, n):
+    """Markers over the grid and up to a third of it past each side."""
+    span = h * (num - 1)
+    return (lower[:, None] + rng.uniform(-0.3, 1.3, (len(num), n)) * span[:, None]).copy()
+
+
+def cuda(a):
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+
+
+def rel_err(got, want):
+    return np.abs(got - want).max() / max(np.abs(want).max(), 1e-300)
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+@pytest.mark.parametrize("ndim,codes", CASES)
+def test_forms_match_the_oracle(degree, ndim, codes):
+    rng = np.random.default_rng(degree + 7 * ndim + sum(codes))
+    g, lower, h, num, extent = grid_case(ndim, codes, degree)
+    n = 4000
+    for form in range(1, ndim + 1):
+        print(f"forms p={degree} {ndim}D codes={codes} form={form}", flush=True)
+        ncomp = K.form_components(form, ndim)
+        pos = markers(rng, lower, h, num, n)
+        field = rng.normal(size=(*extent, ncomp))
+        want = np.zeros((ncomp, n))
+        orc.gather_forms(pos, field, want, *g, degree=degree, form=form)
+        got = torch.empty((ncomp, n), dtype=torch.float64, device="cuda")
+        K.gather_forms(degree, form, cuda(pos), cuda(field), got, g)
+        assert np.array_equal(got.cpu().numpy(), want), rel_err(got.cpu().numpy(), want)
+        p32 = pos.astype(np.float32)
+        want32 = np.zeros((ncomp, n))
+        orc.gather_forms(p32.astype(np.float64), field, want32, *g, degree=degree, form=form)
+        got32 = torch.empty((ncomp, n), dtype=torch.float32, device="cuda")
+        K.gather_forms(degree, form, cuda(p32), cuda(field), got32, g)
+        np.testing.assert_allclose(got32.cpu().numpy(), want32, rtol=1e-6, atol=1e-6 * np.abs(want32).max())
+
+        values = rng.normal(size=(ncomp, n))
+        values[:, ::7] = 0.0
+        pos[:, ::7] = np.nan  # dead markers: their positions are not read
+        expect = np.zeros((*extent, ncomp))
+        orc.scatter_forms(pos, values, expect, *g, degree=degree, form=form)
+        for mode, bar in (("atomic", 1e-12), ("fixed", 1e-10)):
+            out = torch.zeros((*extent, ncomp), dtype=torch.float64, device="cuda")
+            with K.deposit_mode(mode):
+                K.scatter_forms(degree, form, cuda(pos), cuda(values), out, g)
+            assert rel_err(out.cpu().numpy(), expect) < bar, mode
+
+
+@pytest.mark.parametrize("ndim,codes,form", [(3, (1, 1, 1), 2), (3, (2, 0, 1), 1), (2, (1, 1), 2)])
+def test_fixed_forms_deposit_is_reproducible_under_any_order(ndim, codes, form):
+    rng = np.random.default_rng(5)
+    g, lower, h, num, extent = grid_case(ndim, codes, 3)
+    n = 60000
+    ncomp = K.form_components(form, ndim)
+    pos = markers(rng, lower, h, num, n)
+    values = rng.normal(size=(ncomp, n))
+    order = rng.permutation(n)
+    fields = []
+    for p, v in ((pos, values), (pos[:, order], values[:, order])):
+        out = torch.zeros((*extent, ncomp), dtype=torch.float64, device="cuda")
+        with K.deposit_mode("fixed"):
+            K.scatter_forms(3, form, cuda(p), cuda(v), out, g)
+        fields.append(out.cpu().numpy())
+    assert np.array_equal(fields[0], fields[1])
+    expect = np.zeros((*extent, ncomp))
+    orc.scatter_forms(pos, values, expect, *g, degree=3, form=form, threads=4)
+    assert rel_err(fields[0], expect) < 1e-10
+
+
+def test_faces_and_cells_through_the_weightings():
+    """Splines.faces / Splines.cells (splines.py:185-204): the GPU deposit and gather on
+    device tensors and NumPy arrays, cells summing to one inside and holding the whole
+    deposit when periodic (test_spline_forms.py:184-195), and the transpose identity."""
+    import mpifield_amd as m
+    from mpifield_amd.splines import CLAMPED_SPLINES, SPLINES
+
+    rng = np.random.default_rng(1)
+    grid2 = m.BlockGrid.from_geometry([0.0, 0.0], [0.5, 0.4], [9, 8], (True, True))
+    points = rng.uniform(0.0, 3.0, (2, 200))
+    for degree in (1, 2, 3):
+        cells = SPLINES[degree].cells
+        deposit = np.zeros((9, 8, 1))
+        cells.scatter_block(points, np.ones(200), deposit, grid2)
+        assert deposit.sum() == pytest.approx(200.0)
+        gathered = np.zeros((1, 200))
+        cells.gather_block(points, np.ones((9, 8, 1)), gathered, grid2)
+        np.testing.assert_allclose(gathered, 1.0, rtol=1e-14)
+
+    grid3 = m.BlockGrid.from_geometry([0.0, -1.0, 2.0], [0.25, 0.5, 0.1], [9, 7, 6], (False, True, False))
+    g3 = grid3.geometry
+    pts = rng.uniform(0.0, 1.0, (3, 3000)) * np.array([[2.0], [3.5], [0.5]]) + np.array([[0.0], [-1.0], [2.0]])
+    for degree in (1, 2, 3):
+        faces = SPLINES[degree].faces
+        field = torch.as_tensor(rng.normal(size=(9, 7, 6, 3)), device="cuda")
+        result = torch.empty((3, 3000), dtype=torch.float64, device="cuda")
+        faces.gather_block(cuda(pts), field, result, grid3)
+        want = np.zeros((3, 3000))
+        orc.gather_forms(pts, field.cpu().numpy(), want, *g3, degree=degree, form=2)
+        assert np.array_equal(result.cpu().numpy(), want)
+        values = rng.normal(size=(3, 3000))
+        deposit = torch.zeros((9, 7, 6, 3), dtype=torch.float64, device="cuda")
+        faces.scatter_block(cuda(pts), cuda(values), deposit, grid3)
+        assert float(torch.sum(deposit * field)) == pytest.approx(float(np.sum(want * values)), rel=1e-11)
+    with pytest.raises(ValueError, match="no 2D"):
+        SPLINES[2].faces.gather_block(points, np.zeros((9, 8, 3)), np.zeros((3, 200)), grid2)
+
+    clamped = m.BlockGrid.from_geometry([0.0] * 3, [0.25, 0.5, 0.1], [9, 7, 6], None, [True] * 3)
+    cells = CLAMPED_SPLINES[3].cells
+    density = np.zeros((1, 3000))
+    cube = rng.normal(size=(11, 9, 8, 1))
+    cells.gather_block(rng.uniform(0.0, 0.5, (3, 3000)), cube, density, clamped)
+    assert np.isfinite(density).all()
+
+
+def test_forms_reject_bad_forms_and_component_counts():
+    """spline_forms.py:46-55 _check_form and spline_edges.py:31-34 _check."""
+    g, lower, h, num, extent = grid_case(2, (1, 1), 2)
+    pos = cuda(np.zeros((2, 4)))
+    with pytest.raises(ValueError, match="forms of 2 dimensions are 1 to 2"):
+        K.gather_forms(2, 3, pos, torch.zeros((*extent, 1), dtype=torch.float64, device="cuda"),
+                       torch.empty((1, 4), dtype=torch.float64, device="cuda"), g)
+    with pytest.raises(ValueError, match="1-form field on this grid"):
+        K.scatter_forms(2, 1, pos, cuda(np.ones((1, 4))), torch.zeros((*extent, 1), dtype=torch.float64,
+                                                                        device="cuda"), g)
+    lower_, cell_, num_, code_ = (np.asarray(v) for v in (lower, h, num, [1, 1]))
+    field = torch.zeros((*extent, 1), dtype=torch.float64, device="cuda")
+    out = torch.empty((1, 4), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="has 2 components, got 1"):
+        _lib.call("mpf_gather_forms", 2, 1, 2, pos.data_ptr(), 4, _lib.F64, 4, field.data_ptr(), 1, out.data_ptr(), 4,
+                  lower_.astype(np.float64).ctypes.data, cell_.astype(np.float64).ctypes.data,
+                  num_.astype(np.int64).ctypes.data, code_.astype(np.int64).ctypes.data, None, K.stream_handle())
+    with pytest.raises(ValueError, match="spline edge degrees"):
+        _lib.call("mpf_gather_forms", 0, 2, 2, pos.data_ptr(), 4, _lib.F64, 4, field.data_ptr(), 1, out.data_ptr(), 4,
+                  lower_.astype(np.float64).ctypes.data, cell_.astype(np.float64).ctypes.data,
+                  num_.astype(np.int64).ctypes.data, code_.astype(np.int64).ctypes.data, None, K.stream_handle())

@@ -591,3 +591,162 @@ def test_spline_edge_gather_of_degree_one_is_whitney(ndim, periodic):
     orc.gather_edges(start, edges, whitney, *g, degree=0)
     orc.gather_edges(start, edges, spline, *g, degree=1)
     np.testing.assert_allclose(spline, whitney, rtol=1e-13, atol=1e-13)
+
+
+# ----------------------------------------------------------------------------------------
+# Forms of the spline sequence (faces, cells): gather_forms_spline_*d, scatter_forms_spline_*d
+
+FORM_CODES = [(1, (1,)), (1, (0,)), (1, (2,)), (2, (1, 0)), (2, (2, 1)), (3, (0, 1, 2)), (3, (1, 1, 1)),
+              (3, (0, 0, 0)), (3, (2, 2, 2))]
+
+
+def _form_grid(ndim, codes, degree):
+    num = np.array([7, 6, 8])[:ndim]
+    h = np.array([0.5, 0.4, 0.3])[:ndim]
+    lower = np.array([-1.0, 0.5, 2.0])[:ndim]
+    extent = tuple(int(n) + (degree - 1 if c == 2 else 0) for n, c in zip(num, codes))
+    return geom(lower, h, num, codes), lower, h, num, extent
+
+
+def _forward(values, axis, code):
+    """values[k + 1] - values[k] along ``axis``; a bounded or clamped axis has no last difference."""
+    out = np.roll(values, -1, axis=axis) - values
+    if code != 1:
+        index = [slice(None)] * values.ndim
+        index[axis] = -1
+        out[tuple(index)] = 0.0
+    return out
+
+
+def _exterior(cochain, form, codes):
+    """The discrete exterior derivative of a ``form``-cochain (*extent, ncomp): gradient,
+    curl (the circulation around each face) and divergence."""
+    ndim = len(codes)
+    if form == 0:
+        return np.stack([_forward(cochain[..., 0], a, codes[a]) for a in range(ndim)], axis=-1)
+    if form == 1 and ndim == 2:
+        return (_forward(cochain[..., 1], 0, codes[0]) - _forward(cochain[..., 0], 1, codes[1]))[..., None]
+    if form == 1:
+        return np.stack([_forward(cochain[..., (c + 2) % 3], (c + 1) % 3, codes[(c + 1) % 3])
+                         - _forward(cochain[..., (c + 1) % 3], (c + 2) % 3, codes[(c + 2) % 3]) for c in range(3)],
+                        axis=-1)
+    return sum(_forward(cochain[..., a], a, codes[a]) for a in range(3))[..., None]
+
+
+def _form_field(points, cochain, form, g, degree):
+    out = np.zeros((cochain.shape[-1], points.shape[1]))
+    if form == 0:
+        orc.gather("spline", points, cochain, out, *g, degree=degree)
+    else:
+        orc.gather_forms(points, cochain, out, *g, degree=degree, form=form)
+    return out
+
+
+def _inside_pieces(rng, ndim, codes, num, degree, n):
+    """Positions in cells, each at least 0.15 cells from the planes where the interpolants change
+    piece (integers for odd degrees and clamped axes, half-integers for even degrees)."""
+    rel = np.empty((ndim, n))
+    for d in range(ndim):
+        half = degree % 2 == 0 and codes[d] != 2
+        top = int(num[d]) - (1 if codes[d] == 1 else (3 if half else 2))
+        rel[d] = rng.integers(0, top + 1, n) + (0.5 if half else 0.0) + rng.uniform(0.15, 0.85, n)
+    return rel
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+@pytest.mark.parametrize("ndim,codes", FORM_CODES)
+def test_form_fields_of_the_exterior_derivative_are_the_derivatives(degree, ndim, codes):
+    """test_spline_forms.py:72-100/103-148 and test_clamped.py:152-195: the (k+1)-form field of
+    d c is the exterior derivative (gradient, curl, divergence) of the k-form field of c,
+    for every k, on periodic, bounded and clamped axes.  The derivatives in cells by a
+    five-point difference, exact for the cubic pieces, at points inside one piece."""
+    rng = np.random.default_rng(degree + 10 * ndim)
+    g, lower, h, num, extent = _form_grid(ndim, codes, degree)
+    rel = _inside_pieces(rng, ndim, codes, num, degree, 80)
+    points = lower[:, None] + rel * h[:, None]
+    step = 0.02
+    for form in range(ndim):
+        ncomp = 1 if form == 0 else orc.form_components(form, ndim)
+        cochain = rng.normal(size=(*extent, ncomp))
+        found = _form_field(points, _exterior(cochain, form, codes), form + 1, g, degree)
+        partial = []
+        for axis in range(ndim):
+            shift = np.zeros((ndim, 1))
+            shift[axis] = step * h[axis]
+            f = [_form_field(points + s * shift, cochain, form, g, degree) for s in (-2, -1, 1, 2)]
+            partial.append((f[0] - 8.0 * f[1] + 8.0 * f[2] - f[3]) / (12.0 * step))
+        if form == 0:
+            expected = np.array([partial[a][0] for a in range(ndim)])
+        elif form == 1 and ndim == 2:
+            expected = (partial[0][1] - partial[1][0])[None]
+        elif form == 1:
+            expected = np.array([partial[(c + 1) % 3][(c + 2) % 3] - partial[(c + 2) % 3][(c + 1) % 3]
+                                 for c in range(3)])
+        else:
+            expected = sum(partial[a][a] for a in range(3))[None]
+        np.testing.assert_allclose(found, expected, rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+@pytest.mark.parametrize("ndim,codes", FORM_CODES)
+def test_one_forms_are_the_edge_field_times_the_spacing(degree, ndim, codes):
+    """The 1-form kernels are the spline edge gather without its division by the cell size."""
+    rng = np.random.default_rng(7 * degree + ndim)
+    g, lower, h, num, extent = _form_grid(ndim, codes, degree)
+    points = lower[:, None] + rng.uniform(-0.5, 1.0, (ndim, 300)) * (h * (num - 1))[:, None] * 1.2
+    edges = rng.normal(size=(*extent, ndim))
+    field, forms = np.zeros((ndim, 300)), np.zeros((ndim, 300))
+    orc.gather_edges(points, edges, field, *g, degree=degree)
+    orc.gather_forms(points, edges, forms, *g, degree=degree, form=1)
+    np.testing.assert_allclose(forms, field * h[:, None], rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+@pytest.mark.parametrize("ndim,codes", FORM_CODES)
+def test_form_scatter_is_the_transpose_of_the_gather(degree, ndim, codes):
+    """test_spline_forms.py:151-181: sum(gathered * values) == sum(field * deposit), markers
+    partly outside the grid, every 7th dead."""
+    rng = np.random.default_rng(3 * degree + ndim)
+    g, lower, h, num, extent = _form_grid(ndim, codes, degree)
+    points = lower[:, None] + rng.uniform(-0.3, 1.2, (ndim, 400)) * (h * (num - 1))[:, None]
+    for form in range(1, ndim + 1):
+        ncomp = orc.form_components(form, ndim)
+        values = rng.normal(size=(ncomp, 400))
+        values[:, ::7] = 0.0
+        field = rng.normal(size=(*extent, ncomp))
+        deposit = np.zeros_like(field)
+        orc.scatter_forms(points, values, deposit, *g, degree=degree, form=form)
+        gathered = np.zeros_like(values)
+        orc.gather_forms(points, field, gathered, *g, degree=degree, form=form)
+        assert np.sum(gathered * values) == pytest.approx(np.sum(field * deposit), rel=1e-12)
+
+
+def test_cells_sum_to_one_inside_and_hold_the_whole_deposit_when_periodic():
+    """test_spline_forms.py:184-195."""
+    g = geom([0.0, 0.0], [0.5, 0.4], [9, 8], [1, 1])
+    points = np.random.default_rng(1).uniform(0.0, 3.0, (2, 200))
+    for degree in (1, 2, 3):
+        deposit = np.zeros((9, 8, 1))
+        orc.scatter_forms(points, np.ones((1, 200)), deposit, *g, degree=degree, form=2)
+        assert deposit.sum() == pytest.approx(200.0)
+        gathered = np.zeros((1, 200))
+        orc.gather_forms(points, np.ones((9, 8, 1)), gathered, *g, degree=degree, form=2)
+        np.testing.assert_allclose(gathered, 1.0, rtol=1e-14)
+
+
+def test_forms_skip_dead_markers_and_thread_like_the_serial_deposit():
+    """A marker whose values are all zero is not read (NaN positions deposit nothing); the
+    per-thread copies summed in thread order equal the serial deposit to rounding."""
+    rng = np.random.default_rng(12)
+    g = geom([0.0] * 3, [0.1] * 3, [8, 8, 8], [1, 0, 2])
+    points = rng.uniform(0.0, 0.7, (3, 30000))
+    values = rng.normal(size=(3, 30000))
+    points[:, ::5] = np.nan
+    values[:, ::5] = 0.0
+    serial, threaded = np.zeros((8, 8, 9, 3)), np.zeros((8, 8, 9, 3))
+    orc.scatter_forms(points, values, serial, *g, degree=2, form=2)
+    orc.scatter_forms(points, values, threaded, *g, degree=2, form=2, threads=4)
+    assert np.isfinite(serial).all()
+    np.testing.assert_allclose(threaded, serial, rtol=1e-12, atol=1e-12)
+    with pytest.raises(ValueError, match="has 3 components"):
+        orc.scatter_forms(points, values[:1], np.zeros((8, 8, 9, 1)), *g, degree=2, form=2)
