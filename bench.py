@@ -51,11 +51,12 @@ def parse() -> argparse.Namespace:
     p.add_argument("--only-headline", action="store_true", help="time the headline step only (for rocprofv3)")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only copy MIB MiB device-to-device (the counter calibration for rocprofv3 --pmc)")
-    p.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6, 7, 8),
+    p.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6, 7, 8, 9),
                    help="2: the headline (default); 3: 2D quadratic-edge path deposit, 1024^2, 2^25 paths; "
                    "4: 4096 batched 2D patches, ~2e8 markers; 5: differentiable 3D cubic gather of scatter, "
                    "128^3, 2^24 markers; 6: 2D Whitney path deposit, same paths as 3; 7: halo fold + fill, "
-                   "256^3 x 3, halo 2; 8: 2D quadratic-edge field at 2^25 markers (one JSON line each)")
+                   "256^3 x 3, halo 2; 8: 2D quadratic-edge field at 2^25 markers; 9: 3D quadratic faces (2-forms) "
+                   "scatter+gather, 256^3 x 3, 2^25 markers (one JSON line each)")
     return p.parse_args()
 
 
@@ -311,6 +312,66 @@ def config8(torch, args) -> dict:
     }
 
 
+def config9(torch, args) -> dict:
+    """The 2-forms (faces) of degree 2 in 3D on a 256^3 periodic grid: 2^25 float32 markers
+    uniform in [0, 1)^3, 3 values N(1, 0.1) each; one step = the fixed-point deposit of the
+    face values + the gather of a face field.  Timed on the markers sorted by stencil cell
+    (the order a particle code keeps; the sort runs once before timing) and in input order."""
+    from mpifield_amd import binning
+    from mpifield_amd import kernels as K
+
+    nodes, n, C = 256, 1 << 25, 3
+    h = 1.0 / nodes
+    g = (0.0, 0.0, 0.0, h, h, h, nodes, nodes, nodes, 1, 1, 1)
+    gen = torch.Generator(device="cuda").manual_seed(9)
+    pos = torch.rand((3, n), generator=gen, device="cuda", dtype=torch.float32)
+    val = (1.0 + 0.1 * torch.randn((C, n), generator=gen, device="cuda")).float()
+    bins = binning.bin_markers(pos, val, (2, 2), g)
+    spos, sval = bins.positions, bins.values
+    field = torch.zeros((nodes, nodes, nodes, C), dtype=torch.float64, device="cuda")
+    faces = torch.randn((nodes, nodes, nodes, C), generator=gen, device="cuda", dtype=torch.float64)
+    result = torch.empty((C, n), dtype=torch.float32, device="cuda")
+    out = {}
+    sc = time_events(torch, lambda: binning.scatter_forms_binned(bins, field, 2), args.steps, 2)
+    ga = time_events(torch, lambda: K.gather_forms(2, 2, spos, faces, result, g), args.steps, 2)
+    with K.deposit_mode("fixed"):
+        out["input"] = (time_events(torch, lambda: K.scatter_forms(2, 2, pos, val, field, g), max(1, args.steps // 4), 1),
+                        time_events(torch, lambda: K.gather_forms(2, 2, pos, faces, result, g), args.steps, 2))
+    with K.deposit_mode("atomic"):
+        out["direct_atomic_sorted"] = time_events(torch, lambda: K.scatter_forms(2, 2, spos, sval, field, g),
+                                                  max(1, args.steps // 4), 1)
+    per_marker = 12 + 12  # float32 positions + values (scatter) or results (gather)
+    alg_sc, alg_ga = n * per_marker + 2 * nodes**3 * C * 8, n * per_marker + nodes**3 * C * 8
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        sample, cores = 1 << 21, cpu_cores()
+        p_h = spos[:, :sample].double().cpu().numpy()
+        v_h = sval[:, :sample].double().cpu().numpy()
+        f_h = np.zeros((nodes, nodes, nodes, C))
+        r_h = np.zeros((C, sample))
+        t_sc = cpu_seconds(lambda: orc.scatter_forms(p_h, v_h, f_h, *g, degree=2, form=2, threads=cores))
+        t_ga = cpu_seconds(lambda: orc.gather_forms(p_h, f_h, r_h, *g, degree=2, form=2, threads=cores))
+        cpu = {"value": sample / (t_sc + t_ga), "unit": "markers/s", "cores": cores, "kind": "port",
+               "sample": f"first {sample} sorted markers: scatter {t_sc:.2f} s (serial: the grid exceeds the "
+                         f"reference's MAX_BUFFER_VALUES) + gather {t_ga:.2f} s on {cores} threads"}
+    return {
+        "metric": "markers/s, 3D quadratic faces (2-forms) scatter+gather 256^3 x 3, 2^25 markers",
+        "value": n / ((sc + ga) * 1e-3), "unit": "markers/s", "n_gpus": 1, "ms_per_step": sc + ga,
+        "higher_is_better": True, "dtype": "int64", "data": "synthetic",
+        "config": {"workload": f"3D 2-forms of degree 2, 256^3 x 3 periodic, {n} float32 markers sorted by stencil "
+                   "cell: tiled fixed-point deposit (bins made once before timing) + one-marker-per-work-item gather"},
+        "roofline": {"bound": "hbm", "achieved": alg_sc / (sc * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg_sc / (sc * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None, "kernel": "scatter_forms_tiled"},
+        "scatter_ms": sc, "gather_ms": ga, "gather_GBs": alg_ga / (ga * 1e-3) / 1e9,
+        "input_order": {"scatter_ms": out["input"][0], "gather_ms": out["input"][1],
+                        "note": "scatter bins inside the call (fixed point, tiled)"},
+        "direct_fp64_atomic_sorted_scatter_ms": out["direct_atomic_sorted"],
+        "cpu_baseline": cpu,
+    }
+
+
 def config7(torch, args) -> dict:
     """Halo fold then fill of one block's storage: 256^3 interior nodes, halo 2 on every side,
     3 components, periodic (the exchange a single block does locally)."""
@@ -488,7 +549,8 @@ def main() -> None:
         return
 
     if args.config != 2:
-        line = {3: config3, 4: config4, 5: config5, 6: config6, 7: config7, 8: config8}[args.config](torch, args)
+        line = {3: config3, 4: config4, 5: config5, 6: config6, 7: config7, 8: config8, 9: config9}[args.config](
+            torch, args)
         print(json.dumps(line), flush=True)
         return
 

@@ -224,6 +224,16 @@ int mpf_gather_forms(int degree, int form, int ndim, const void *positions, int6
                      const double *field, int64_t ncomp, void *result, int64_t result_stride, const double *lower,
                      const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
                      mpf_stream stream);
+/* The forms deposit of markers binned by mpf_bin_markers (kind MPF_KIND_SPLINE, the same
+ * degree, 2D or 3D, no clamped axis): sorted_positions (ndim, n) and sorted_values
+ * (ncomp, n) contiguous rows, start the bin offsets and scalars the fixed-point scale it
+ * wrote.  One workgroup per tile of anchor cells, int64 fixed point in LDS: the field is
+ * the MODE_FIXED mpf_scatter_forms deposit of the same markers, bit for bit.  The same
+ * call as scatter_forms_spline_*d on markers a particle code keeps sorted. */
+int mpf_scatter_forms_binned(int degree, int form, int ndim, const void *sorted_positions, const void *sorted_values,
+                             int dtype, int64_t n, int64_t ncomp, const void *start, const double *scalars,
+                             double *field, const double *lower, const double *cell_size, const int64_t *num_nodes,
+                             const int64_t *code, mpf_context *ctx, mpf_stream stream);
 
 /* ---- halos, gradient, divergence -------------------------------------------------- */
 

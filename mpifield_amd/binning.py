@@ -9,6 +9,11 @@ gathers many times::
     scatter_binned(bins, field)              # adds into field, bit-reproducible
     gather_binned(bins, field, result)       # result (ncomp, N) in bins order
 
+The forms deposit (faces, cells) reads bins of the spline nodes of its degree::
+
+    bins = bin_markers(positions, values, (KIND_SPLINE, degree), grid)
+    scatter_forms_binned(bins, faces, form=2)
+
 ``bins.order[i]`` is the input index of binned marker ``i``. `Weighting.scatter_block` /
 `gather_block` take this path by themselves for large batches (binning inside the call).
 """
@@ -201,3 +206,20 @@ def gather_derivative_binned(bins: MarkerBins, field: torch.Tensor, coef: torch.
                   field.data_ptr(), ncomp, k.data_ptr(), ks, out.data_ptr(), bins.n, lower.ctypes.data,
                   cell.ctypes.data, num.ctypes.data, code.ctypes.data, None, K.stream_handle(field.device))
     return out
+
+
+def scatter_forms_binned(bins: MarkerBins, field: torch.Tensor, form: int) -> None:
+    """Add the binned markers' values into the ``form``-form ``field`` with the tiled forms
+    deposit (mpf_scatter_forms_binned): the fixed-point deposit of scatter_forms_spline_*d,
+    bit for bit.  ``bins`` are of the spline nodes of the forms' degree (kind 2)."""
+    if bins.values is None:
+        raise ValueError("these bins hold no values (binned for a gather)")
+    if bins.kind != _lib.KIND_SPLINE:
+        raise ValueError("the forms deposit reads bins of the spline nodes (kind 2)")
+    field = K._field(field)
+    ndim, lower, cell, num, code = _geometry_arrays(bins.geometry)
+    with torch.cuda.device(field.device):
+        _lib.call("mpf_scatter_forms_binned", bins.degree, form, ndim, bins.positions.data_ptr(),
+                  bins.values.data_ptr(), K._dtype_code(bins.positions), bins.n, bins.ncomp, bins.start.data_ptr(),
+                  bins.scalars.data_ptr(), field.data_ptr(), lower.ctypes.data, cell.ctypes.data, num.ctypes.data,
+                  code.ctypes.data, None, K.stream_handle(field.device))

@@ -103,8 +103,6 @@ __device__ __forceinline__ int64_t wrap_index(int64_t k, int64_t num, int code) 
 // contiguous run of the binned arrays.  2D markers likewise in tiles of TILE2 x TILE2
 // cells (x-major, y fastest).  Partial tiles at the grid's end keep their empty bins.  1D
 // markers are binned in cell order.
-constexpr int TILE_X = 8, TILE_Y = 8, TILE_Z = 16, TILE_CELLS = TILE_X * TILE_Y * TILE_Z;
-constexpr int TILE2 = 32, TILE2_CELLS = TILE2 * TILE2;
 
 // Bins of a grid (before the dead bin): tile-padded cells in 2D and 3D, cells in 1D.
 __host__ __device__ __forceinline__ int64_t bins_of(int ndim, const int64_t *num) {

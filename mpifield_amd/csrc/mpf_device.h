@@ -30,6 +30,15 @@ struct Grid {
   double inverse[3];   // 1.0 / cell, as the reference computes it once per call
 };
 
+// k modulo num for a stencil node of a wrapped position (at most 3 nodes outside
+// [0, num)): on an axis of 8 nodes or more one conditional add or subtract, instead of the
+// 64-bit division a remainder costs; the remainder itself on tinier axes.
+__device__ __forceinline__ int64_t wrap_near(int64_t k, int64_t num) {
+  if (num < 8) return ((k % num) + num) % num;
+  k = k < 0 ? k + num : k;
+  return k >= num ? k - num : k;
+}
+
 __device__ __forceinline__ double periodic_position(double rel, int64_t num) {
   const double n = (double)num;
   return (rel >= 0.0 && rel < n) ? rel : rel - n * floor(rel / n);
@@ -62,9 +71,9 @@ __device__ __forceinline__ int quadratic_axis(double rel, int64_t num, int code,
   if (code == 1) {
     if (!(r >= 0.0 && r < (double)num)) r = r - (double)num * floor(r / (double)num);
     c = (int64_t)(r + 0.5);
-    node[0] = (c + num - 1) % num;
-    node[1] = c % num;
-    node[2] = (c + 1) % num;
+    node[0] = wrap_near(c - 1, num);
+    node[1] = wrap_near(c, num);
+    node[2] = wrap_near(c + 1, num);
   } else {
     if (r < 0.0) r = 0.0;
     if (r > (double)(num - 1)) r = (double)num - 1.0;
@@ -151,7 +160,7 @@ __device__ inline int spline_axis(double rel, int64_t num, int code, int degree,
   for (int a = 0; a <= degree; ++a) {
     int64_t k = first + a;
     if (code == 1) {
-      k = (k + num) % num;
+      k = wrap_near(k, num);
     } else {
       if (k < 0) k = 0;
       if (k > num - 1) k = num - 1;
@@ -202,7 +211,7 @@ __device__ __forceinline__ void along_stencil(double rel, int64_t num, int code,
   for (int a = 0; a < degree; ++a) {
     const int64_t k = first + a;
     if (code == 1) {
-      edge[a] = (k + 2 * num) % num;
+      edge[a] = wrap_near(k, num);
       exists[a] = 1;
     } else if (k < 0 || k > num - 2) {
       edge[a] = 0;
@@ -265,6 +274,15 @@ __device__ __forceinline__ double load_as_double(const T *p) {
 // Fixed point: the nearest integer of x, in two's complement (|x| < 2^62 by the choice of scale).
 __device__ __forceinline__ unsigned long long to_fixed(double x) {
   return (unsigned long long)__double2ll_rn(x);
+}
+
+// The same integer for |x| < 2^51 in two float64/integer operations: x + 1.5 * 2^52 lies in
+// [2^52, 2^53), where the unit in the last place is 1, so the add rounds x to the nearest
+// integer (ties to even, as __double2ll_rn), and the sum's bits less the bias's bits are
+// that integer in two's complement.
+__device__ __forceinline__ unsigned long long to_fixed_small(double x) {
+  const double bias = 6755399441055744.0;  // 1.5 * 2^52
+  return (unsigned long long)__double_as_longlong(x + bias) - (unsigned long long)__double_as_longlong(bias);
 }
 
 }  // namespace mpf

@@ -72,10 +72,10 @@ struct FormAxis {
   }
 };
 
-// Calls f(offset, weight) for every present k-cell of component C: the weights multiplied
-// x, y, z left to right (the trailing multiplies by 1.0 of absent axes are exact).
+// Calls f(i0, i1, i2, weight) for every present k-cell of component C (its index on each
+// axis, 0 past NDIM): the weights multiplied x, y, z left to right.
 template <int NDIM, int FORM, int DEG, int C, class F>
-__device__ __forceinline__ void for_each_cell(const FormStencil<NDIM, FORM, DEG> &s, int64_t s0, int64_t s1, F &&f) {
+__device__ __forceinline__ void for_each_cell(const FormStencil<NDIM, FORM, DEG> &s, F &&f) {
   using X = FormAxis<NDIM, FORM, DEG, C, 0>;
   using Y = FormAxis<NDIM, FORM, DEG, C, 1>;
   using Z = FormAxis<NDIM, FORM, DEG, C, 2>;
@@ -83,27 +83,30 @@ __device__ __forceinline__ void for_each_cell(const FormStencil<NDIM, FORM, DEG>
   for (int a = 0; a < X::size; ++a) {
     if (!X::present(s, a)) continue;
     const double wa = X::weight(s, a);
-    const int64_t ia = X::index(s, a) * s0;
 #pragma unroll
     for (int b = 0; b < Y::size; ++b) {
       if (!Y::present(s, b)) continue;
       const double wb = NDIM >= 2 ? wa * Y::weight(s, b) : wa;
-      const int64_t ib = ia + Y::index(s, b) * s1;
 #pragma unroll
       for (int e = 0; e < Z::size; ++e) {
         if (!Z::present(s, e)) continue;
         const double w = NDIM == 3 ? wb * Z::weight(s, e) : wb;
-        f(ib + Z::index(s, e), w);
+        f(X::index(s, a), Y::index(s, b), Z::index(s, e), w);
       }
     }
   }
 }
 
+// The position in cells, clamped to [0, num - 1] on a non-periodic axis
+// (gather_forms_spline_2d_pyccel.py:52-63).  A non-finite one, for which the reference's
+// integer cast is undefined, is taken as 0 (the cell the binning gives it), so that every
+// index stays inside the field.
 template <int NDIM>
 __device__ __forceinline__ void form_relative(const double *x, const Grid &g, double *rel) {
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
     double r = (x[d] - g.lower[d]) * g.inverse[d];
+    if (!isfinite(r)) r = 0.0;
     if (g.code[d] != 1) {
       if (r < 0.0) r = 0.0;
       if (r > (double)(g.num[d] - 1)) r = (double)g.num[d] - 1.0;
@@ -125,7 +128,9 @@ __device__ __forceinline__ void gather_component(const FormStencil<NDIM, FORM, D
   constexpr int NC = form_components<NDIM, FORM>();
   if constexpr (C < NC) {
     double total = 0.0;
-    for_each_cell<NDIM, FORM, DEG, C>(s, s0, s1, [&](int64_t at, double w) { total += w * field[at * NC + C]; });
+    for_each_cell<NDIM, FORM, DEG, C>(s, [&](int64_t i0, int64_t i1, int64_t i2, double w) {
+      total += w * field[(i0 * s0 + i1 * s1 + i2) * NC + C];
+    });
     res[C * rs + m] = (T)total;
   }
 }
@@ -155,7 +160,8 @@ __device__ __forceinline__ void scatter_component(const FormStencil<NDIM, FORM, 
                                                   unsigned long long *__restrict__ accum) {
   constexpr int NC = form_components<NDIM, FORM>();
   if constexpr (C < NC) {
-    for_each_cell<NDIM, FORM, DEG, C>(s, s0, s1, [&](int64_t at, double w) {
+    for_each_cell<NDIM, FORM, DEG, C>(s, [&](int64_t i0, int64_t i1, int64_t i2, double w) {
+      const int64_t at = i0 * s0 + i1 * s1 + i2;
       const double contribution = w * value;
       if (MODE == MODE_ATOMIC)
         unsafeAtomicAdd(field + at * NC + C, contribution);
@@ -194,6 +200,209 @@ __global__ void __launch_bounds__(256) scatter_forms_kernel(const T *__restrict_
     if constexpr (NC > 1) scatter_component<MODE, NDIM, FORM, DEG, 1>(s, s0, s1, v[NC > 1 ? 1 : 0], scale, field, accum);
     if constexpr (NC > 2) scatter_component<MODE, NDIM, FORM, DEG, 2>(s, s0, s1, v[NC > 2 ? 2 : 0], scale, field, accum);
   }
+}
+
+// ---- the tiled deposit of binned markers --------------------------------------------------
+//
+// Markers binned by the anchor cell of the degree-DEG spline nodes (mpf_bin_markers, kind
+// SPLINE), the same tiles as the node kernels: one workgroup per tile, an int64
+// fixed-point accumulator over the tile's footprint in LDS, each work-item a contiguous run
+// of the tile's markers adding to_fixed(w * v * 2^k) per k-cell -- the integers of the
+// one-marker-per-work-item fixed-point deposit, so the field is the same bit for bit --
+// then the tile's nonzero words into the global int64 accumulator.  Along each axis a
+// marker's nodes are anchor - DEG/2 .. anchor - DEG/2 + DEG and its staggered k-cells lie
+// in anchor - DEG/2 - 1 .. anchor + 1 (one lower where a bounded wall clamps the edge
+// stencil), so the footprint is the tile plus DEG/2 + 1 below and DEG - DEG/2 + 1 above.
+// A periodic axis folds the footprint (it needs num >= the footprint); a bounded one
+// clamps.  The tiles add their words with integer atomics (exact, so the field does not
+// depend on the order); running overlapping tiles in separate launches without atomics
+// measured slower (the launches' tails cost more than the atomics).
+template <int NDIM, int DEG>
+struct FormTile {
+  static constexpr int TX = NDIM == 3 ? TILE_X : TILE2, TY = NDIM == 3 ? TILE_Y : TILE2, TZ = NDIM == 3 ? TILE_Z : 1;
+  static constexpr int TC = TX * TY * TZ;
+  static constexpr int LO = DEG / 2 + 1;             // footprint nodes below the tile
+  static constexpr int SPAN = DEG + 2;               // footprint nodes beyond the tile's own
+  static constexpr int LX = TX + SPAN, LY = TY + SPAN, LZ = NDIM == 3 ? TZ + SPAN : 1;
+};
+
+// The footprint slot of node k (already wrapped or clamped) of an axis whose footprint
+// starts at unwrapped node `base`; -1 when outside (which the stencils never give).
+__device__ __forceinline__ int form_slot(int64_t k, int base, int len, int64_t num, int code) {
+  int64_t s = k - base;
+  if (code == 1) s = s < 0 ? s + num : (s >= num ? s - num : s);
+  return (s >= 0 && s < len) ? (int)s : -1;
+}
+
+// Component C's present k-cells of one marker as tile slots: per axis the slot offset of
+// each stencil entry (x: slot * LY * LZ, y: slot * LZ, z: slot), its weight, and whether it
+// exists; `inside` is false when some entry falls outside the footprint.
+template <int NDIM, int FORM, int DEG, int C, int LX, int LY, int LZ>
+struct TileCells {
+  using X = FormAxis<NDIM, FORM, DEG, C, 0>;
+  using Y = FormAxis<NDIM, FORM, DEG, C, 1>;
+  using Z = FormAxis<NDIM, FORM, DEG, C, 2>;
+  int ox[X::size], oy[Y::size], oz[Z::size];
+  bool inside = true;
+
+  __device__ __forceinline__ TileCells(const FormStencil<NDIM, FORM, DEG> &s, const Grid &g, int bx, int by, int bz) {
+#pragma unroll
+    for (int a = 0; a < X::size; ++a) {
+      const int sl = form_slot(X::index(s, a), bx, LX, g.num[0], g.code[0]);
+      inside = inside && (sl >= 0 || !X::present(s, a));
+      ox[a] = sl * (LY * LZ);
+    }
+#pragma unroll
+    for (int b = 0; b < Y::size; ++b) {
+      const int sl = NDIM >= 2 ? form_slot(Y::index(s, b), by, LY, g.num[1], g.code[1]) : 0;
+      inside = inside && (sl >= 0 || !Y::present(s, b));
+      oy[b] = sl * LZ;
+    }
+#pragma unroll
+    for (int e = 0; e < Z::size; ++e) {
+      const int sl = NDIM == 3 ? form_slot(Z::index(s, e), bz, LZ, g.num[2], g.code[2]) : 0;
+      inside = inside && (sl >= 0 || !Z::present(s, e));
+      oz[e] = sl;
+    }
+  }
+};
+
+template <class T, int NDIM, int FORM, int DEG>
+__global__ void __launch_bounds__(256) scatter_forms_tiled(const T *__restrict__ pos, const T *__restrict__ val,
+                                                           int64_t n, const uint32_t *__restrict__ start, Grid g,
+                                                           int nty, int ntz, unsigned long long *__restrict__ accum,
+                                                           const double *__restrict__ scalars) {
+  using Ti = FormTile<NDIM, DEG>;
+  constexpr int NC = form_components<NDIM, FORM>(), BLOCK = 256;
+  constexpr int LX = Ti::LX, LY = Ti::LY, LZ = Ti::LZ, VOL = LX * LY * LZ, WORDS = VOL * NC;
+  __shared__ unsigned long long acc[WORDS];  // component-major: acc[c * VOL + slot]
+  const int tile = blockIdx.x, t = threadIdx.x;
+  const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
+  const int bx = tx * Ti::TX - Ti::LO, by = ty * Ti::TY - Ti::LO, bz = NDIM == 3 ? tz * Ti::TZ - Ti::LO : 0;
+  for (int i = t; i < WORDS; i += BLOCK) acc[i] = 0ull;
+  const uint32_t first = start[(int64_t)tile * Ti::TC], last = start[(int64_t)(tile + 1) * Ti::TC];
+  const double scale = scalars[0];
+  int64_t s0, s1;
+  form_strides<NDIM>(g, s0, s1);
+  __syncthreads();
+  const uint32_t chunk = (last - first + BLOCK - 1) / BLOCK;
+  const uint32_t lo = first + t * chunk, hi = lo + chunk < last ? lo + chunk : last;
+  for (uint32_t m = lo; m < hi; ++m) {
+    double x[NDIM], rel[NDIM], vs[NC];
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) x[d] = (double)pos[d * n + m];
+    // value * 2^k: a power-of-two scale, so round(w * value) * 2^k == round(w * (value * 2^k))
+    // and the integers are the direct deposit's
+    double vmax = 0.0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      vs[c] = (double)val[c * n + m] * scale;
+      vmax = fmax(vmax, fabs(vs[c]));
+    }
+    form_relative<NDIM>(x, g, rel);
+    FormStencil<NDIM, FORM, DEG> s;
+    s.make(rel, g);
+    // the weights are at most 1 off clamped axes: below 2^50 quanta the contributions take
+    // the two-operation rounding (the scale the binning picks keeps a value below 2^49)
+    const bool small = vmax < 0x1p50;
+    auto add = [&](auto cc, auto fast) {
+      constexpr int C = decltype(cc)::value;
+      if constexpr (C < NC) {
+        const double v = vs[C];
+        TileCells<NDIM, FORM, DEG, C, LX, LY, LZ> cells(s, g, bx, by, bz);
+        using X = typename decltype(cells)::X;
+        using Y = typename decltype(cells)::Y;
+        using Z = typename decltype(cells)::Z;
+        if (cells.inside) {
+#pragma unroll
+          for (int a = 0; a < X::size; ++a) {
+            if (!X::present(s, a)) continue;
+            const double wa = X::weight(s, a);
+#pragma unroll
+            for (int b = 0; b < Y::size; ++b) {
+              if (!Y::present(s, b)) continue;
+              const double wb = NDIM >= 2 ? wa * Y::weight(s, b) : wa;
+              const int oab = C * VOL + cells.ox[a] + cells.oy[b];
+#pragma unroll
+              for (int e = 0; e < Z::size; ++e) {
+                if (!Z::present(s, e)) continue;
+                const double y = (NDIM == 3 ? wb * Z::weight(s, e) : wb) * v;
+                atomicAdd(acc + oab + cells.oz[e], decltype(fast)::value ? to_fixed_small(y) : to_fixed(y));
+              }
+            }
+          }
+        } else {  // outside the footprint (which the stencils never give): the global accumulator
+          for_each_cell<NDIM, FORM, DEG, C>(s, [&](int64_t i0, int64_t i1, int64_t i2, double w) {
+            atomicAdd(accum + (i0 * s0 + i1 * s1 + i2) * NC + C, to_fixed(w * v));
+          });
+        }
+      }
+    };
+    auto add_all = [&](auto fast) {
+      add(std::integral_constant<int, 0>{}, fast);
+      add(std::integral_constant<int, 1>{}, fast);
+      add(std::integral_constant<int, 2>{}, fast);
+    };
+    if (small)
+      add_all(std::true_type{});
+    else
+      add_all(std::false_type{});
+  }
+  __syncthreads();
+  for (int i = t; i < WORDS; i += BLOCK) {
+    const unsigned long long q = acc[i];
+    if (q == 0ull) continue;
+    const int c = i / VOL, sl = i - c * VOL, sz = sl % LZ, sy = (sl / LZ) % LY, sx = sl / (LZ * LY);
+    int64_t gx = bx + sx, gy = by + sy, gz = bz + sz;
+    if (g.code[0] == 1) gx = gx < 0 ? gx + g.num[0] : (gx >= g.num[0] ? gx - g.num[0] : gx);
+    if (NDIM >= 2 && g.code[1] == 1) gy = gy < 0 ? gy + g.num[1] : (gy >= g.num[1] ? gy - g.num[1] : gy);
+    if (NDIM == 3 && g.code[2] == 1) gz = gz < 0 ? gz + g.num[2] : (gz >= g.num[2] ? gz - g.num[2] : gz);
+    atomicAdd(accum + (gx * s0 + gy * s1 + gz) * NC + c, q);
+  }
+}
+
+// Whether the tiled deposit takes this case: 2D or 3D, no clamped axis, and a periodic
+// axis at least as long as the footprint.
+static bool forms_tiled_supported(int degree, int ndim, const Grid &g) {
+  if (ndim != 2 && ndim != 3) return false;
+  const int span = degree + 2;
+  const int tile[3] = {ndim == 3 ? TILE_X : TILE2, ndim == 3 ? TILE_Y : TILE2, TILE_Z};
+  for (int d = 0; d < ndim; ++d) {
+    if (g.code[d] == 2) return false;
+    if (g.code[d] == 1 && g.num[d] < tile[d] + span) return false;
+  }
+  return true;
+}
+
+template <class T, int NDIM>
+static int launch_forms_tiled(int form, int degree, const void *pos, const void *val, int64_t n,
+                              const uint32_t *start, const Grid &g, unsigned long long *accum,
+                              const double *scalars, hipStream_t stream) {
+  const int tx = NDIM == 3 ? TILE_X : TILE2, ty = NDIM == 3 ? TILE_Y : TILE2;
+  const int ntx = (int)((g.num[0] + tx - 1) / tx), nty = (int)((g.num[1] + ty - 1) / ty);
+  const int ntz = NDIM == 3 ? (int)((g.num[2] + TILE_Z - 1) / TILE_Z) : 1;
+  const int blocks = ntx * nty * ntz;
+#define FT(F, D)                                                                                                  \
+  hipLaunchKernelGGL((scatter_forms_tiled<T, NDIM, F, D>), dim3(blocks), dim3(256), 0, stream, (const T *)pos,     \
+                     (const T *)val, n, start, g, nty, ntz, accum, scalars)
+#define FT_DEG(F)       \
+  if (degree == 1)      \
+    FT(F, 1);           \
+  else if (degree == 2) \
+    FT(F, 2);           \
+  else                  \
+    FT(F, 3);
+  if (form == 1) {
+    FT_DEG(1)
+  } else if (form == 2) {
+    FT_DEG(2)
+  } else if constexpr (NDIM == 3) {
+    FT_DEG(3)
+  }
+#undef FT_DEG
+#undef FT
+  MPF_LAUNCH_CHECK();
+  return MPF_OK;
 }
 
 template <class T, int NDIM, int FORM>
@@ -338,6 +547,49 @@ extern "C" int mpf_gather_forms(int degree, int form, int ndim, const void *posi
                            nullptr, result, result_stride, nullptr, nullptr, stream);
   MPF_LAUNCH_CHECK();
   return MPF_OK;
+}
+
+extern "C" int mpf_scatter_forms_binned(int degree, int form, int ndim, const void *sorted_positions,
+                                        const void *sorted_values, int dtype, int64_t n, int64_t ncomp,
+                                        const void *start, const double *scalars, double *field,
+                                        const double *lower, const double *cell_size, const int64_t *num_nodes,
+                                        const int64_t *code, mpf_context *ctx, mpf_stream stream_) {
+  int rc = check_forms(degree, form, ndim, dtype, n, ncomp);
+  if (rc) return rc;
+  Grid g;
+  rc = make_grid(g, ndim, SPLINE, degree, lower, cell_size, num_nodes, code);
+  if (rc) return rc;
+  if (!forms_tiled_supported(degree, ndim, g)) {
+    set_error("the tiled forms deposit takes 2D and 3D grids without clamped axes whose periodic axes hold at "
+              "least a tile and %d nodes", degree + 2);
+    return MPF_ERR_UNSUPPORTED;
+  }
+  if (n == 0) return MPF_OK;
+  hipStream_t stream = (hipStream_t)stream_;
+  if (!ctx) ctx = default_context();
+  if (!ctx) return MPF_ERR_HIP;
+  const int64_t count = g.extent[0] * g.extent[1] * (ndim > 2 ? g.extent[2] : 1) * ncomp;
+  rc = ensure(ctx->accum, (size_t)count * 8, true, stream);
+  if (rc) return rc;
+  unsigned long long *accum = (unsigned long long *)ctx->accum.ptr;
+  const uint32_t *bins = (const uint32_t *)start;
+  {
+    ProfileScope prof("scatter_forms_tiled", stream);
+    if (ndim == 2 && dtype == F32)
+      rc = launch_forms_tiled<float, 2>(form, degree, sorted_positions, sorted_values, n, bins, g, accum, scalars,
+                                        stream);
+    if (ndim == 2 && dtype == F64)
+      rc = launch_forms_tiled<double, 2>(form, degree, sorted_positions, sorted_values, n, bins, g, accum, scalars,
+                                         stream);
+    if (ndim == 3 && dtype == F32)
+      rc = launch_forms_tiled<float, 3>(form, degree, sorted_positions, sorted_values, n, bins, g, accum, scalars,
+                                        stream);
+    if (ndim == 3 && dtype == F64)
+      rc = launch_forms_tiled<double, 3>(form, degree, sorted_positions, sorted_values, n, bins, g, accum, scalars,
+                                         stream);
+  }
+  if (rc) return rc;
+  return finalize_fixed_with(ctx, field, count, scalars, stream);
 }
 
 #define GEOM1 MPF_GEOM_1D

@@ -34,6 +34,12 @@ struct mpf_context {
 
 namespace mpf {
 
+// The tiles of the binned kernels: 3D anchor cells in tiles of TILE_X x TILE_Y x TILE_Z
+// (x-major, z fastest), 2D ones in TILE2 x TILE2 (x-major, y fastest); the bins of a tile
+// are consecutive and the tiles follow each other in x-major order.
+constexpr int TILE_X = 8, TILE_Y = 8, TILE_Z = 16, TILE_CELLS = TILE_X * TILE_Y * TILE_Z;
+constexpr int TILE2 = 32, TILE2_CELLS = TILE2 * TILE2;
+
 void set_error(const char *fmt, ...);
 int fail_hip(hipError_t e, const char *what);
 

@@ -745,7 +745,7 @@ __global__ void __launch_bounds__(BLOCK, 4) spline_paths_tiled(const T *__restri
 #pragma unroll
     for (int d = 0; d < NDIM; ++d) {
       int64_t k = base[d] + l[d];
-      if (g.code[d] == 1) k = ((k % g.num[d]) + g.num[d]) % g.num[d];
+      if (g.code[d] == 1 && (k < 0 || k >= g.num[d])) k = ((k % g.num[d]) + g.num[d]) % g.num[d];
       inside = inside && k >= 0 && k < g.extent[d];
       at += k * g.stride[d];
     }

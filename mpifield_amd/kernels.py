@@ -300,11 +300,27 @@ def scatter_forms(degree: int, form: int, positions, values, field: torch.Tensor
     val, vs = marker_rows(values, field.shape[-1], pos.dtype)
     if val.shape[1] != pos.shape[1]:
         raise ValueError(f"{pos.shape[1]} positions for {val.shape[1]} values")
+    mode_code = _mode_code() if mode is None else mode
+    if (mode_code == _lib.MODE_FIXED and BINNED_MIN_MARKERS <= pos.shape[1] < 1 << 31
+            and forms_tiled_ok(degree, num, code)):
+        from mpifield_amd import binning
+
+        bins = binning.bin_markers(pos, val, (_lib.KIND_SPLINE, degree), geometry)
+        binning.scatter_forms_binned(bins, field, form)
+        return
     with torch.cuda.device(field.device):
         _lib.call("mpf_scatter_forms", degree, form, ndim, pos.data_ptr(), ps, val.data_ptr(), vs, _dtype_code(pos),
                   pos.shape[1], val.shape[0], field.data_ptr(), lower.ctypes.data, cell.ctypes.data,
-                  num.ctypes.data, code.ctypes.data, _mode_code() if mode is None else mode, _lib.SCALE_AUTO, None,
-                  stream_handle(field.device))
+                  num.ctypes.data, code.ctypes.data, mode_code, _lib.SCALE_AUTO, None, stream_handle(field.device))
+
+
+def forms_tiled_ok(degree: int, num, code) -> bool:
+    """Whether the tiled forms deposit takes this grid (mpf_forms.hip's forms_tiled_supported):
+    2D or 3D, no clamped axis, every periodic axis at least a tile and degree + 2 nodes long."""
+    if len(num) not in (2, 3):
+        return False
+    tiles = (_TILE if len(num) == 3 else (32, 32))
+    return all(int(c) != 2 and (int(c) != 1 or int(n) >= t + degree + 2) for n, c, t in zip(num, code, tiles))
 
 
 def gather_forms(degree: int, form: int, positions, field: torch.Tensor, result: torch.Tensor, geometry) -> None:

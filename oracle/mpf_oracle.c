@@ -982,14 +982,13 @@ static void form_relative(const grid_t *g, const double *pos, int64_t ps, int64_
 /* gather_forms_spline_{1,2,3}d_pyccel.py _gather_chunk (2D :25-90, 3D :30-130): per
  * component the sum over the present k-cells of the weight product (left to right) times
  * the field, loops x outer to z inner; overwrites result (ncomp, n). */
-void orc_gather_forms(int ndim, int degree, int form, const double *pos, int64_t ps, int64_t n, const double *field,
-                      double *result, int64_t rs, const double *lower, const double *cell, const int64_t *num,
-                      const int64_t *code) {
-    grid_t g;
-    make_grid(&g, ndim, KIND_SPLINE, degree, lower, cell, num, code);
+static void gather_forms_chunk(int degree, int form, const grid_t *gp, const double *pos, int64_t ps,
+                               const double *field, double *result, int64_t rs, int64_t first, int64_t last) {
+    const grid_t g = *gp;
+    const int ndim = g.ndim;
     const int ncomp = orc_form_components(form, ndim);
     const int64_t s0 = g.extent[1] * g.extent[2], s1 = g.extent[2];
-    for (int64_t m = 0; m < n; ++m) {
+    for (int64_t m = first; m < last; ++m) {
         double rel[3];
         form_relative(&g, pos, ps, m, rel);
         for (int c = 0; c < ncomp; ++c) {
@@ -1013,6 +1012,30 @@ void orc_gather_forms(int ndim, int degree, int form, const double *pos, int64_t
             }
             result[c * rs + m] = total;
         }
+    }
+}
+
+/* gather_forms_spline_2d_pyccel.py:93-150: serial for one thread or fewer than 4096 markers,
+ * else contiguous chunks of the markers, one per thread. */
+void orc_gather_forms(int ndim, int degree, int form, const double *pos, int64_t ps, int64_t n, const double *field,
+                      double *result, int64_t rs, const double *lower, const double *cell, const int64_t *num,
+                      const int64_t *code, int threads) {
+    grid_t g;
+    make_grid(&g, ndim, KIND_SPLINE, degree, lower, cell, num, code);
+    if (threads <= 1 || n < MIN_PARALLEL_GATHER) {
+        gather_forms_chunk(degree, form, &g, pos, ps, field, result, rs, 0, n);
+        return;
+    }
+    const int64_t chunk = (n + threads - 1) / threads;
+#pragma omp parallel num_threads(threads)
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        int64_t first = (int64_t)t * chunk < n ? (int64_t)t * chunk : n;
+        int64_t last = first + chunk < n ? first + chunk : n;
+        gather_forms_chunk(degree, form, &g, pos, ps, field, result, rs, first, last);
     }
 }
 

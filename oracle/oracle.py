@@ -45,7 +45,7 @@ def _load() -> ctypes.CDLL:
     lib.orc_deposit_spline.argtypes = [_INT, _INT, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P]
     lib.orc_gather_edges_whitney.argtypes = [_INT, _P, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P]
     lib.orc_gather_edges_spline.argtypes = [_INT, _INT, _P, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P]
-    lib.orc_gather_forms.argtypes = [_INT, _INT, _INT, _P, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P]
+    lib.orc_gather_forms.argtypes = [_INT, _INT, _INT, _P, _I64, _I64, _P, _P, _I64, _P, _P, _P, _P, _INT]
     lib.orc_scatter_forms.argtypes = [_INT, _INT, _INT, _P, _I64, _P, _I64, _I64, _P, _P, _P, _P, _P, _INT]
     lib.orc_form_components.argtypes = [_INT, _INT]
     lib.orc_form_components.restype = _INT
@@ -174,7 +174,8 @@ def _form_field(field: np.ndarray, form: int, ndim: int) -> None:
         raise ValueError(f"a {form}-form in {ndim}D has {form_components(form, ndim)} components")
 
 
-def gather_forms(positions, field: np.ndarray, result: np.ndarray, *geometry, degree: int, form: int) -> None:
+def gather_forms(positions, field: np.ndarray, result: np.ndarray, *geometry, degree: int, form: int,
+                 threads: int = 1) -> None:
     """Overwrite ``result`` ``(ncomp, N)`` with the ``form``-form ``field`` (*extent, ncomp) at
     ``positions`` (gather_forms_spline_*d)."""
     ndim = len(geometry) // 4
@@ -185,7 +186,7 @@ def gather_forms(positions, field: np.ndarray, result: np.ndarray, *geometry, de
     rs = result.strides[0] // 8 if result.shape[0] > 1 else result.shape[1]
     lower, cell, num, code = _geometry(geometry, ndim)
     lib().orc_gather_forms(ndim, degree, form, _ptr(pos), ps, pos.shape[1], _ptr(field), _ptr(result), rs,
-                           _ptr(lower), _ptr(cell), _ptr(num), _ptr(code))
+                           _ptr(lower), _ptr(cell), _ptr(num), _ptr(code), threads)
 
 
 def scatter_forms(positions, values, field: np.ndarray, *geometry, degree: int, form: int, threads: int = 1) -> None:

@@ -108,6 +108,70 @@ def test_fixed_forms_deposit_is_reproducible_under_any_order(ndim, codes, form):
     assert rel_err(fields[0], expect) < 1e-10
 
 
+# colored launches (periodic axes of whole tiles, 2 or 3 colours) and atomic ones
+TILED = [(3, (1, 1, 1), (16, 16, 32)), (3, (1, 1, 1), (24, 16, 48)), (3, (0, 0, 0), (9, 17, 17)),
+         (3, (1, 0, 1), (16, 11, 21)), (2, (1, 1), (64, 40)), (2, (1, 1), (96, 64)), (2, (0, 0), (33, 45)),
+         (2, (0, 1), (31, 37))]
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+@pytest.mark.parametrize("ndim,codes,num", TILED)
+def test_tiled_forms_deposit_is_the_direct_fixed_deposit(degree, ndim, codes, num):
+    """The tiled deposit of binned markers (mpf_scatter_forms_binned) against the
+    one-marker-per-work-item fixed-point deposit: the same integers, so the same field bit
+    for bit; markers partly outside the grid (clamped onto the walls, where the edge stencil
+    reaches one node below the anchor), every 7th dead at a NaN position."""
+    from mpifield_amd import binning
+
+    rng = np.random.default_rng(degree + 10 * ndim + sum(num))
+    num = np.array(num)
+    h = np.array([0.1, 0.2, 0.05])[:ndim]
+    lower = np.array([0.5, -1.0, 0.0])[:ndim]
+    g = geom(lower, h, num, codes)
+    n = 40000
+    pos = markers(rng, lower, h, num, n)
+    for form in range(1, ndim + 1):
+        print(f"tiled forms p={degree} {ndim}D codes={codes} num={tuple(num)} form={form}", flush=True)
+        ncomp = K.form_components(form, ndim)
+        values = rng.normal(size=(ncomp, n))
+        values[:, ::7] = 0.0
+        p_ = pos.copy()
+        p_[:, ::7] = np.nan
+        direct = torch.zeros((*num, ncomp), dtype=torch.float64, device="cuda")
+        with K.deposit_mode("fixed"):
+            K.scatter_forms(degree, form, cuda(p_), cuda(values), direct, g)
+        bins = binning.bin_markers(cuda(p_), cuda(values), (_lib.KIND_SPLINE, degree), g)
+        tiled = torch.zeros_like(direct)
+        binning.scatter_forms_binned(bins, tiled, form)
+        assert torch.equal(tiled, direct)
+        expect = np.zeros((*num, ncomp))
+        orc.scatter_forms(p_, values, expect, *g, degree=degree, form=form)
+        assert rel_err(tiled.cpu().numpy(), expect) < 1e-10
+
+
+def test_large_fixed_forms_batches_take_the_tiled_deposit():
+    """Above BINNED_MIN_MARKERS a fixed-point forms deposit bins and tiles by itself: the
+    field equals the explicit binned call and the oracle within the fixed-point bar."""
+    from mpifield_amd import binning
+
+    rng = np.random.default_rng(21)
+    g, lower, h, num = (0.0, 0.0, 0.0, 1 / 32, 1 / 32, 1 / 32, 32, 32, 32, 1, 1, 1), np.zeros(3), np.full(3, 1 / 32), \
+        np.array([32, 32, 32])
+    n = K.BINNED_MIN_MARKERS + 1000
+    pos = rng.uniform(0.0, 1.0, (3, n)).astype(np.float32)
+    values = rng.normal(size=(3, n)).astype(np.float32)
+    auto = torch.zeros((32, 32, 32, 3), dtype=torch.float64, device="cuda")
+    with K.deposit_mode("fixed"):
+        K.scatter_forms(2, 2, cuda(pos), cuda(values), auto, g)
+    bins = binning.bin_markers(cuda(pos), cuda(values), (_lib.KIND_SPLINE, 2), g)
+    explicit = torch.zeros_like(auto)
+    binning.scatter_forms_binned(bins, explicit, 2)
+    assert torch.equal(auto, explicit)
+    expect = np.zeros((32, 32, 32, 3))
+    orc.scatter_forms(pos.astype(np.float64), values.astype(np.float64), expect, *g, degree=2, form=2, threads=4)
+    assert rel_err(auto.cpu().numpy(), expect) < 1e-10
+
+
 def test_faces_and_cells_through_the_weightings():
     """Splines.faces / Splines.cells (splines.py:185-204): the GPU deposit and gather on
     device tensors and NumPy arrays, cells summing to one inside and holding the whole

@@ -747,6 +747,10 @@ def test_forms_skip_dead_markers_and_thread_like_the_serial_deposit():
     orc.scatter_forms(points, values, serial, *g, degree=2, form=2)
     orc.scatter_forms(points, values, threaded, *g, degree=2, form=2, threads=4)
     assert np.isfinite(serial).all()
+    gathered = [np.zeros((3, 30000)), np.zeros((3, 30000))]
+    for out, threads in zip(gathered, (1, 4)):
+        orc.gather_forms(points, serial, out, *g, degree=2, form=2, threads=threads)
+    assert np.array_equal(gathered[0], gathered[1], equal_nan=True)
     np.testing.assert_allclose(threaded, serial, rtol=1e-12, atol=1e-12)
     with pytest.raises(ValueError, match="has 3 components"):
         orc.scatter_forms(points, values[:1], np.zeros((8, 8, 9, 1)), *g, degree=2, form=2)
