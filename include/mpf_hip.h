@@ -234,6 +234,13 @@ int mpf_scatter_forms_binned(int degree, int form, int ndim, const void *sorted_
                              int dtype, int64_t n, int64_t ncomp, const void *start, const double *scalars,
                              double *field, const double *lower, const double *cell_size, const int64_t *num_nodes,
                              const int64_t *code, mpf_context *ctx, mpf_stream stream);
+/* The forms gather of the same binned markers: result (ncomp, n) in bins order, the values
+ * of gather_forms_spline_*d at those markers bit for bit (the tile's field footprint staged
+ * in LDS; markers in the dead bin gathered one per work-item). */
+int mpf_gather_forms_binned(int degree, int form, int ndim, const void *sorted_positions, int dtype, int64_t n,
+                            const void *start, const double *field, int64_t ncomp, void *result, const double *lower,
+                            const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
+                            mpf_stream stream);
 
 /* ---- halos, gradient, divergence -------------------------------------------------- */
 

@@ -77,6 +77,7 @@ SIGNATURES: dict[str, list] = {
                           _P],
     "mpf_gather_forms": [_INT, _INT, _INT, _P, _I64, _INT, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P],
     "mpf_scatter_forms_binned": [_INT, _INT, _INT, _P, _P, _INT, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "mpf_gather_forms_binned": [_INT, _INT, _INT, _P, _INT, _I64, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P],
 }
 for _d in (1, 2, 3):
     for _kind in ("linear", "quadratic"):

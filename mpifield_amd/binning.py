@@ -13,6 +13,7 @@ The forms deposit (faces, cells) reads bins of the spline nodes of its degree::
 
     bins = bin_markers(positions, values, (KIND_SPLINE, degree), grid)
     scatter_forms_binned(bins, faces, form=2)
+    gather_forms_binned(bins, faces, result, form=2)   # result (ncomp, N) in bins order
 
 ``bins.order[i]`` is the input index of binned marker ``i``. `Weighting.scatter_block` /
 `gather_block` take this path by themselves for large batches (binning inside the call).
@@ -223,3 +224,20 @@ def scatter_forms_binned(bins: MarkerBins, field: torch.Tensor, form: int) -> No
                   bins.values.data_ptr(), K._dtype_code(bins.positions), bins.n, bins.ncomp, bins.start.data_ptr(),
                   bins.scalars.data_ptr(), field.data_ptr(), lower.ctypes.data, cell.ctypes.data, num.ctypes.data,
                   code.ctypes.data, None, K.stream_handle(field.device))
+
+
+def gather_forms_binned(bins: MarkerBins, field: torch.Tensor, result: torch.Tensor, form: int) -> None:
+    """The ``form``-form ``field`` at the binned markers into ``result`` (ncomp, N) in bins order
+    (mpf_gather_forms_binned): gather_forms_spline_*d's values bit for bit."""
+    if bins.kind != _lib.KIND_SPLINE:
+        raise ValueError("the forms gather reads bins of the spline nodes (kind 2)")
+    field = K._field(field)
+    ndim, lower, cell, num, code = _geometry_arrays(bins.geometry)
+    ncomp = field.shape[-1]
+    if tuple(result.shape) != (ncomp, bins.n) or not result.is_contiguous() or result.dtype != bins.positions.dtype:
+        raise ValueError(f"result must be a contiguous ({ncomp}, {bins.n}) {bins.positions.dtype} tensor")
+    with torch.cuda.device(field.device):
+        _lib.call("mpf_gather_forms_binned", bins.degree, form, ndim, bins.positions.data_ptr(),
+                  K._dtype_code(bins.positions), bins.n, bins.start.data_ptr(), field.data_ptr(), ncomp,
+                  result.data_ptr(), lower.ctypes.data, cell.ctypes.data, num.ctypes.data, code.ctypes.data, None,
+                  K.stream_handle(field.device))

@@ -334,6 +334,14 @@ def gather_forms(degree: int, form: int, positions, field: torch.Tensor, result:
     result2 = result.reshape(1, -1) if result.dim() == 1 else result
     if tuple(result2.shape) != (ncomp, pos.shape[1]):
         raise ValueError(f"result must be ({ncomp}, {pos.shape[1]}), got {tuple(result.shape)}")
+    if BINNED_MIN_MARKERS <= pos.shape[1] < 1 << 31 and forms_tiled_ok(degree, num, code):
+        from mpifield_amd import binning
+
+        bins = binning.bin_markers(pos, None, (_lib.KIND_SPLINE, degree), geometry)
+        sorted_out = torch.empty((ncomp, pos.shape[1]), dtype=pos.dtype, device=field.device)
+        binning.gather_forms_binned(bins, field, sorted_out, form)
+        result2.copy_(bins.unsort(sorted_out))
+        return
     direct = result2.dtype == pos.dtype and (result2.shape[1] <= 1 or result2.stride(1) == 1)
     out = result2 if direct else torch.empty((ncomp, pos.shape[1]), dtype=pos.dtype, device=field.device)
     rs = out.stride(0) if out.shape[0] > 1 else out.shape[1]

@@ -116,11 +116,13 @@ TILED = [(3, (1, 1, 1), (16, 16, 32)), (3, (1, 1, 1), (24, 16, 48)), (3, (0, 0, 
 
 @pytest.mark.parametrize("degree", [1, 2, 3])
 @pytest.mark.parametrize("ndim,codes,num", TILED)
-def test_tiled_forms_deposit_is_the_direct_fixed_deposit(degree, ndim, codes, num):
+def test_tiled_forms_match_the_direct_kernels(degree, ndim, codes, num):
     """The tiled deposit of binned markers (mpf_scatter_forms_binned) against the
     one-marker-per-work-item fixed-point deposit: the same integers, so the same field bit
-    for bit; markers partly outside the grid (clamped onto the walls, where the edge stencil
-    reaches one node below the anchor), every 7th dead at a NaN position."""
+    for bit; the tiled gather (mpf_gather_forms_binned) against the oracle, bit for bit.
+    Markers partly outside the grid (clamped onto the walls, where the edge stencil reaches
+    one node below the anchor); every 7th dead (in the dead bin, gathered one per
+    work-item), every 14th of them at a NaN position."""
     from mpifield_amd import binning
 
     rng = np.random.default_rng(degree + 10 * ndim + sum(num))
@@ -136,7 +138,7 @@ def test_tiled_forms_deposit_is_the_direct_fixed_deposit(degree, ndim, codes, nu
         values = rng.normal(size=(ncomp, n))
         values[:, ::7] = 0.0
         p_ = pos.copy()
-        p_[:, ::7] = np.nan
+        p_[:, ::14] = np.nan
         direct = torch.zeros((*num, ncomp), dtype=torch.float64, device="cuda")
         with K.deposit_mode("fixed"):
             K.scatter_forms(degree, form, cuda(p_), cuda(values), direct, g)
@@ -144,6 +146,15 @@ def test_tiled_forms_deposit_is_the_direct_fixed_deposit(degree, ndim, codes, nu
         tiled = torch.zeros_like(direct)
         binning.scatter_forms_binned(bins, tiled, form)
         assert torch.equal(tiled, direct)
+        # the tiled gather of the same bins (dead markers included) in bins order
+        field = rng.normal(size=(*num, ncomp))
+        sorted_out = torch.empty((ncomp, n), dtype=torch.float64, device="cuda")
+        binning.gather_forms_binned(bins, cuda(field), sorted_out, form)
+        live = ~np.isnan(p_[0])  # the oracle's integer cast of a NaN position is undefined
+        want = np.zeros((ncomp, int(live.sum())))
+        orc.gather_forms(p_[:, live], field, want, *g, degree=degree, form=form)
+        got = bins.unsort(sorted_out).cpu().numpy()
+        assert np.array_equal(got[:, live], want)
         expect = np.zeros((*num, ncomp))
         orc.scatter_forms(p_, values, expect, *g, degree=degree, form=form)
         assert rel_err(tiled.cpu().numpy(), expect) < 1e-10
@@ -170,6 +181,12 @@ def test_large_fixed_forms_batches_take_the_tiled_deposit():
     expect = np.zeros((32, 32, 32, 3))
     orc.scatter_forms(pos.astype(np.float64), values.astype(np.float64), expect, *g, degree=2, form=2, threads=4)
     assert rel_err(auto.cpu().numpy(), expect) < 1e-10
+    # the gather bins, tiles and unsorts by itself: the direct gather's values in input order
+    result = torch.empty((3, n), dtype=torch.float32, device="cuda")
+    K.gather_forms(2, 2, cuda(pos), auto, result, g)
+    want = np.zeros((3, n))
+    orc.gather_forms(pos.astype(np.float64), auto.cpu().numpy(), want, *g, degree=2, form=2, threads=4)
+    assert np.array_equal(result.cpu().numpy(), want.astype(np.float32))
 
 
 def test_faces_and_cells_through_the_weightings():
