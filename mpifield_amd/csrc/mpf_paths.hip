@@ -527,6 +527,8 @@ __device__ __forceinline__ void edge_axis(double rel, const AxisTile &ax, int *s
   }
 }
 
+// One path's spline-edge contributions into the tile (rounded with to_fixed_small, as the
+// Whitney sink's: the automatic scale keeps them below 2^49 quanta).
 template <int NDIM, int DEG>
 __device__ __forceinline__ void path_into_tile(const double *r0, const double *r1, double charge, const Grid &g,
                                                const AxisTile *ax, lds_u64 *lds,
@@ -637,7 +639,7 @@ __device__ __forceinline__ void path_into_tile(const double *r0, const double *r
                 if (se == -2) continue;
                 const double v = NDIM < 3 ? vb : vb * (axis == 2 ? we[NDIM - 1][e] : wn[NDIM - 1][e]);
                 const int ie = NDIM < 3 ? 0 : (axis == 2 ? en[NDIM - 1][e] : nn[NDIM - 1][e]);
-                const unsigned long long f = to_fixed(v * scale);
+                const unsigned long long f = to_fixed_small(v * scale);
                 if (sa >= 0 && sb >= 0 && se >= 0) {
                   __hip_atomic_fetch_add(lds + ((sa * PT::LY + sb) * PT::LZ + se) * NDIM + axis, f,
                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -658,7 +660,9 @@ __device__ __forceinline__ void path_into_tile(const double *r0, const double *r
 
 // Whitney contributions into a tile's LDS accumulator: the wrapped node index of each axis
 // folded into the footprint (one +-num step), a contribution outside it to the global
-// accumulator.
+// accumulator.  The tiles run under the automatic scale only, which keeps every
+// contribution below 2^49 quanta (|q| times the path length bounds it), so the rounding is
+// to_fixed_small's.
 template <int NDIM>
 struct WhitneyTileSink {
   lds_u64 *lds;
@@ -677,7 +681,7 @@ struct WhitneyTileSink {
       in = in && k >= 0 && k < ax[d].len;
       l[d] = k;
     }
-    const unsigned long long f = to_fixed(v * scale);
+    const unsigned long long f = to_fixed_small(v * scale);
     if (in) {
       __hip_atomic_fetch_add(lds + ((l[0] * PT::LY + l[1]) * PT::LZ + l[2]) * NDIM + axis, f, __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1018,7 +1022,9 @@ extern "C" int mpf_deposit_paths(int kind, int degree, int ndim, const void *sta
   unsigned long long *accum = nullptr;
   const double *slot = nullptr;
   // large spline-edge deposits in fixed point: binned by start cell, tiled (the same sums)
-  bool tiled = mode == MODE_FIXED && (ndim == 2 || ndim == 3) && n >= PATHS_TILED_MIN;
+  // (with the automatic scale, under which every contribution stays below 2^49 quanta and
+  // the tiles round it with to_fixed_small)
+  bool tiled = mode == MODE_FIXED && (ndim == 2 || ndim == 3) && n >= PATHS_TILED_MIN && scale_exp == MPF_SCALE_AUTO;
   for (int d = 0; d < ndim; ++d) tiled = tiled && code[d] != 2;
   void *spos = nullptr, *srows = nullptr;
   uint32_t *bins = nullptr;
