@@ -547,7 +547,7 @@ def main() -> None:
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path runs on one MI355X; "
-              "DESIGN.md describes how it would shard)", file=sys.stderr)
+              "the split over GPUs by spatial blocks with halo folds is not built)", file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -211,18 +211,19 @@ __global__ void __launch_bounds__(256) scatter_forms_kernel(const T *__restrict_
 // one-marker-per-work-item fixed-point deposit, so the field is the same bit for bit --
 // then the tile's nonzero words into the global int64 accumulator.  Along each axis a
 // marker's nodes are anchor - DEG/2 .. anchor - DEG/2 + DEG and its staggered k-cells lie
-// in anchor - DEG/2 - 1 .. anchor + 1 (one lower where a bounded wall clamps the edge
-// stencil), so the footprint is the tile plus DEG/2 + 1 below and DEG - DEG/2 + 1 above.
-// A periodic axis folds the footprint (it needs num >= the footprint); a bounded one
-// clamps.  The tiles add their words with integer atomics (exact, so the field does not
+// among them, so the footprint is the tile plus DEG/2 below and DEG - DEG/2 above; the one
+// exception, an edge stencil clamped at a bounded wall that reaches one node lower, leaves
+// the footprint only in a tile that starts at the wall's last node, and goes to the global
+// accumulator.  A periodic axis folds the footprint (it needs num >= the footprint); a
+// bounded one clamps.  The tiles add their words with integer atomics (exact, so the field does not
 // depend on the order); running overlapping tiles in separate launches without atomics
 // measured slower (the launches' tails cost more than the atomics).
 template <int NDIM, int DEG>
 struct FormTile {
   static constexpr int TX = NDIM == 3 ? TILE_X : TILE2, TY = NDIM == 3 ? TILE_Y : TILE2, TZ = NDIM == 3 ? TILE_Z : 1;
   static constexpr int TC = TX * TY * TZ;
-  static constexpr int LO = DEG / 2 + 1;             // footprint nodes below the tile
-  static constexpr int SPAN = DEG + 2;               // footprint nodes beyond the tile's own
+  static constexpr int LO = DEG / 2;                 // footprint nodes below the tile
+  static constexpr int SPAN = DEG;                   // footprint nodes beyond the tile's own
   static constexpr int LX = TX + SPAN, LY = TY + SPAN, LZ = NDIM == 3 ? TZ + SPAN : 1;
 };
 
@@ -506,7 +507,7 @@ static int launch_gather_forms_tiled(int form, int degree, const void *pos, int6
 // axis at least as long as the footprint.
 static bool forms_tiled_supported(int degree, int ndim, const Grid &g) {
   if (ndim != 2 && ndim != 3) return false;
-  const int span = degree + 2;
+  const int span = degree;
   const int tile[3] = {ndim == 3 ? TILE_X : TILE2, ndim == 3 ? TILE_Y : TILE2, TILE_Z};
   for (int d = 0; d < ndim; ++d) {
     if (g.code[d] == 2) return false;
@@ -706,7 +707,7 @@ extern "C" int mpf_scatter_forms_binned(int degree, int form, int ndim, const vo
   if (rc) return rc;
   if (!forms_tiled_supported(degree, ndim, g)) {
     set_error("the tiled forms deposit takes 2D and 3D grids without clamped axes whose periodic axes hold at "
-              "least a tile and %d nodes", degree + 2);
+              "least a tile and %d nodes", degree);
     return MPF_ERR_UNSUPPORTED;
   }
   if (n == 0) return MPF_OK;
@@ -750,7 +751,7 @@ extern "C" int mpf_gather_forms_binned(int degree, int form, int ndim, const voi
   if (rc) return rc;
   if (!forms_tiled_supported(degree, ndim, g)) {
     set_error("the tiled forms gather takes 2D and 3D grids without clamped axes whose periodic axes hold at "
-              "least a tile and %d nodes", degree + 2);
+              "least a tile and %d nodes", degree);
     return MPF_ERR_UNSUPPORTED;
   }
   if (n == 0) return MPF_OK;

@@ -316,11 +316,11 @@ def scatter_forms(degree: int, form: int, positions, values, field: torch.Tensor
 
 def forms_tiled_ok(degree: int, num, code) -> bool:
     """Whether the tiled forms deposit takes this grid (mpf_forms.hip's forms_tiled_supported):
-    2D or 3D, no clamped axis, every periodic axis at least a tile and degree + 2 nodes long."""
+    2D or 3D, no clamped axis, every periodic axis at least a tile and degree nodes long."""
     if len(num) not in (2, 3):
         return False
     tiles = (_TILE if len(num) == 3 else (32, 32))
-    return all(int(c) != 2 and (int(c) != 1 or int(n) >= t + degree + 2) for n, c, t in zip(num, code, tiles))
+    return all(int(c) != 2 and (int(c) != 1 or int(n) >= t + degree) for n, c, t in zip(num, code, tiles))
 
 
 def gather_forms(degree: int, form: int, positions, field: torch.Tensor, result: torch.Tensor, geometry) -> None:
