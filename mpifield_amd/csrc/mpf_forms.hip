@@ -98,16 +98,19 @@ __device__ __forceinline__ void for_each_cell(const FormStencil<NDIM, FORM, DEG>
 }
 
 // The position in cells, clamped to [0, num - 1] on a non-periodic axis
-// (gather_forms_spline_2d_pyccel.py:52-63).  A non-finite one, for which the reference's
-// integer cast is undefined, is taken as 0 (the cell the binning gives it), so that every
-// index stays inside the field.
+// (gather_forms_spline_2d_pyccel.py:52-63; an infinite position lands on the wall).  A NaN,
+// or a non-finite position on a periodic axis, for which the reference's integer cast is
+// undefined, is taken as 0 (the cell the binning gives it), so that every index stays
+// inside the field.
 template <int NDIM>
 __device__ __forceinline__ void form_relative(const double *x, const Grid &g, double *rel) {
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
     double r = (x[d] - g.lower[d]) * g.inverse[d];
-    if (!isfinite(r)) r = 0.0;
-    if (g.code[d] != 1) {
+    if (g.code[d] == 1) {
+      if (!isfinite(r)) r = 0.0;
+    } else {
+      if (r != r) r = 0.0;
       if (r < 0.0) r = 0.0;
       if (r > (double)(g.num[d] - 1)) r = (double)g.num[d] - 1.0;
     }

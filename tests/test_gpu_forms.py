@@ -189,6 +189,43 @@ def test_large_fixed_forms_batches_take_the_tiled_deposit():
     assert np.array_equal(result.cpu().numpy(), want.astype(np.float32))
 
 
+def test_infinite_positions_land_on_the_walls_of_bounded_axes():
+    """On a bounded or clamped axis the reference clamps the position, infinities included
+    (gather_forms_spline_2d_pyccel.py:52-63): the direct and tiled kernels agree with the
+    oracle there."""
+    from mpifield_amd import binning
+
+    rng = np.random.default_rng(31)
+    for codes, tiled in (((0, 0, 0), True), ((0, 0, 1), False), ((2, 0, 2), False)):
+        g, lower, h, num, extent = grid_case(3, codes, 2)
+        n = 3000
+        pos = markers(rng, lower, h, num, n)
+        pos[0, ::5] = np.inf
+        pos[1, 1::5] = -np.inf
+        ncomp = 3
+        field = rng.normal(size=(*extent, ncomp))
+        want = np.zeros((ncomp, n))
+        orc.gather_forms(pos, field, want, *g, degree=2, form=2)
+        got = torch.empty((ncomp, n), dtype=torch.float64, device="cuda")
+        K.gather_forms(2, 2, cuda(pos), cuda(field), got, g)
+        assert np.array_equal(got.cpu().numpy(), want)
+        values = rng.normal(size=(ncomp, n))
+        expect = np.zeros((*extent, ncomp))
+        orc.scatter_forms(pos, values, expect, *g, degree=2, form=2)
+        out = torch.zeros((*extent, ncomp), dtype=torch.float64, device="cuda")
+        with K.deposit_mode("atomic"):
+            K.scatter_forms(2, 2, cuda(pos), cuda(values), out, g)
+        assert rel_err(out.cpu().numpy(), expect) < 1e-12
+        if tiled:
+            bins = binning.bin_markers(cuda(pos), cuda(values), (_lib.KIND_SPLINE, 2), g)
+            tiled_out = torch.zeros_like(out)
+            binning.scatter_forms_binned(bins, tiled_out, 2)
+            assert rel_err(tiled_out.cpu().numpy(), expect) < 1e-10
+            sorted_out = torch.empty((ncomp, n), dtype=torch.float64, device="cuda")
+            binning.gather_forms_binned(bins, cuda(field), sorted_out, 2)
+            assert np.array_equal(bins.unsort(sorted_out).cpu().numpy(), want)
+
+
 def test_faces_and_cells_through_the_weightings():
     """Splines.faces / Splines.cells (splines.py:185-204): the GPU deposit and gather on
     device tensors and NumPy arrays, cells summing to one inside and holding the whole
