@@ -51,12 +51,13 @@ def parse() -> argparse.Namespace:
     p.add_argument("--only-headline", action="store_true", help="time the headline step only (for rocprofv3)")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only copy MIB MiB device-to-device (the counter calibration for rocprofv3 --pmc)")
-    p.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6, 7, 8, 9),
+    p.add_argument("--config", type=int, default=2, choices=(2, 3, 4, 5, 6, 7, 8, 9, 10),
                    help="2: the headline (default); 3: 2D quadratic-edge path deposit, 1024^2, 2^25 paths; "
                    "4: 4096 batched 2D patches, ~2e8 markers; 5: differentiable 3D cubic gather of scatter, "
                    "128^3, 2^24 markers; 6: 2D Whitney path deposit, same paths as 3; 7: halo fold + fill, "
                    "256^3 x 3, halo 2; 8: 2D quadratic-edge field at 2^25 markers; 9: 3D quadratic faces (2-forms) "
-                   "scatter+gather, 256^3 x 3, 2^25 markers (one JSON line each)")
+                   "scatter+gather, 256^3 x 3, 2^25 markers; 10: 3D quadratic-edge path deposit, 128^3, 2^23 paths "
+                   "(one JSON line each)")
     return p.parse_args()
 
 
@@ -227,6 +228,57 @@ def config3(torch, args) -> dict:
                      "frac": alg / (out["fixed"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
         "variants": {"fp64_atomic_ms": out["atomic"], "fp64_atomic_paths_per_s": n / (out["atomic"] * 1e-3)},
         "continuity_max_abs_residual": residual, "total_charge": float(n), "cpu_baseline": cpu,
+    }
+
+
+def config10(torch, args) -> dict:
+    """3D charge-conserving path deposit (QUADRATIC_EDGES, spline degree 2) on a 128^3 periodic
+    grid: 2^23 float32 paths, start uniform, per-axis displacement clip(N(0, h/4), +-h/2),
+    unit charges.  The deposit time and the continuity residual on the GPU."""
+    from mpifield_amd import _lib
+    from mpifield_amd import kernels as K
+
+    nodes, n = 128, 1 << 23
+    h = 1.0 / nodes
+    g = (0.0, 0.0, 0.0, h, h, h, nodes, nodes, nodes, 1, 1, 1)
+    gen = torch.Generator(device="cuda").manual_seed(10)
+    start = torch.rand((3, n), generator=gen, device="cuda", dtype=torch.float32)
+    d = (torch.randn((3, n), generator=gen, device="cuda") * (0.25 * h)).clamp(-0.5 * h, 0.5 * h)
+    end = (start.double() + d.double()).float()
+    q = torch.ones(n, device="cuda", dtype=torch.float32)
+    j = torch.zeros((nodes, nodes, nodes, 3), dtype=torch.float64, device="cuda")
+    out = {}
+    for label, mode in (("fixed", _lib.MODE_FIXED), ("atomic", _lib.MODE_ATOMIC)):
+        out[label] = time_events(torch, lambda: K.deposit_paths(1, 2, start, end, q, j, g, mode=mode), args.steps, 1)
+    j.zero_()
+    K.deposit_paths(1, 2, start, end, q, j, g, mode=_lib.MODE_FIXED)
+    rho0 = torch.zeros((nodes, nodes, nodes, 1), dtype=torch.float64, device="cuda")
+    rho1 = torch.zeros_like(rho0)
+    K.scatter(2, 2, start, q[None], rho0, g, mode=_lib.MODE_FIXED)
+    K.scatter(2, 2, end, q[None], rho1, g, mode=_lib.MODE_FIXED)
+    residual = (K.divergence(j, (1, 1, 1)) - (rho1 - rho0)[..., 0]).abs().max().item()
+    alg = n * (24 + 4) + 2 * nodes**3 * 3 * 8
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        sample = 1 << 20
+        s_h, e_h = start[:, :sample].double().cpu().numpy(), end[:, :sample].double().cpu().numpy()
+        q_h = q[:sample].double().cpu().numpy()
+        edges = np.zeros((nodes, nodes, nodes, 3))
+        sec = cpu_seconds(lambda: orc.deposit_spline(s_h, e_h, q_h, edges, *g, degree=2))
+        cpu = {"value": sample / sec, "unit": "paths/s", "cores": 1, "kind": "port",
+               "sample": f"first {sample} paths: {sec:.2f} s (the reference's spline deposit is serial)"}
+    return {
+        "metric": "paths/s, 3D QUADRATIC_EDGES path deposit 128^3, 2^23 paths", "value": n / (out["fixed"] * 1e-3),
+        "unit": "paths/s", "n_gpus": 1, "steps": args.steps, "ms_per_step": out["fixed"], "higher_is_better": True,
+        "dtype": "int64", "data": "synthetic",
+        "config": {"workload": "3D spline-edge (degree 2) charge-conserving path deposit, 128^3 periodic, "
+                   f"{n} float32 paths, displacements clip(N(0,h/4), h/2); binned by start cell, tiled"},
+        "roofline": {"bound": "hbm", "achieved": alg / (out["fixed"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / (out["fixed"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "variants": {"fp64_atomic_ms": out["atomic"]},
+        "continuity_max_abs_residual": residual, "cpu_baseline": cpu,
     }
 
 
@@ -561,7 +613,8 @@ def main() -> None:
         return
 
     if args.config != 2:
-        line = {3: config3, 4: config4, 5: config5, 6: config6, 7: config7, 8: config8, 9: config9}[args.config](
+        line = {3: config3, 4: config4, 5: config5, 6: config6, 7: config7, 8: config8, 9: config9,
+                10: config10}[args.config](
             torch, args)
         print(json.dumps(line), flush=True)
         return
