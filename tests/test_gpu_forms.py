@@ -292,3 +292,31 @@ def test_forms_reject_bad_forms_and_component_counts():
         _lib.call("mpf_gather_forms", 0, 2, 2, pos.data_ptr(), 4, _lib.F64, 4, field.data_ptr(), 1, out.data_ptr(), 4,
                   lower_.astype(np.float64).ctypes.data, cell_.astype(np.float64).ctypes.data,
                   num_.astype(np.int64).ctypes.data, code_.astype(np.int64).ctypes.data, None, K.stream_handle())
+
+
+def test_full_size_forms_conserve_the_total_and_are_transposes():
+    """At the benchmark's size (256^3 periodic, 2^24 float32 markers, tiled): the cells of
+    degree 2 hold the whole deposit (the staggered B-splines sum to one), and the faces
+    deposit is the transpose of the faces gather (sum gathered * v = sum field * deposit)."""
+    nodes, n = 256, 1 << 24
+    h = 1.0 / nodes
+    g = (0.0, 0.0, 0.0, h, h, h, nodes, nodes, nodes, 1, 1, 1)
+    gen = torch.Generator(device="cuda").manual_seed(41)
+    pos = torch.rand((3, n), generator=gen, device="cuda", dtype=torch.float32)
+    print("full-size cells and faces, 256^3, 2^24 markers", flush=True)
+    v = (1.0 + 0.1 * torch.randn((1, n), generator=gen, device="cuda")).float()
+    cells = torch.zeros((nodes, nodes, nodes, 1), dtype=torch.float64, device="cuda")
+    with K.deposit_mode("fixed"):
+        K.scatter_forms(2, 3, pos, v, cells, g)
+    total = float(v.double().sum())
+    assert abs(float(cells.sum()) - total) <= 1e-9 * total
+    values = torch.randn((3, n), generator=gen, device="cuda").float()
+    faces = torch.zeros((nodes, nodes, nodes, 3), dtype=torch.float64, device="cuda")
+    with K.deposit_mode("fixed"):
+        K.scatter_forms(2, 2, pos, values, faces, g)
+    field = torch.randn((nodes, nodes, nodes, 3), generator=gen, device="cuda", dtype=torch.float64)
+    gathered = torch.empty((3, n), dtype=torch.float64, device="cuda")
+    K.gather_forms(2, 2, pos.double(), field, gathered, g)
+    lhs = float((gathered * values.double()).sum())
+    rhs = float((field * faces).sum())
+    assert abs(lhs - rhs) <= 1e-9 * float((gathered.abs() * values.double().abs()).sum())
