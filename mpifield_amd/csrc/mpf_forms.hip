@@ -218,9 +218,9 @@ __global__ void __launch_bounds__(256) scatter_forms_kernel(const T *__restrict_
 // exception, an edge stencil clamped at a bounded wall that reaches one node lower, leaves
 // the footprint only in a tile that starts at the wall's last node, and goes to the global
 // accumulator.  A periodic axis folds the footprint (it needs num >= the footprint); a
-// bounded one clamps.  The tiles add their words with integer atomics (exact, so the field does not
-// depend on the order); running overlapping tiles in separate launches without atomics
-// measured slower (the launches' tails cost more than the atomics).
+// bounded one clamps.  The tiles add their words with integer atomics (exact, so the field
+// does not depend on the order); running overlapping tiles in separate launches without
+// atomics measured slower (the launches' tails cost more than the atomics).
 template <int NDIM, int DEG>
 struct FormTile {
   static constexpr int TX = NDIM == 3 ? TILE_X : TILE2, TY = NDIM == 3 ? TILE_Y : TILE2, TZ = NDIM == 3 ? TILE_Z : 1;
