@@ -103,6 +103,30 @@ def test_clamped_axes_match_oracle(degree, ndim, mode):
     assert rel_err(got_g.cpu().numpy(), want_g) < 1e-12
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_2d_linear_cloud_in_cell_on_101_by_51_nodes(mode):
+    """The reference's plain CPU case: LINEAR (cloud-in-cell) scatter then gather on a 101 x 51
+    node grid over [0, 0.1] x [0, 0.05], 10,000 seeded uniform float64 markers, unit weights
+    (1, N), one bounded block -- through the catalogue kernels, against the oracle."""
+    rng = np.random.default_rng(0)
+    n = 10_000
+    pos = np.stack([rng.uniform(0.0, 0.1, n), rng.uniform(0.0, 0.05, n)])
+    g = geom([0.0, 0.0], [0.1 / 100, 0.05 / 50], [101, 51], [0, 0])
+    weights = np.ones((1, n))
+    want = np.zeros((101, 51, 1))
+    orc.scatter("linear", pos, weights, want, *g)
+    field = torch.zeros((101, 51, 1), dtype=torch.float64, device="cuda")
+    with K.deposit_mode(mode):
+        K.scatter_linear_2d(cuda(pos), cuda(weights), field, *g)
+    assert rel_err(field.cpu().numpy(), want) < BAR[mode]
+    assert float(field.sum()) == pytest.approx(n, rel=1e-12)
+    back = np.zeros((1, n))
+    orc.gather("linear", pos, want, back, *g)
+    got = torch.empty((1, n), dtype=torch.float64, device="cuda")
+    K.gather_linear_2d(cuda(pos), cuda(want), got, *g)
+    assert rel_err(got.cpu().numpy(), back) < 1e-12
+
+
 def test_strided_rows_and_float32_markers():
     """Rows with gaps (storage[:, :n], weighting.py:33-34) and float32 markers."""
     g, pts, vals, field = case(3, (1, 0, 1), seed=3, n=5000, ncomp=3)
