@@ -1370,60 +1370,65 @@ __global__ void __launch_bounds__(256) scatter_tiled_2d(const T *__restrict__ po
   const uint32_t first = start[(int64_t)tile * TILE2_CELLS], last = start[(int64_t)(tile + 1) * TILE2_CELLS];
   const double scale = scalars[0];
   __syncthreads();
-  const uint32_t chunk = (last - first + BLOCK - 1) / BLOCK;
-  const uint32_t lo = first + t * chunk, hi = lo + chunk < last ? lo + chunk : last;
-  // a lane's run of markers (a tile holds tens per cell, so neighbouring lanes stay on
-  // different cells) taken four at a time: the four markers' rows are loaded together, so
-  // the run waits on one memory latency per four markers, not per marker
+  // runs of about one cell's markers (the tile's mean per cell; the last run clipped at the
+  // tile's end), lane t taking run t of each pass of BLOCK runs: the lanes of a wavefront
+  // walk neighbouring cells, whose nodes are neighbouring LDS words (component-major), so
+  // their atomics take distinct banks.  A run is taken four markers at a time, their rows
+  // loaded together (one memory latency per four markers).
+  const uint32_t per_cell = (last - first) / TILE2_CELLS;
+  const uint32_t chunk = per_cell > 0 ? per_cell : 1;
   constexpr int BATCH = 4;
-  for (uint32_t m0 = lo; m0 < hi; m0 += BATCH) {
-    double px[BATCH], py[BATCH], vs[BATCH][C];
+  for (uint32_t lo = first + t * chunk; lo < last; lo += BLOCK * chunk) {
+    const uint32_t hi = lo + chunk < last ? lo + chunk : last;
+    for (uint32_t m0 = lo; m0 < hi; m0 += BATCH) {
+      double px[BATCH], py[BATCH], vs[BATCH][C];
 #pragma unroll
-    for (int j = 0; j < BATCH; ++j) {
-      const uint32_t m = m0 + j < hi ? m0 + j : hi - 1;  // in range: repeats the last marker
-      px[j] = (double)pos[m];
-      py[j] = (double)pos[n + m];
+      for (int j = 0; j < BATCH; ++j) {
+        const uint32_t m = m0 + j < hi ? m0 + j : hi - 1;  // in range: repeats the last marker
+        px[j] = (double)pos[m];
+        py[j] = (double)pos[n + m];
 #pragma unroll
-      for (int c = 0; c < C; ++c) vs[j][c] = (double)val[c * n + m] * scale;
-    }
+        for (int c = 0; c < C; ++c) vs[j][c] = (double)val[c * n + m] * scale;
+      }
 #pragma unroll
-    for (int j = 0; j < BATCH; ++j) {
-      if (m0 + j >= hi) break;
-      double wx[4], wy[4];
-      const int ax = anchor_axis<KIND, P>((px[j] - g.lower[0]) * g.inverse[0], num0, g.code[0], wx);
-      const int ay = anchor_axis<KIND, P>((py[j] - g.lower[1]) * g.inverse[1], num1, g.code[1], wy);
-      // value * 2^k: a power-of-two scale, so the integers are round(w * v) * 2^k as in
-      // the direct deposit; below 2^50 quanta they take the two-operation rounding
-      double vmax = 0.0;
+      for (int j = 0; j < BATCH; ++j) {
+        if (m0 + j >= hi) break;
+        double wx[4], wy[4];
+        const int ax = anchor_axis<KIND, P>((px[j] - g.lower[0]) * g.inverse[0], num0, g.code[0], wx);
+        const int ay = anchor_axis<KIND, P>((py[j] - g.lower[1]) * g.inverse[1], num1, g.code[1], wy);
+        // value * 2^k: a power-of-two scale, so the integers are round(w * v) * 2^k as in
+        // the direct deposit; below 2^50 quanta they take the two-operation rounding
+        double vmax = 0.0;
 #pragma unroll
-      for (int c = 0; c < C; ++c) vmax = fmax(vmax, fabs(vs[j][c]));
-      auto deposit = [&](auto fast) {
+        for (int c = 0; c < C; ++c) vmax = fmax(vmax, fabs(vs[j][c]));
+        auto deposit = [&](auto fast) {
 #pragma unroll
-        for (int a = 0; a < S; ++a) {
-          const int sx = slot_2d(ax - OFF + a, bx, num0, g.code[0]);
+          for (int a = 0; a < S; ++a) {
+            const int sx = slot_2d(ax - OFF + a, bx, num0, g.code[0]);
 #pragma unroll
-          for (int b = 0; b < S; ++b) {
-            const int sy = slot_2d(ay - OFF + b, by, num1, g.code[1]);
-            const double weight = wx[a] * wy[b];
+            for (int b = 0; b < S; ++b) {
+              const int sy = slot_2d(ay - OFF + b, by, num1, g.code[1]);
+              const double weight = wx[a] * wy[b];
 #pragma unroll
-            for (int c = 0; c < C; ++c) {
-              const double y = weight * vs[j][c];
-              atomicAdd(acc + (sx * L + sy) * C + c, decltype(fast)::value ? to_fixed_small(y) : to_fixed(y));
+              for (int c = 0; c < C; ++c) {
+                const double y = weight * vs[j][c];
+                atomicAdd(acc + c * (L * L) + sx * L + sy, decltype(fast)::value ? to_fixed_small(y) : to_fixed(y));
+              }
             }
           }
-        }
-      };
-      if (vmax < 0x1p50)
-        deposit(std::true_type{});
-      else
-        deposit(std::false_type{});
+        };
+        if (vmax < 0x1p50)
+          deposit(std::true_type{});
+        else
+          deposit(std::false_type{});
+      }
     }
   }
   __syncthreads();
   for (int i = t; i < L * L * C; i += BLOCK) {
     const unsigned long long q = acc[i];
     if (q == 0ull) continue;
-    const int c = i % C, sl = i / C, sx = sl / L, sy = sl % L;
+    const int c = i / (L * L), sl = i - c * (L * L), sx = sl / L, sy = sl % L;
     int gx = bx + sx, gy = by + sy;
     if (g.code[0] == 1) gx = gx < 0 ? gx + num0 : (gx >= num0 ? gx - num0 : gx);
     if (g.code[1] == 1) gy = gy < 0 ? gy + num1 : (gy >= num1 ? gy - num1 : gy);
