@@ -334,6 +334,13 @@ int mpf_gather_derivative_binned(int kind, int degree, int ndim, const void *sor
 int mpf_fixed_scale(const void *values, int64_t val_stride, int dtype, int64_t n, int64_t ncomp, int scale_exp,
                     double *scalars, mpf_context *ctx, mpf_stream stream);
 
+/* Rows (rows, n) of binned markers back to the input order: out[c * n + order[m]] =
+ * sorted[c * n + m] with order the permutation mpf_bin_markers wrote; both arrays
+ * contiguous rows of dtype.  No counterpart in the reference (its markers are never
+ * reordered). */
+int mpf_unsort_rows(const void *sorted, void *out, const int64_t *order, int64_t rows, int64_t n, int dtype,
+                    mpf_stream stream);
+
 /* ---- many patches in one launch ---------------------------------------------------- */
 
 /* One refined-mesh patch: the arguments of one Weighting.scatter_block / gather_block call

@@ -72,6 +72,7 @@ SIGNATURES: dict[str, list] = {
     "mpf_gather_derivative_binned": [_INT, _INT, _INT, _P, _INT, _I64, _P, _P, _P, _I64, _P, _I64, _P, _I64, _P, _P,
                                      _P, _P, _P, _P],
     "mpf_fixed_scale": [_P, _I64, _INT, _I64, _I64, _INT, _P, _P, _P],
+    "mpf_unsort_rows": [_P, _P, _P, _I64, _I64, _INT, _P],
     "mpf_gather_edges": [_INT, _INT, _INT, _P, _I64, _INT, _I64, _P, _P, _I64, _P, _P, _P, _P, _P, _P],
     "mpf_scatter_forms": [_INT, _INT, _INT, _P, _I64, _P, _I64, _INT, _I64, _I64, _P, _P, _P, _P, _P, _INT, _INT, _P,
                           _P],

@@ -87,10 +87,18 @@ class MarkerBins:
         return self.tables.data_ptr(), self.table_counts.data_ptr()
 
     def unsort(self, sorted_rows: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
-        """Rows (k, N) in bins order back to the input order."""
+        """Rows (k, N) in bins order back to the input order (mpf_unsort_rows)."""
         if out is None:
             out = torch.empty_like(sorted_rows)
-        out.index_copy_(1, self.order, sorted_rows)
+        ok = (sorted_rows.dtype in (torch.float32, torch.float64) and out.dtype == sorted_rows.dtype
+              and sorted_rows.is_contiguous() and out.is_contiguous() and sorted_rows.dim() == 2
+              and tuple(out.shape) == tuple(sorted_rows.shape) and sorted_rows.shape[1] == self.n)
+        if not ok:  # other dtypes or layouts: torch's own scatter
+            out.index_copy_(1, self.order, sorted_rows)
+            return out
+        with torch.cuda.device(out.device):
+            _lib.call("mpf_unsort_rows", sorted_rows.data_ptr(), out.data_ptr(), self.order.data_ptr(),
+                      sorted_rows.shape[0], self.n, K._dtype_code(sorted_rows), K.stream_handle(out.device))
         return out
 
 

@@ -2196,3 +2196,63 @@ extern "C" int mpf_bin_items(int ndim, const int64_t *num_nodes, const uint32_t 
   }
   return MPF_OK;
 }
+
+// out[c, order[m]] = sorted[c, m]: the reads coalesced, the writes scattered by the
+// permutation.  Up to 512 MB of rows one work-item per marker writes all its rows (its
+// input index read once); above, one launch per row keeps each wavefront's scattered
+// writes within one row (3 rows of 2^26 float32: 10.0 ms row by row against 13.8 for all
+// rows at once; 3 rows of 2^25: 5.2 against 5.6 the other way round).
+template <class T>
+__global__ void __launch_bounds__(256) unsort_rows_kernel(const T *__restrict__ sorted, T *__restrict__ out,
+                                                          const int64_t *__restrict__ order, int64_t rows,
+                                                          int64_t n) {
+  for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = order[m];
+    for (int64_t c = 0; c < rows; ++c) out[c * n + o] = sorted[c * n + m];
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) unsort_row_kernel(const T *__restrict__ sorted, T *__restrict__ out,
+                                                         const int64_t *__restrict__ order, int64_t n) {
+  for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x)
+    out[order[m]] = sorted[m];
+}
+
+/* Rows (rows, n) in bins order back to the input order: out[c * n + order[m]] =
+ * sorted[c * n + m], order a permutation of 0..n-1 (mpf_bin_markers'). */
+extern "C" int mpf_unsort_rows(const void *sorted, void *out, const int64_t *order, int64_t rows, int64_t n,
+                               int dtype, mpf_stream stream_) {
+  if (dtype != F32 && dtype != F64) {
+    set_error("dtype must be MPF_F32 or MPF_F64, got %d", dtype);
+    return MPF_ERR_ARGUMENT;
+  }
+  if (rows < 0 || n < 0) {
+    set_error("unsort_rows: need rows >= 0 and n >= 0");
+    return MPF_ERR_ARGUMENT;
+  }
+  if (rows == 0 || n == 0) return MPF_OK;
+  hipStream_t stream = (hipStream_t)stream_;
+  const int blocks = blocks_for(n, 256, 1 << 16);
+  const int64_t bytes = rows * n * (dtype == F32 ? 4 : 8);
+  if (bytes <= (int64_t)512 << 20) {
+    if (dtype == F32)
+      hipLaunchKernelGGL(unsort_rows_kernel<float>, dim3(blocks), dim3(256), 0, stream, (const float *)sorted,
+                         (float *)out, order, rows, n);
+    else
+      hipLaunchKernelGGL(unsort_rows_kernel<double>, dim3(blocks), dim3(256), 0, stream, (const double *)sorted,
+                         (double *)out, order, rows, n);
+    MPF_LAUNCH_CHECK();
+    return MPF_OK;
+  }
+  for (int64_t c = 0; c < rows; ++c) {
+    if (dtype == F32)
+      hipLaunchKernelGGL(unsort_row_kernel<float>, dim3(blocks), dim3(256), 0, stream, (const float *)sorted + c * n,
+                         (float *)out + c * n, order, n);
+    else
+      hipLaunchKernelGGL(unsort_row_kernel<double>, dim3(blocks), dim3(256), 0, stream,
+                         (const double *)sorted + c * n, (double *)out + c * n, order, n);
+    MPF_LAUNCH_CHECK();
+  }
+  return MPF_OK;
+}
