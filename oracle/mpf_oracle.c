@@ -28,9 +28,11 @@
  *
  * Parity pins (tests/test_oracle.py): known answers of the reference tests, closed
  * forms, scipy's BSpline for clamped knots, the adjoint identity, charge totals,
- * continuity G^T j = rho(end) - rho(start), halo fold/fill = bare periodic grid, and
- * the reference's own C kernels (scatter_linear_1d_c.c, scatter_rows_3d_c.c) built
- * into oracle/_ref.
+ * continuity G^T j = rho(end) - rho(start), halo fold/fill = bare periodic grid, the
+ * work identity j.e = sum q int E.dl for the edge gathers, the commuting property of the
+ * forms (the (k+1)-form field of the discrete exterior derivative is the derivative of the
+ * k-form field), and the reference's own C kernels (scatter_linear_1d_c.c,
+ * scatter_rows_3d_c.c) built into oracle/_ref.
  */
 #include <math.h>
 #include <stdint.h>
