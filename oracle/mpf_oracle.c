@@ -25,6 +25,8 @@
  *   Whitney paths  per-thread copies of the edges when N >= 20000 and N >= nodes,
  *                  added thread by thread (deposit_paths_2d_pyccel.py:155-183).
  *   spline paths   serial (deposit_paths_spline_2d_pyccel.py:7).
+ *   forms          deposits as the node deposits (scatter_forms_spline_3d_pyccel.py:
+ *                  153-210); gathers split into contiguous chunks.
  *
  * Parity pins (tests/test_oracle.py): known answers of the reference tests, closed
  * forms, scipy's BSpline for clamped knots, the adjoint identity, charge totals,
