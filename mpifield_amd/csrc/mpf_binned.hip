@@ -1371,15 +1371,42 @@ __global__ void __launch_bounds__(256) scatter_tiled_2d(const T *__restrict__ po
   const double scale = scalars[0];
   __syncthreads();
   // runs of about one cell's markers (the tile's mean per cell; the last run clipped at the
-  // tile's end), lane t taking run t of each pass of BLOCK runs: the lanes of a wavefront
-  // walk neighbouring cells, whose nodes are neighbouring LDS words (component-major), so
-  // their atomics take distinct banks.  A run is taken four markers at a time, their rows
-  // loaded together (one memory latency per four markers).
+  // tile's end), lane t taking run t of each pass of BLOCK runs, so the lanes of a
+  // wavefront walk neighbouring cells (neighbouring LDS words, component-major).  A lane
+  // sums its run's contributions in int64 registers while the anchor cell stays the same
+  // and adds them into LDS when it changes and at the run's end: one LDS atomic per node
+  // and cell instead of one per marker (LDS int64 atomics run at about two lanes per CU
+  // clock).  The integer sums are exact, so the field is the direct deposit's bit for bit.
+  // A run is taken four markers at a time, their rows loaded together.
   const uint32_t per_cell = (last - first) / TILE2_CELLS;
   const uint32_t chunk = per_cell > 0 ? per_cell : 1;
   constexpr int BATCH = 4;
   for (uint32_t lo = first + t * chunk; lo < last; lo += BLOCK * chunk) {
     const uint32_t hi = lo + chunk < last ? lo + chunk : last;
+    int cx = 0, cy = 0;
+    bool held = false;
+    unsigned long long q[S][S][C];
+#pragma unroll
+    for (int a = 0; a < S; ++a)
+#pragma unroll
+      for (int b = 0; b < S; ++b)
+#pragma unroll
+        for (int c = 0; c < C; ++c) q[a][b][c] = 0ull;
+    auto flush = [&]() {
+#pragma unroll
+      for (int a = 0; a < S; ++a) {
+        const int sx = slot_2d(cx - OFF + a, bx, num0, g.code[0]);
+#pragma unroll
+        for (int b = 0; b < S; ++b) {
+          const int sy = slot_2d(cy - OFF + b, by, num1, g.code[1]);
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+            atomicAdd(acc + c * (L * L) + sx * L + sy, q[a][b][c]);
+            q[a][b][c] = 0ull;
+          }
+        }
+      }
+    };
     for (uint32_t m0 = lo; m0 < hi; m0 += BATCH) {
       double px[BATCH], py[BATCH], vs[BATCH][C];
 #pragma unroll
@@ -1396,33 +1423,35 @@ __global__ void __launch_bounds__(256) scatter_tiled_2d(const T *__restrict__ po
         double wx[4], wy[4];
         const int ax = anchor_axis<KIND, P>((px[j] - g.lower[0]) * g.inverse[0], num0, g.code[0], wx);
         const int ay = anchor_axis<KIND, P>((py[j] - g.lower[1]) * g.inverse[1], num1, g.code[1], wy);
+        if (held && (ax != cx || ay != cy)) flush();
+        cx = ax;
+        cy = ay;
+        held = true;
         // value * 2^k: a power-of-two scale, so the integers are round(w * v) * 2^k as in
         // the direct deposit; below 2^50 quanta they take the two-operation rounding
         double vmax = 0.0;
 #pragma unroll
         for (int c = 0; c < C; ++c) vmax = fmax(vmax, fabs(vs[j][c]));
-        auto deposit = [&](auto fast) {
+        auto accumulate = [&](auto fast) {
 #pragma unroll
-          for (int a = 0; a < S; ++a) {
-            const int sx = slot_2d(ax - OFF + a, bx, num0, g.code[0]);
+          for (int a = 0; a < S; ++a)
 #pragma unroll
             for (int b = 0; b < S; ++b) {
-              const int sy = slot_2d(ay - OFF + b, by, num1, g.code[1]);
               const double weight = wx[a] * wy[b];
 #pragma unroll
               for (int c = 0; c < C; ++c) {
                 const double y = weight * vs[j][c];
-                atomicAdd(acc + c * (L * L) + sx * L + sy, decltype(fast)::value ? to_fixed_small(y) : to_fixed(y));
+                q[a][b][c] += decltype(fast)::value ? to_fixed_small(y) : to_fixed(y);
               }
             }
-          }
         };
         if (vmax < 0x1p50)
-          deposit(std::true_type{});
+          accumulate(std::true_type{});
         else
-          deposit(std::false_type{});
+          accumulate(std::false_type{});
       }
     }
+    if (held) flush();
   }
   __syncthreads();
   for (int i = t; i < L * L * C; i += BLOCK) {
