@@ -717,10 +717,15 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 4 : 1) spline_paths_tile
   for (int i = t; i < PT::WORDS; i += BLOCK) lds[i] = 0ull;
   const uint32_t first = start[(int64_t)tile * PT::CELLS], last = start[(int64_t)(tile + 1) * PT::CELLS];
   __syncthreads();
-  const uint32_t count = last - first, chunk = (count + BLOCK - 1) / BLOCK;
-  const uint32_t lo = first + t * chunk, hi = lo + chunk < last ? lo + chunk : last;
+  // 2D tiles hold tens of thousands of paths: lanes take runs of 8 consecutive paths, run t
+  // of each pass of BLOCK runs, so a wavefront's row loads stay coalesced (a lane's share as
+  // one run made every load touch 64 lines: Whitney 5.5 -> 3.4 ms for 2^25 paths on 1024^2).
+  // 3D tiles hold a few thousand for 1024 lanes: an even split, one run per lane.
+  const uint32_t count = last - first;
+  const uint32_t run = NDIM == 2 ? 8u : (count + BLOCK - 1) / BLOCK;
   WhitneyTileSink<NDIM> wsink{(lds_u64 *)lds, accum, scale, &g, ax};
-  for (uint32_t p = lo; p < hi; ++p) {
+  for (uint32_t lo = first + t * run; lo < last; lo += BLOCK * run)
+  for (uint32_t p = lo; p < (lo + run < last ? lo + run : last); ++p) {
     const double charge = (double)rows[p];
     if (charge == 0.0) continue;
     double r0[NDIM], r1[NDIM];
