@@ -289,9 +289,13 @@ __global__ void __launch_bounds__(BLOCK) scatter_forms_tiled(const T *__restrict
   int64_t s0, s1;
   form_strides<NDIM>(g, s0, s1);
   __syncthreads();
-  const uint32_t chunk = (last - first + BLOCK - 1) / BLOCK;
-  const uint32_t lo = first + t * chunk, hi = lo + chunk < last ? lo + chunk : last;
-  for (uint32_t m = lo; m < hi; ++m) {
+  // 3D tiles: an even split of the tile's few thousand markers, one run per lane; 2D tiles
+  // (tens of thousands): runs of 8 consecutive markers, run t of each pass of BLOCK runs, so
+  // a wavefront's row loads stay coalesced
+  const uint32_t count = last - first;
+  const uint32_t run = NDIM == 2 ? 8u : (count + BLOCK - 1) / BLOCK;
+  for (uint32_t lo = first + t * run; lo < last; lo += BLOCK * run)
+  for (uint32_t m = lo; m < (lo + run < last ? lo + run : last); ++m) {
     double x[NDIM], rel[NDIM], vs[NC];
 #pragma unroll
     for (int d = 0; d < NDIM; ++d) x[d] = (double)pos[d * n + m];
