@@ -324,3 +324,36 @@ def test_2d_tiles_match_the_oracle_and_the_direct_fixed_deposit(kind, degree, nc
     binning.gather_binned(bins, cuda(field), res)
     back = bins.unsort(res).cpu().numpy()
     np.testing.assert_allclose(back[:, live], want_g, rtol=1e-12, atol=1e-12 * np.abs(want_g).max())
+
+
+@pytest.mark.parametrize("n", [7, 300, 5000])
+def test_2d_tiles_with_few_markers_per_tile(n):
+    """Sparse 2D tiles (fewer markers than a lane's run of 8, empty tiles, partial runs): the
+    tiled node and forms deposits equal the one-marker-per-work-item fixed-point deposits bit
+    for bit."""
+    from mpifield_amd import _lib
+
+    print(f"sparse 2d tiles n={n}", flush=True)
+    num = (70, 45)
+    g = (0.5, -1.0, 0.03125, 0.0625, *num, 0, 1)
+    rng = np.random.default_rng(n)
+    lo, span = np.array([0.5, -1.0]), np.array([0.03125, 0.0625]) * np.array(num)
+    pos = rng.uniform(lo, lo + span, (n, 2)).T.copy()
+    for kind, degree, ncomp in ((1, 2, 1), (1, 2, 3), (2, 3, 1)):
+        val = rng.normal(1.0, 0.3, (ncomp, n))
+        bins = binning.bin_markers(cuda(pos), cuda(val), (kind, degree), g)
+        tiled = torch.zeros((*num, ncomp), dtype=torch.float64, device="cuda")
+        binning.scatter_binned(bins, tiled)
+        direct = torch.zeros_like(tiled)
+        with K.deposit_mode("fixed"):
+            K.scatter(kind, degree, cuda(pos), cuda(val), direct, g)
+        assert torch.equal(tiled, direct)
+    for form, ncomp in ((1, 2), (2, 1)):
+        val = rng.normal(1.0, 0.3, (ncomp, n))
+        bins = binning.bin_markers(cuda(pos), cuda(val), (_lib.KIND_SPLINE, 2), g)
+        tiled = torch.zeros((*num, ncomp), dtype=torch.float64, device="cuda")
+        binning.scatter_forms_binned(bins, tiled, form)
+        direct = torch.zeros_like(tiled)
+        with K.deposit_mode("fixed"):
+            K.scatter_forms(2, form, cuda(pos), cuda(val), direct, g)
+        assert torch.equal(tiled, direct)
