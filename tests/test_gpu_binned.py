@@ -357,3 +357,29 @@ def test_2d_tiles_with_few_markers_per_tile(n):
         with K.deposit_mode("fixed"):
             K.scatter_forms(2, form, cuda(pos), cuda(val), direct, g)
         assert torch.equal(tiled, direct)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_bins_are_reused_across_steps_with_new_values(dtype):
+    """One sort of the positions serves many deposits: ``with_values`` re-orders each step's
+    values through the bins' permutation, and the deposit equals the one from bins made with
+    those values, bit for bit.  Bins made with values refuse new ones."""
+    num, code = (32, 16, 32), (1, 0, 1)
+    g = geom(num, code)
+    pos, _ = markers(num, code, 200000, 3, seed=12, dtype=dtype)
+    pos[:, ::97] = np.array([[0.3], [-0.2], [1.1]], dtype=dtype)  # finite everywhere
+    bins = binning.bin_markers(cuda(pos), None, (1, 2), g)
+    rng = np.random.default_rng(13)
+    for step in range(2):
+        values = rng.normal(0.0, 2.0 + step, (3, pos.shape[1])).astype(dtype)
+        reused = torch.zeros((*num, 3), dtype=torch.float64, device="cuda")
+        binning.scatter_binned(binning.with_values(bins, cuda(values)), reused)
+        fresh = torch.zeros_like(reused)
+        binning.scatter_binned(binning.bin_markers(cuda(pos), cuda(values), (1, 2), g), fresh)
+        assert torch.equal(reused, fresh), step
+        want = np.zeros((*num, 3))
+        orc.scatter("quadratic", pos.astype(np.float64), values.astype(np.float64), want, *g)
+        # the fixed-point quantum of the automatic scale, a few hundred terms per node
+        assert np.abs(reused.cpu().numpy() - want).max() / np.abs(want).max() < 1e-10
+    with pytest.raises(ValueError, match="made with values"):
+        binning.with_values(binning.bin_markers(cuda(pos), cuda(values), (1, 2), g), cuda(values))
