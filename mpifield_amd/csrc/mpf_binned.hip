@@ -1866,6 +1866,25 @@ static int launch_group_and_cells(const void *pos, int64_t ps, int ndim, const v
   return MPF_OK;
 }
 
+// The bins of no markers: every offset 0, and a unit scale (there is nothing to deposit).
+__global__ void empty_bins_kernel(uint32_t *__restrict__ start, int64_t count, double *__restrict__ scalars) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    start[i] = 0u;
+  if (scalars && blockIdx.x == 0 && threadIdx.x == 0) {
+    scalars[0] = 1.0;
+    scalars[1] = 1.0;
+    scalars[2] = 4.0e9;
+  }
+}
+
+static int empty_bins(uint32_t *start, int64_t count, double *scalars, hipStream_t stream) {
+  const int threads = 256;
+  hipLaunchKernelGGL(empty_bins_kernel, dim3(blocks_for(count, threads, 4096)), dim3(threads), 0, stream, start,
+                     count, scalars);
+  MPF_LAUNCH_CHECK();
+  return MPF_OK;
+}
+
 /* Bin markers by stencil cell (tile order in 3D, row-major cells otherwise).  Writes
  * sorted positions (ndim, n) and values (ncomp, n) with contiguous rows, the permutation
  * order[sorted] = original index (int64), the bin offsets start[bins + 2] (uint32;
@@ -1884,7 +1903,7 @@ extern "C" int mpf_bin_markers(int kind, int degree, int ndim, const void *posit
     set_error("dtype must be MPF_F32 or MPF_F64, got %d", dtype);
     return MPF_ERR_ARGUMENT;
   }
-  if (n < 0 || n > 0x7FFFFFFFll || ncomp < 0 || ncomp > 64 || (ncomp > 0 && !values)) {
+  if (n < 0 || n > 0x7FFFFFFFll || ncomp < 0 || ncomp > 64 || (ncomp > 0 && !values && n > 0)) {
     set_error("bin_markers: need 0 <= n < 2^31 and 0 <= ncomp <= 64 (0 only without values)");
     return MPF_ERR_ARGUMENT;
   }
@@ -1902,6 +1921,7 @@ extern "C" int mpf_bin_markers(int kind, int degree, int ndim, const void *posit
     set_error("bin_markers: too many cells for 32-bit keys");
     return MPF_ERR_ARGUMENT;
   }
+  if (n == 0) return empty_bins(start, cells + 2, scalars, (hipStream_t)stream_);
   // groups of 2^shift keys: about BIN_STAGE / 2 markers per group on average (at most
   // BIN_KEYS_CAP keys), and few enough groups for the LDS counters of passes 1 and 2
   int shift = 0;
@@ -2069,6 +2089,7 @@ extern "C" int mpf_scatter_binned(int kind, int degree, int ndim, const void *so
               (long long)ncomp);
     return MPF_ERR_UNSUPPORTED;
   }
+  if (n == 0) return MPF_OK;  // no markers: nothing to add, no results
   hipStream_t stream = (hipStream_t)stream_;
   if (tiled_2d_supported(kind, degree, ndim, ncomp, g)) {
     if (!ctx) ctx = default_context();
@@ -2110,6 +2131,7 @@ extern "C" int mpf_gather_binned(int kind, int degree, int ndim, const void *sor
               (long long)ncomp);
     return MPF_ERR_UNSUPPORTED;
   }
+  if (n == 0) return MPF_OK;  // no markers: nothing to add, no results
   hipStream_t stream = (hipStream_t)stream_;
   if (tiled_2d_supported(kind, degree, ndim, ncomp, g)) {
 #define GA2(T, K, P, C) run_gather_2d<T, K, P, C>(sorted_positions, n, start, g, field, result, stream)
@@ -2173,6 +2195,7 @@ extern "C" int mpf_gather_derivative_binned(int kind, int degree, int ndim, cons
               (long long)ncomp);
     return MPF_ERR_UNSUPPORTED;
   }
+  if (n == 0) return MPF_OK;  // no markers: nothing to add, no results
   hipStream_t stream = (hipStream_t)stream_;
 #define DV(T, K, P, C) \
   run_derivative_tiled<T, K, P, C>(sorted_positions, n, start, g, field, coef, coef_stride, order, out, out_stride, stream)
@@ -2195,11 +2218,12 @@ extern "C" int mpf_fixed_scale(const void *values, int64_t val_stride, int dtype
     set_error("dtype must be MPF_F32 or MPF_F64, got %d", dtype);
     return MPF_ERR_ARGUMENT;
   }
-  if (n < 0 || ncomp < 1 || !values) {
+  if (n < 0 || ncomp < 1 || (!values && n > 0)) {
     set_error("fixed_scale: need n >= 0, ncomp >= 1 and values");
     return MPF_ERR_ARGUMENT;
   }
   hipStream_t stream = (hipStream_t)stream_;
+  if (n == 0) return empty_bins(nullptr, 0, scalars, stream);
   if (!ctx) ctx = default_context();
   if (!ctx) return MPF_ERR_HIP;
   int rc = prepare_scale(ctx, values, val_stride, dtype, ncomp, n, 1.0, scale_exp, stream);

@@ -383,3 +383,40 @@ def test_bins_are_reused_across_steps_with_new_values(dtype):
         assert np.abs(reused.cpu().numpy() - want).max() / np.abs(want).max() < 1e-10
     with pytest.raises(ValueError, match="made with values"):
         binning.with_values(binning.bin_markers(cuda(pos), cuda(values), (1, 2), g), cuda(values))
+
+
+def test_no_markers_leave_the_field_and_raise_nothing():
+    """Empty inputs (the reference's kernels loop over zero markers): every entry point takes
+    n = 0, adds nothing to the field and writes no results; empty bins hold zero offsets."""
+    num, code = (32, 16, 32), (1, 0, 1)
+    g = geom(num, code)
+    for dtype in (torch.float32, torch.float64):
+        pos = torch.empty((3, 0), dtype=dtype, device="cuda")
+        val = torch.empty((3, 0), dtype=dtype, device="cuda")
+        field = torch.full((*num, 3), 0.5, dtype=torch.float64, device="cuda")
+        res = torch.empty((3, 0), dtype=dtype, device="cuda")
+        for mode in ("atomic", "fixed"):
+            with K.deposit_mode(mode):
+                K.scatter(1, 2, pos, val, field, g)
+                K.scatter_forms(2, 2, pos, val, field, g)
+        K.gather(1, 2, pos, field, res, g)
+        K.gather_forms(2, 2, pos, field, res, g)
+        bins = binning.bin_markers(pos, val, (1, 2), g)
+        assert bins.n == 0 and not bool(bins.start.any())
+        binning.scatter_binned(bins, field)
+        binning.gather_binned(bins, field, res)
+        spline_bins = binning.bin_markers(pos, val, (2, 2), g)  # the forms read bins of the spline nodes
+        binning.scatter_forms_binned(spline_bins, field, 2)
+        binning.gather_forms_binned(spline_bins, field, res, 2)
+        assert bins.unsort(res).shape == (3, 0)
+        bare = binning.bin_markers(pos, None, (1, 2), g)
+        binning.scatter_binned(binning.with_values(bare, val), field)
+        binning.gather_binned(bare.prepare_items(), field, res)
+        grid2 = (0.0, 0.0, 0.1, 0.1, 16, 16, 1, 1)
+        edges = torch.zeros((16, 16, 2), dtype=torch.float64, device="cuda")
+        for kind, degree in ((0, 1), (1, 2)):
+            K.deposit_paths(kind, degree, pos[:2], pos[:2], torch.empty(0, dtype=dtype, device="cuda"), edges,
+                            grid2)
+            K.gather_edges(kind, degree, pos[:2], edges, res[:2], grid2)
+        torch.cuda.synchronize()
+        assert bool((field == 0.5).all()) and not bool(edges.any())
