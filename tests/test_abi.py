@@ -60,6 +60,15 @@ def test_argument_errors_need_no_gpu():
     rc = lib.mpf_deposit_paths(1, 5, 1, None, 0, None, 0, None, 1, 0, None, lower.ctypes.data, cell.ctypes.data,
                                num.ctypes.data, code.ctypes.data, 0, _lib.SCALE_AUTO, None, None)
     assert rc == 1 and b"spline edge degrees" in lib.mpf_last_error()
+    # the bins' limits: 32-bit offsets and at most 64 value rows, refused before any device work
+    rc = lib.mpf_bin_markers(1, 2, 1, None, 0, None, 0, 1, 1 << 31, 0, lower.ctypes.data, cell.ctypes.data,
+                             num.ctypes.data, code.ctypes.data, None, None, None, None, None, _lib.SCALE_AUTO, None,
+                             None)
+    assert rc == 1 and b"n < 2^31" in lib.mpf_last_error()
+    rc = lib.mpf_bin_markers(1, 2, 1, None, 0, None, 0, 1, 10, 65, lower.ctypes.data, cell.ctypes.data,
+                             num.ctypes.data, code.ctypes.data, None, None, None, None, None, _lib.SCALE_AUTO, None,
+                             None)
+    assert rc == 1 and b"ncomp <= 64" in lib.mpf_last_error()
 
 
 def test_plugin_api_names_and_messages():
