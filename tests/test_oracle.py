@@ -390,8 +390,61 @@ def test_spline_path_continuity(degree, ndim, periodic):
     np.testing.assert_allclose(orc.divergence(current, [periodic] * ndim), change, atol=1e-12)
 
 
+def clamped_paths(ndim, degree, seed=0, n=300):
+    """The reference's clamped case (tests/unit/test_clamped.py:78-90): 8 x 7 x 6 nodes with
+    spacings 0.5, 0.4, 0.3, the first and third axes clamped and the second periodic, starts
+    0.2 beyond the walls, ends two cells away on average."""
+    rng = np.random.default_rng(seed)
+    num = np.array((8, 7, 6)[:ndim])
+    h = np.array((0.5, 0.4, 0.3)[:ndim])
+    codes = [2, 1, 2][:ndim]
+    extent = tuple(int(k) + (degree - 1 if c == 2 else 0) for k, c in zip(num, codes))
+    upper = h * (num - 1)
+    start = np.ascontiguousarray(rng.uniform(-0.2, upper + 0.2, (n, ndim)).T)
+    q = rng.normal(size=n)
+    end = np.ascontiguousarray(start + rng.normal(scale=2 * h[:, None], size=start.shape))
+    return geom(np.zeros(ndim), h, num, codes), extent, codes, start, end, q
+
+
 @pytest.mark.parametrize("degree", [1, 2, 3])
-def test_path_deposit_is_the_line_integral_of_the_edge_functions(degree):
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_clamped_axes_keep_the_charge_and_conserve_it_along_paths(ndim, degree):
+    """tests/unit/test_clamped.py:103-131: on clamped axes (the node arrays num + degree - 1
+    long) the clamped spline deposit keeps the whole charge, and the spline-edge current of
+    the paths has G^T j = rho(end) - rho(start) at 1e-12, the walls and clamped knots cut."""
+    g, extent, codes, start, end, q = clamped_paths(ndim, degree, seed=degree)
+    before = np.zeros((*extent, 1))
+    orc.scatter("spline", start, q[None], before, *g, degree=degree)
+    after = np.zeros((*extent, 1))
+    orc.scatter("spline", end, q[None], after, *g, degree=degree)
+    assert before.sum() == pytest.approx(q.sum(), abs=1e-12)
+    current = np.zeros((*extent, ndim))
+    orc.deposit_spline(start, end, q, current, *g, degree=degree)
+    div = orc.divergence(current, [c == 1 for c in codes])
+    np.testing.assert_allclose(div, (after - before)[..., 0], atol=1e-12)
+    # the edges of a clamped axis stop one short of its nodes (num + degree - 2 of them)
+    for axis, c in enumerate(codes):
+        if c == 2:
+            assert np.all(current.take(-1, axis=axis)[..., axis] == 0.0)
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+def test_clamped_continuity_fails_for_a_planted_mistake(degree):
+    """The clamped continuity check above is sharp: a current deposited with the walls'
+    cuts missing (the path taken as periodic along the clamped axis) breaks it."""
+    g, extent, codes, start, end, q = clamped_paths(2, degree, seed=7)
+    before = np.zeros((*extent, 1))
+    orc.scatter("spline", start, q[None], before, *g, degree=degree)
+    after = np.zeros((*extent, 1))
+    orc.scatter("spline", end, q[None], after, *g, degree=degree)
+    wrong = list(g)
+    wrong[6] = 1  # the first axis (clamped) read as periodic
+    current = np.zeros((extent[0] - (degree - 1), extent[1], 2))
+    orc.deposit_spline(start, end, q, current, *wrong, degree=degree)
+    padded = np.zeros((*extent, 2))
+    padded[: current.shape[0]] = current
+    div = orc.divergence(padded, [c == 1 for c in codes])
+    assert np.abs(div - (after - before)[..., 0]).max() > 1e-3
     """Brute force on a periodic 2D grid: j_(k,a) = q int B_(p-1)(r_a - k_a - 1/2) B_p(r_b - k_b) dr_a."""
     rng = np.random.default_rng(degree)
     shape, h = (8, 6), (0.5, 0.25)
@@ -525,20 +578,24 @@ def test_whitney_field_of_a_gradient_is_the_gradient_of_linear(ndim, periodic):
                                    atol=1e-10)
 
 
-def _work_along(start, end, q, edges, g, degree, periodic):
-    """sum q * integral of E . dl along the paths (clamped to a bounded grid), exact: each
-    path cut where the field's pieces change (the knot planes, integers for WHITNEY and odd
-    degrees, half-integers for even ones, and the walls), 5-point Gauss-Legendre on every
-    piece (E . dl is a polynomial of degree at most 8 in t there)."""
+def _work_along(start, end, q, edges, g, degree):
+    """sum q * integral of E . dl along the paths (clamped to the grid on bounded and clamped
+    axes), exact: each path cut where the field's pieces change (the knot planes: integers
+    for WHITNEY, odd degrees and clamped axes, half-integers for even degrees elsewhere; and
+    the walls), 5-point Gauss-Legendre on every piece (E . dl is a polynomial of degree at
+    most 8 in t there)."""
     ndim = start.shape[0]
     lower = np.asarray(g[:ndim])
     h = np.asarray(g[ndim:2 * ndim])
     num = np.asarray(g[2 * ndim:3 * ndim])
-    shift = 0.5 if degree % 2 == 0 and degree > 0 else 0.0
+    codes = [int(c) for c in g[3 * ndim:4 * ndim]]
+    even = 0.5 if degree % 2 == 0 and degree > 0 else 0.0
     nodes, weights = np.polynomial.legendre.leggauss(5)
+    walled = np.array([c != 1 for c in codes])[:, None]
 
     def clamp(points):
-        return points if periodic else np.clip(points, lower[:, None], (lower + h * (num - 1))[:, None])
+        inside = np.clip(points, lower[:, None], (lower + h * (num - 1))[:, None])
+        return np.where(walled, inside, points)
 
     work = 0.0
     for p in range(q.size):
@@ -548,8 +605,9 @@ def _work_along(start, end, q, edges, g, degree, periodic):
             if r1[d] == r0[d]:
                 continue
             lo, hi = min(r0[d], r1[d]), max(r0[d], r1[d])
+            shift = 0.0 if codes[d] == 2 else even
             planes = np.arange(np.floor(lo - shift) + shift, hi + 1.0)
-            if not periodic:
+            if codes[d] != 1:
                 planes = np.concatenate([planes, [0.0, num[d] - 1.0]])
             t = (planes - r0[d]) / (r1[d] - r0[d])
             cuts.extend(t[(t > 0.0) & (t < 1.0)])
@@ -577,7 +635,21 @@ def test_the_current_is_the_transpose_of_the_edge_gather(degree, ndim, periodic)
         orc.deposit_whitney(start, end, q, current, *g)
     else:
         orc.deposit_spline(start, end, q, current, *g, degree=degree)
-    work = _work_along(start, end, q, edges, g, degree, periodic)
+    work = _work_along(start, end, q, edges, g, degree)
+    assert float(np.sum(current * edges)) == pytest.approx(work, rel=1e-11, abs=1e-12)
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_clamped_current_is_the_transpose_of_the_edge_gather(degree, ndim):
+    """The work identity j . e = sum q * integral of E . dl (test_spline_edges.py:88-115) on
+    the clamped case of tests/unit/test_clamped.py:78-90: the walls of the clamped axes
+    (code 2) clamp the paths and cut them, the clamped knots sit at the nodes."""
+    g, extent, codes, start, end, q = clamped_paths(ndim, degree, seed=20 + degree, n=16)
+    edges = np.random.default_rng(degree).normal(size=(*extent, ndim))
+    current = np.zeros_like(edges)
+    orc.deposit_spline(start, end, q, current, *g, degree=degree)
+    work = _work_along(start, end, q, edges, g, degree)
     assert float(np.sum(current * edges)) == pytest.approx(work, rel=1e-11, abs=1e-12)
 
 
