@@ -1105,70 +1105,151 @@ __device__ __forceinline__ void weights_f32(double rel, int num, int code, float
   }
 }
 
+// A node of the gather's float32 footprint: one component, or three held as an (x, y)
+// pair and a z value (read as one 8-byte and one 4-byte LDS word: no padding word in the
+// registers, and no 12-byte read, which the LDS serves at a third of the rate).
+struct Node3 {
+  float x, y, z;
+};
 template <int C>
 struct NodeSlot {
-  using type = float4;  // three components and a pad: one 16-byte LDS read per node
+  using type = Node3;
 };
 template <>
 struct NodeSlot<1> {
   using type = float;
 };
 
-template <int C>
-__device__ __forceinline__ typename NodeSlot<C>::type make_slot(const double *f) {
-  if constexpr (C == 1)
-    return (float)f[0];
-  else
-    return make_float4((float)f[0], (float)f[1], (float)f[2], 0.0f);
+// The bounds of rel on one axis within which a marker's stencil needs neither a periodic
+// wrap nor a wall clamp: [0, num) on a periodic axis, [0, num - 1] on a bounded one; none
+// on a clamped axis (hi < lo).
+struct AxisRange {
+  double hi[3];
+};
+
+__device__ __forceinline__ AxisRange axis_ranges(const Grid &g) {
+  AxisRange r;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double nd = (double)g.num[d];
+    r.hi[d] = g.code[d] == 1 ? nd : (g.code[d] == 0 ? nd - 1.0 : -1.0);
+  }
+  return r;
 }
 
-// Interpolation at one marker from the (S^3) node slots of its cell, in float32.
-template <int KIND, int P, int C, int S>
-__device__ __forceinline__ void interpolate_marker(const typename NodeSlot<C>::type (&nd)[S][S][S], float x, float y,
-                                                   float z, const Grid &g, int num0, int num1, int num2,
-                                                   float *out) {
-  float wx[4], wy[4], wz[4];
-  weights_f32<KIND, P>(((double)x - g.lower[0]) * g.inverse[0], num0, g.code[0], wx);
-  weights_f32<KIND, P>(((double)y - g.lower[1]) * g.inverse[1], num1, g.code[1], wy);
-  weights_f32<KIND, P>(((double)z - g.lower[2]) * g.inverse[2], num2, g.code[2], wz);
-  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
-#pragma unroll
-  for (int a = 0; a < S; ++a)
-#pragma unroll
-    for (int b = 0; b < S; ++b) {
-      const float wab = wx[a] * wy[b];
-#pragma unroll
-      for (int e = 0; e < S; ++e) {
-        const float w = wab * wz[e];
-        if constexpr (C == 1) {
-          s0 = fmaf(w, nd[a][b][e], s0);
-        } else {
-          s0 = fmaf(w, nd[a][b][e].x, s0);
-          s1 = fmaf(w, nd[a][b][e].y, s1);
-          s2 = fmaf(w, nd[a][b][e].z, s2);
-        }
-      }
+// The float32 weights of rel on one axis for a marker binned into stencil cell `cell`
+// (in [0, num)).  Where the position needs neither a periodic wrap nor a wall clamp
+// (0 <= rel <= hi; rel < num holds on a periodic axis once |o| <= 1/2), the reference's
+// offset r - middle is rel - cell exactly: the binning found middle = cell with the same
+// float64 arithmetic.  Other positions (a wrap, a clamp, the node past a periodic seam)
+// take the full selection of weights_f32.  Linear bounded axes need no test: the left node
+// is the bin's cell for any rel and the fraction is clamped.
+template <int KIND, int P>
+__device__ __forceinline__ void cell_weights_f32(double rel, double cell, double hi, int num, int code, float *w) {
+  const double o = rel - cell;
+  if (KIND == LINEAR) {
+    const bool fast = (code == 0) | ((rel >= 0.0) & (rel < hi) & (o >= 0.0) & (o < 1.0));
+    if (fast) {
+      const float f = (float)(o < 0.0 ? 0.0 : (o > 1.0 ? 1.0 : o));
+      w[0] = 1.0f - f;
+      w[1] = f;
+    } else {
+      weights_f32<KIND, P>(rel, num, code, w);
     }
-  out[0] = s0;
-  if constexpr (C == 3) {
-    out[1] = s1;
-    out[2] = s2;
+    return;
+  }
+  const bool near = (P % 2 == 0) ? (fabs(o) <= 0.5) : ((o >= 0.0) & (o < 1.0));
+  if (!((rel >= 0.0) & (rel <= hi) & near)) {
+    weights_f32<KIND, P>(rel, num, code, w);
+    return;
+  }
+  const float f = (float)o;
+  if (P == 0) {
+    w[0] = 1.0f;
+  } else if (P == 1) {
+    w[0] = 1.0f - f;
+    w[1] = f;
+  } else if (P == 2) {
+    w[0] = 0.5f * (0.5f - f) * (0.5f - f);
+    w[1] = 0.75f - f * f;
+    w[2] = 0.5f * (0.5f + f) * (0.5f + f);
+  } else {
+    const float cm = 1.0f - f, sq = f * f, cu = sq * f;
+    w[0] = cm * cm * cm * (1.0f / 6.0f);
+    w[1] = (3.0f * cu - 6.0f * sq + 4.0f) * (1.0f / 6.0f);
+    w[2] = (-3.0f * cu + 3.0f * sq + 3.0f * f + 1.0f) * (1.0f / 6.0f);
+    w[3] = cu * (1.0f / 6.0f);
   }
 }
 
-template <int C>
+// Interpolation at one marker of cell (cx, cy, cz) from the (S^3) node slots of the cell,
+// in float32: the y-z weight products once, then per x-slice and component a sum over the
+// slice's S^2 nodes, then the x weights (S^2 + C S^3 + C S operations).
+template <int KIND, int P, int C, int S>
+__device__ __forceinline__ void interpolate_marker(const typename NodeSlot<C>::type (&nd)[S][S][S], float x, float y,
+                                                   float z, const double (&cell)[3], const AxisRange &ar,
+                                                   const Grid &g, int num0, int num1, int num2, float *out) {
+  float wx[4], wy[4], wz[4];
+  cell_weights_f32<KIND, P>(((double)x - g.lower[0]) * g.inverse[0], cell[0], ar.hi[0], num0, g.code[0], wx);
+  cell_weights_f32<KIND, P>(((double)y - g.lower[1]) * g.inverse[1], cell[1], ar.hi[1], num1, g.code[1], wy);
+  cell_weights_f32<KIND, P>(((double)z - g.lower[2]) * g.inverse[2], cell[2], ar.hi[2], num2, g.code[2], wz);
+  float wyz[S][S];
+#pragma unroll
+  for (int b = 0; b < S; ++b)
+#pragma unroll
+    for (int e = 0; e < S; ++e) wyz[b][e] = wy[b] * wz[e];
+  float s[C];
+#pragma unroll
+  for (int a = 0; a < S; ++a) {
+    float t[C];
+#pragma unroll
+    for (int q = 0; q < C; ++q) t[q] = 0.0f;
+#pragma unroll
+    for (int b = 0; b < S; ++b)
+#pragma unroll
+      for (int e = 0; e < S; ++e) {
+        if constexpr (C == 1) {
+          t[0] = fmaf(wyz[b][e], nd[a][b][e], t[0]);
+        } else {
+          t[0] = fmaf(wyz[b][e], nd[a][b][e].x, t[0]);
+          t[1] = fmaf(wyz[b][e], nd[a][b][e].y, t[1]);
+          t[2] = fmaf(wyz[b][e], nd[a][b][e].z, t[2]);
+        }
+      }
+#pragma unroll
+    for (int q = 0; q < C; ++q) s[q] = a == 0 ? wx[0] * t[q] : fmaf(wx[a], t[q], s[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < C; ++q) out[q] = s[q];
+}
+
+// A float32 marker row entry by binned index: with SMALL (every row of the binned arrays
+// below 4 GiB from its base) the byte offset is a 32-bit value added to the row's base,
+// which the compiler keeps in scalar registers (no 64-bit address arithmetic per load).
+template <bool SMALL>
+__device__ __forceinline__ const float *row_at(const float *row, int64_t m) {
+  if constexpr (SMALL) return (const float *)((const char *)row + (uint32_t)((uint32_t)m * 4u));
+  else return row + m;
+}
+template <bool SMALL>
+__device__ __forceinline__ float *row_at(float *row, int64_t m) {
+  if constexpr (SMALL) return (float *)((char *)row + (uint32_t)((uint32_t)m * 4u));
+  else return row + m;
+}
+
 struct PosAhead {
   float x[ITEM], y[ITEM], z[ITEM];
 };
 
-__device__ __forceinline__ void load_pos_ahead(const float *__restrict__ pos, int64_t n, const Item &I,
-                                               PosAhead<1> &a) {
+template <bool SMALL>
+__device__ __forceinline__ void load_pos_ahead(const float *__restrict__ px, const float *__restrict__ py,
+                                               const float *__restrict__ pz, const Item &I, PosAhead &a) {
 #pragma unroll
   for (int r = 0; r < ITEM; ++r) {
     const int64_t m = I.begin + (r < (int)I.size ? r : 0);
-    a.x[r] = pos[m];
-    a.y[r] = pos[n + m];
-    a.z[r] = pos[2 * n + m];
+    a.x[r] = *row_at<SMALL>(px, m);
+    a.y[r] = *row_at<SMALL>(py, m);
+    a.z[r] = *row_at<SMALL>(pz, m);
   }
 }
 
@@ -1179,8 +1260,8 @@ __device__ __forceinline__ void load_pos_ahead(const float *__restrict__ pos, in
 // node slots from LDS into registers once and interpolates the item's markers, whose
 // positions were loaded while the previous item was computed.  Tiles are numbered so that
 // each XCD (workgroups go round-robin over the eight) takes a contiguous range of them.
-template <int KIND, int P, int C, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) gather_items(const float *__restrict__ pos, int64_t n,
+template <int KIND, int P, int C, int BLOCK, bool SMALL>
+__global__ void __launch_bounds__(BLOCK, 3) gather_items(const float *__restrict__ pos, int64_t n,
                                                       const uint32_t *__restrict__ start, Grid g, int ntx, int nty,
                                                       int ntz, const double *__restrict__ field,
                                                       float *__restrict__ res, const uint16_t *__restrict__ tables,
@@ -1192,7 +1273,9 @@ __global__ void __launch_bounds__(BLOCK) gather_items(const float *__restrict__ 
   constexpr int LX = TX + S - 1, LY = TY + S - 1, LZ = TZ + S - 1, L = LX * LY * LZ;
   constexpr int PARTS = TC / BLOCK;
   static_assert(C == 1 || C == 3, "float32 gather tiles hold 1 or 3 components");
-  __shared__ Slot tile[L];
+  // the footprint: component 0 (C = 1), or the (x, y) pairs and the z values (C = 3)
+  __shared__ float2 tile_xy[C == 3 ? L : 1];
+  __shared__ float tile_z[L];
   __shared__ uint32_t cfirst[TC + 1];
   __shared__ __align__(16) uint16_t items[SWEEPS * TC];
   __shared__ unsigned long long wave_sums[(BLOCK / 64) * PARTS];
@@ -1213,7 +1296,7 @@ __global__ void __launch_bounds__(BLOCK) gather_items(const float *__restrict__ 
     // row is LZ * C consecutive words of the field), every load of the work-item issued
     // before the first one is used: one memory latency for both
     constexpr int EO = (TC + 1 + BLOCK - 1) / BLOCK;
-    constexpr int E = (L * C + BLOCK - 1) / BLOCK, CV = C == 1 ? 1 : 4, ROW = LZ * C;
+    constexpr int E = (L * C + BLOCK - 1) / BLOCK, ROW = LZ * C;
     const bool plain_z = g.code[2] != 2 && z0 - OFF >= 0 && z0 - OFF + LZ <= num2;
     constexpr int Q = SWEEPS * TC * 2 / 16 / BLOCK;
     uint32_t o[EO];
@@ -1240,13 +1323,14 @@ __global__ void __launch_bounds__(BLOCK) gather_items(const float *__restrict__ 
       const int i = t + u * BLOCK;
       if (i <= TC) cfirst[i] = o[u];
     }
-    float *flat = reinterpret_cast<float *>(tile);
+    float *flat_xy = reinterpret_cast<float *>(tile_xy);
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       const int e = t + u * BLOCK;
       if (e < L * C) {
         const int i = e / C, q = e - (e / C) * C;
-        flat[i * CV + q] = (float)v[u];
+        if (q == C - 1) tile_z[i] = (float)v[u];
+        else flat_xy[2 * i + q] = (float)v[u];
       }
     }
     if (tables) {
@@ -1258,44 +1342,60 @@ __global__ void __launch_bounds__(BLOCK) gather_items(const float *__restrict__ 
   const uint32_t total = tables ? table_counts[id] : build_items<BLOCK, TC>(cfirst, items, wave_sums);
 
   // lookahead as in scatter_tiled: unconditional loads, the first ones drained before the loop
+  const float *px = pos, *py = pos + n, *pz = pos + 2 * n;
+  float *rq[C];
+#pragma unroll
+  for (int q = 0; q < C; ++q) rq[q] = res + q * n;
+  const AxisRange ar = axis_ranges(g);
   uint32_t it = t;
   Item cur{0, 0, 0};
-  PosAhead<1> now;
+  PosAhead now;
   if (total > 0) {
     cur = item_at(items, cfirst, it < total ? it : total - 1);
-    load_pos_ahead(pos, n, cur, now);
+    load_pos_ahead<SMALL>(px, py, pz, cur, now);
     wait_loads();
   }
   while (it < total) {
     const uint32_t nit = it + BLOCK;
     const Item nxt = item_at(items, cfirst, nit < total ? nit : total - 1);
-    PosAhead<1> next;
-    load_pos_ahead(pos, n, nxt, next);
+    PosAhead next;
+    load_pos_ahead<SMALL>(px, py, pz, nxt, next);
     const int c = cur.cell;
-    const int base = ((c / (TY * TZ)) * LY + (c / TZ) % TY) * LZ + c % TZ;
+    const int cx = c / (TY * TZ), cy = (c / TZ) % TY, cz = c % TZ;
+    const int base = (cx * LY + cy) * LZ + cz;
+    const double cell[3] = {(double)(x0 + cx), (double)(y0 + cy), (double)(z0 + cz)};
     Slot nd[S][S][S];
 #pragma unroll
     for (int a = 0; a < S; ++a)
 #pragma unroll
       for (int b2 = 0; b2 < S; ++b2)
 #pragma unroll
-        for (int e = 0; e < S; ++e) nd[a][b2][e] = tile[base + (a * LY + b2) * LZ + e];
+        for (int e = 0; e < S; ++e) {
+          const int k = base + (a * LY + b2) * LZ + e;
+          if constexpr (C == 1) {
+            nd[a][b2][e] = tile_z[k];
+          } else {
+            const float2 xy = tile_xy[k];
+            nd[a][b2][e] = Node3{xy.x, xy.y, tile_z[k]};
+          }
+        }
 #pragma unroll
     for (int r = 0; r < ITEM; ++r) {
       if ((uint32_t)r < cur.size) {
         float out[C];
-        interpolate_marker<KIND, P, C, S>(nd, now.x[r], now.y[r], now.z[r], g, num0, num1, num2, out);
+        interpolate_marker<KIND, P, C, S>(nd, now.x[r], now.y[r], now.z[r], cell, ar, g, num0, num1, num2, out);
         const int64_t m = cur.begin + r;
 #pragma unroll
-        for (int q = 0; q < C; ++q) res[q * n + m] = out[q];
+        for (int q = 0; q < C; ++q) *row_at<SMALL>(rq[q], m) = out[q];
       }
     }
     for (uint32_t r = ITEM; r < cur.size; ++r) {
       const int64_t m = cur.begin + r;
       float out[C];
-      interpolate_marker<KIND, P, C, S>(nd, pos[m], pos[n + m], pos[2 * n + m], g, num0, num1, num2, out);
+      interpolate_marker<KIND, P, C, S>(nd, *row_at<SMALL>(px, m), *row_at<SMALL>(py, m), *row_at<SMALL>(pz, m), cell,
+                                        ar, g, num0, num1, num2, out);
 #pragma unroll
-      for (int q = 0; q < C; ++q) res[q * n + m] = out[q];
+      for (int q = 0; q < C; ++q) *row_at<SMALL>(rq[q], m) = out[q];
     }
     cur = nxt;
     now = next;
@@ -2040,9 +2140,16 @@ static int run_gather_tiled(const void *pos, int64_t n, const uint32_t *start, c
             ntz = (int)((g.num[2] + TZ - 1) / TZ);
   {
     ProfileScope prof("gather_tiled", stream);
-    if constexpr (sizeof(T) == 4)
-      hipLaunchKernelGGL((gather_items<KIND, P, C, BLOCK>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
-                         (const float *)pos, n, start, g, ntx, nty, ntz, field, (float *)res, tables, table_counts);
+    // 32-bit row offsets while every row ends below 4 GiB from its base
+    const bool small = (uint64_t)n * 4u < (1ull << 32);
+    if constexpr (sizeof(T) == 4) {
+      if (small)
+        hipLaunchKernelGGL((gather_items<KIND, P, C, BLOCK, true>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
+                           (const float *)pos, n, start, g, ntx, nty, ntz, field, (float *)res, tables, table_counts);
+      else
+        hipLaunchKernelGGL((gather_items<KIND, P, C, BLOCK, false>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
+                           (const float *)pos, n, start, g, ntx, nty, ntz, field, (float *)res, tables, table_counts);
+    }
     else
       hipLaunchKernelGGL((gather_tiled<T, KIND, P, C, BLOCK>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
                          (const T *)pos, n, start, g, ntx, nty, ntz, field, (T *)res);
