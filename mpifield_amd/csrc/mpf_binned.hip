@@ -84,6 +84,68 @@ __device__ __forceinline__ int anchor_axis(double rel, int num, int code, double
   return middle;
 }
 
+// The bounds of rel on one axis within which a marker's stencil needs neither a periodic
+// wrap nor a wall clamp: [0, num) on a periodic axis, [0, num - 1] on a bounded one; none
+// on a clamped axis (hi < 0).
+struct AxisRange {
+  double hi[3];
+};
+
+__device__ __forceinline__ AxisRange axis_ranges(const Grid &g) {
+  AxisRange r;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double nd = (double)g.num[d];
+    r.hi[d] = g.code[d] == 1 ? nd : (g.code[d] == 0 ? nd - 1.0 : -1.0);
+  }
+  return r;
+}
+
+// Whether anchor_axis's offset of rel from its middle node is rel - cell exactly, for a
+// marker binned into stencil cell `cell` (in [0, num)): the position needs neither a
+// periodic wrap nor a wall clamp (0 <= rel <= hi; rel < num follows on a periodic axis once
+// the offset is in range), and the middle node the binning found is not the wrapped node
+// num.  Linear bounded axes always qualify: the left node is the bin's cell for any rel and
+// the fraction is clamped.
+template <int KIND, int P>
+__device__ __forceinline__ bool offset_in_cell(double rel, double o, double hi, int code) {
+  if (KIND == LINEAR) return (code == 0) | ((rel >= 0.0) & (rel < hi) & (o >= 0.0) & (o < 1.0));
+  const bool near = (P % 2 == 0) ? (fabs(o) <= 0.5) : ((o >= 0.0) & (o < 1.0));
+  return (rel >= 0.0) & (rel <= hi) & near;
+}
+
+// anchor_axis's weights for a marker binned into stencil cell `cell`: from the offset
+// rel - cell where offset_in_cell holds (the same float64 operations, so the same bits),
+// from the full selection elsewhere.
+template <int KIND, int P>
+__device__ __forceinline__ void cell_weights_f64(double rel, double cell, double hi, int num, int code, double *w) {
+  const double o = rel - cell;
+  if (!offset_in_cell<KIND, P>(rel, o, hi, code)) {
+    anchor_axis<KIND, P>(rel, num, code, w);
+    return;
+  }
+  if (KIND == LINEAR) {
+    const double f = o < 0.0 ? 0.0 : (o > 1.0 ? 1.0 : o);
+    w[0] = 1.0 - f;
+    w[1] = f;
+  } else if (P == 0) {
+    w[0] = 1.0;
+  } else if (P == 1) {
+    w[0] = 1.0 - o;
+    w[1] = o;
+  } else if (P == 2) {
+    w[0] = 0.5 * (0.5 - o) * (0.5 - o);
+    w[1] = 0.75 - o * o;
+    w[2] = 0.5 * (0.5 + o) * (0.5 + o);
+  } else {
+    const double cm = 1.0 - o, sq = o * o, cu = sq * o;
+    w[0] = cm * cm * cm / 6.0;
+    w[1] = (3.0 * cu - 6.0 * sq + 4.0) / 6.0;
+    w[2] = (-3.0 * cu + 3.0 * sq + 3.0 * o + 1.0) / 6.0;
+    w[3] = cu / 6.0;
+  }
+}
+
 template <int KIND, int P>
 struct StencilShape {
   static constexpr int S = KIND == LINEAR ? 2 : P + 1;  // nodes per axis
@@ -697,80 +759,121 @@ __device__ __forceinline__ Item item_at(const uint16_t *items, const uint32_t *c
 // of the work-item has completed.
 __device__ __forceinline__ void wait_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-// The first ITEM markers of an item, loaded ahead of their use (indices past the item's end
+// A marker row entry by binned index: with SMALL (every row of the binned arrays ends below
+// 4 GiB from its base) the byte offset is a 32-bit value added to the row's base, which
+// stays in scalar registers (no 64-bit address arithmetic per access).
+template <bool SMALL, class T>
+__device__ __forceinline__ T *row_at(T *row, int64_t m) {
+  if constexpr (SMALL) return (T *)((char *)row + (uint32_t)((uint32_t)m * (uint32_t)sizeof(T)));
+  else return row + m;
+}
+template <bool SMALL, class T>
+__device__ __forceinline__ const T *row_at(const T *row, int64_t m) {
+  if constexpr (SMALL) return (const T *)((const char *)row + (uint32_t)((uint32_t)m * (uint32_t)sizeof(T)));
+  else return row + m;
+}
+
+// Up to ITEM markers of an item, loaded ahead of their use (indices past the item's end
 // repeat its first marker, which exists, so every load is in bounds).
 template <class T, int C>
 struct Ahead {
   T x[ITEM], y[ITEM], z[ITEM], v[ITEM][C > 0 ? C : 1];
 };
 
+// The marker rows of a tiled deposit: positions (3) and values (C), each a base pointer.
 template <class T, int C>
-__device__ __forceinline__ void load_ahead(const T *__restrict__ pos, const T *__restrict__ val, int64_t n,
-                                           const Item &I, Ahead<T, C> &a) {
+struct MarkerRows {
+  const T *x, *y, *z, *v[C];
+};
+
+template <bool SMALL, class T, int C>
+__device__ __forceinline__ void load_ahead(const MarkerRows<T, C> &rows, int64_t first, uint32_t count,
+                                           Ahead<T, C> &a) {
 #pragma unroll
   for (int r = 0; r < ITEM; ++r) {
-    const int64_t m = I.begin + (r < (int)I.size ? r : 0);
-    a.x[r] = pos[m];
-    a.y[r] = pos[n + m];
-    a.z[r] = pos[2 * n + m];
+    const int64_t m = first + (r < (int)count ? r : 0);
+    a.x[r] = *row_at<SMALL>(rows.x, m);
+    a.y[r] = *row_at<SMALL>(rows.y, m);
+    a.z[r] = *row_at<SMALL>(rows.z, m);
 #pragma unroll
-    for (int q = 0; q < C; ++q) a.v[r][q] = val[q * n + m];
+    for (int q = 0; q < C; ++q) a.v[r][q] = *row_at<SMALL>(rows.v[q], m);
   }
 }
 
-template <int KIND, int P, int C, int S, bool FIRST>
-__device__ __forceinline__ void deposit_marker(double (&a)[C][S][S][S], double x, double y, double z,
-                                               const double *v, const Grid &g, int num0, int num1, int num2,
-                                               double scale) {
-  double wx[4], wy[4], wz[4];
-  anchor_axis<KIND, P>((x - g.lower[0]) * g.inverse[0], num0, g.code[0], wx);
-  anchor_axis<KIND, P>((y - g.lower[1]) * g.inverse[1], num1, g.code[1], wy);
-  anchor_axis<KIND, P>((z - g.lower[2]) * g.inverse[2], num2, g.code[2], wz);
-  double wyz[S][S];
+// The register sums of one (component, x-slice) pass back to zero (the bias).
+template <int S>
+__device__ __forceinline__ void rebias(double (&a)[S][S]) {
 #pragma unroll
   for (int j = 0; j < S; ++j)
 #pragma unroll
-    for (int k = 0; k < S; ++k) wyz[j][k] = wy[j] * wz[k];
+    for (int k = 0; k < S; ++k) a[j][k] = BIAS;
+}
+
+// The register sums of one pass into the tile's LDS accumulator: node (i, j, k) of the cell
+// whose footprint position is `base` sits at base + (i * LY + j) * LZ + k, component q at
+// q * L further.  acc - 1.5 * 2^52 as an integer: the low word is the low word of acc's
+// bits (the bias has none), the high word one 32-bit subtraction.
+template <int S, int LY, int LZ>
+__device__ __forceinline__ void flush_slice(unsigned long long *acc, int at, const double (&a)[S][S]) {
+#pragma unroll
+  for (int j = 0; j < S; ++j)
+#pragma unroll
+    for (int k = 0; k < S; ++k) atomicAdd(&acc[at + j * LZ + k], (unsigned long long)biased_to_int(a[j][k]));
+}
+
+// The deposit of up to ITEM markers of one cell (count >= 1; the first always present):
+// their float64 weights once (x weights and y-z products, from the bin's cell), then one
+// pass per (component, x-slice) that sums the markers' S^2 contributions exactly in biased
+// float64 registers (exact fma, see odd_significand) and adds them to the LDS accumulator.
+// A pass holds S^2 registers instead of C S^3.  Slots past `count` hold a copy of the first
+// marker with its value scaled by zero: u = odd_significand(0) is the least subnormal, and
+// fma(u, w, acc) = acc exactly, so the passes have no per-marker branches.  With ONE, every
+// marker is its own pass (for scales under which fewer than ITEM markers fit one register
+// sum); the integers are the same either way.
+template <int KIND, int P, int C, int S, int L, int LY, int LZ, bool ONE, class T, class Between>
+__device__ __forceinline__ void deposit_markers(unsigned long long *acc, int base, const Ahead<T, C> &mk,
+                                                uint32_t count, const double (&cell)[3], const AxisRange &ar,
+                                                const Grid &g, int num0, int num1, int num2, double scale,
+                                                Between &&between) {
+  double wx[ITEM][S], wyz[ITEM][S][S], sc[ITEM];
+#pragma unroll
+  for (int r = 0; r < ITEM; ++r) {
+    double w0[4], w1[4], w2[4];
+    cell_weights_f64<KIND, P>(((double)mk.x[r] - g.lower[0]) * g.inverse[0], cell[0], ar.hi[0], num0, g.code[0], w0);
+    cell_weights_f64<KIND, P>(((double)mk.y[r] - g.lower[1]) * g.inverse[1], cell[1], ar.hi[1], num1, g.code[1], w1);
+    cell_weights_f64<KIND, P>(((double)mk.z[r] - g.lower[2]) * g.inverse[2], cell[2], ar.hi[2], num2, g.code[2], w2);
+#pragma unroll
+    for (int i = 0; i < S; ++i) wx[r][i] = w0[i];
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+#pragma unroll
+      for (int k = 0; k < S; ++k) wyz[r][j][k] = w1[j] * w2[k];
+    sc[r] = (r == 0 || (uint32_t)r < count) ? scale : 0.0;
+  }
+  // LDS reads the caller wants before this deposit's atomics are queued (they complete in
+  // order): the weights took long enough for the previous deposit's atomics to drain
+  between();
 #pragma unroll
   for (int q = 0; q < C; ++q) {
-    const double vs = v[q] * scale;
+    double vs[ITEM];
+#pragma unroll
+    for (int r = 0; r < ITEM; ++r) vs[r] = (double)mk.v[r][q] * sc[r];
 #pragma unroll
     for (int i = 0; i < S; ++i) {
-      const double u = odd_significand(wx[i] * vs);
+      const int at = q * L + base + i * LY * LZ;
+      double a[S][S];
 #pragma unroll
-      for (int j = 0; j < S; ++j)
+      for (int r = 0; r < ITEM; ++r) {
+        const double u = odd_significand(wx[r][i] * vs[r]);
 #pragma unroll
-        for (int k = 0; k < S; ++k) a[q][i][j][k] = __builtin_fma(u, wyz[j][k], FIRST ? BIAS : a[q][i][j][k]);
+        for (int j = 0; j < S; ++j)
+#pragma unroll
+          for (int k = 0; k < S; ++k) a[j][k] = __builtin_fma(u, wyz[r][j][k], (r == 0 || ONE) ? BIAS : a[j][k]);
+        if (ONE && r + 1 < ITEM) flush_slice<S, LY, LZ>(acc, at, a);
+      }
+      flush_slice<S, LY, LZ>(acc, at, a);
     }
   }
-}
-
-// The register sums back to zero (the bias) after a flush.
-template <int C, int S>
-__device__ __forceinline__ void rebias(double (&a)[C][S][S][S]) {
-#pragma unroll
-  for (int q = 0; q < C; ++q)
-#pragma unroll
-    for (int i = 0; i < S; ++i)
-#pragma unroll
-      for (int j = 0; j < S; ++j)
-#pragma unroll
-        for (int k = 0; k < S; ++k) a[q][i][j][k] = BIAS;
-}
-
-// The register sums of an item into the tile's LDS accumulator.  Stencil node (i, j, k) of
-// the cell whose footprint position is `base` sits at base + (i * LY + j) * LZ + k.
-template <int C, int S, int L, int LY, int LZ>
-__device__ __forceinline__ void flush_item(unsigned long long *acc, int base, const double (&a)[C][S][S][S]) {
-#pragma unroll
-  for (int q = 0; q < C; ++q)
-#pragma unroll
-    for (int i = 0; i < S; ++i)
-#pragma unroll
-      for (int j = 0; j < S; ++j)
-#pragma unroll
-        for (int k = 0; k < S; ++k)
-          atomicAdd(&acc[q * L + base + (i * LY + j) * LZ + k], (unsigned long long)biased_to_int(a[q][i][j][k]));
 }
 
 // The field index (times C) of node (gx, gy, z0 - OFF) for each footprint row (ix, iy) of a
@@ -806,7 +909,7 @@ __device__ __forceinline__ int footprint_z(const Grid &g, int gz, bool clamp) {
 // accumulator, which spans the tile's footprint before any wrap or clamp.  The markers of
 // the next item are loaded while the current one is deposited.  Nodes beyond a bounded wall
 // are folded onto the wall node, and the tile is then added into the field once.
-template <class T, int KIND, int P, int C, int BLOCK>
+template <class T, int KIND, int P, int C, int BLOCK, bool SMALL>
 __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ pos, const T *__restrict__ val,
                                                           int64_t n, const uint32_t *__restrict__ start, Grid g,
                                                           Tiling tl, double *__restrict__ field,
@@ -846,73 +949,61 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
   const uint32_t total = tables ? table_counts[tile_id] : build_items<BLOCK, TC>(cfirst, items, wave_sums);
 
   const double scale = slot[0];
-  // markers per register pass: the exact sums stay below 2^51 quanta
-  const uint32_t chunk = slot[2] < 4.0e9 ? (slot[2] >= 1.0 ? (uint32_t)slot[2] : 1u) : 4000000000u;
+  // markers per register pass (the exact sums stay below 2^51 quanta): below ITEM only for a
+  // scale fixed by the caller, when every marker gets its own pass
+  const bool one = slot[2] < (double)ITEM;
 
   // The lookahead: the markers of the next item load while the current one is deposited.
   // The loads are unconditional (past the last item they repeat it) and the first ones are
   // drained before the loop, so the wait before a deposit never covers the next item's
-  // loads.  Item descriptors are read from LDS one item further ahead, between a deposit
-  // and its flush: LDS operations complete in order, so a descriptor read issued after a
-  // flush would wait for its 81 atomics.
+  // loads.  The descriptor of the item after next is read from LDS before the current
+  // deposit's atomics are queued (LDS operations of a wavefront complete in order).
+  MarkerRows<T, C> mrows;
+  mrows.x = pos;
+  mrows.y = pos + n;
+  mrows.z = pos + 2 * n;
+#pragma unroll
+  for (int q = 0; q < C; ++q) mrows.v[q] = val + q * n;
+  const AxisRange ar = axis_ranges(g);
   const uint32_t last = total > 0 ? total - 1 : 0;
   uint32_t it = t;
   Item cur{0, 0, 0}, nxt{0, 0, 0};
   Ahead<T, C> now;
   if (total > 0) {
     cur = item_at(items, cfirst, it < total ? it : last);
-    load_ahead<T, C>(pos, val, n, cur, now);
+    load_ahead<SMALL>(mrows, cur.begin, cur.size, now);
     nxt = item_at(items, cfirst, it + BLOCK < total ? it + BLOCK : last);
     wait_loads();
   }
   while (it < total) {
     Ahead<T, C> next;
-    load_ahead<T, C>(pos, val, n, nxt, next);
-    const int c = cur.cell;
-    const int base = ((c / (TY * TZ)) * LY + (c / TZ) % TY) * LZ + c % TZ;
-    double a[C][S][S][S];
-    {
-      double v[C];
-#pragma unroll
-      for (int q = 0; q < C; ++q) v[q] = (double)now.v[0][q];
-      deposit_marker<KIND, P, C, S, true>(a, (double)now.x[0], (double)now.y[0], (double)now.z[0], v, g, num0, num1,
-                                          num2, scale);
-    }
-    uint32_t held = 1;  // markers in the registers
-#pragma unroll
-    for (int r = 1; r < ITEM; ++r) {
-      if ((uint32_t)r < cur.size) {
-        if (held == chunk) {
-          flush_item<C, S, L, LY, LZ>(acc, base, a);
-          rebias<C, S>(a);
-          held = 0;
-        }
-        double v[C];
-#pragma unroll
-        for (int q = 0; q < C; ++q) v[q] = (double)now.v[r][q];
-        deposit_marker<KIND, P, C, S, false>(a, (double)now.x[r], (double)now.y[r], (double)now.z[r], v, g, num0,
-                                             num1, num2, scale);
-        ++held;
-      }
-    }
-    // items of more than ITEM markers (cells of more than ITEM * SWEEPS markers)
-    for (uint32_t r = ITEM; r < cur.size; ++r) {
-      if (held == chunk) {
-        flush_item<C, S, L, LY, LZ>(acc, base, a);
-        rebias<C, S>(a);
-        held = 0;
-      }
-      const int64_t m = cur.begin + r;
-      double v[C];
-#pragma unroll
-      for (int q = 0; q < C; ++q) v[q] = (double)val[q * n + m];
-      deposit_marker<KIND, P, C, S, false>(a, (double)pos[m], (double)pos[n + m], (double)pos[2 * n + m], v, g, num0,
-                                           num1, num2, scale);
-      ++held;
-    }
+    load_ahead<SMALL>(mrows, nxt.begin, nxt.size, next);
     const uint32_t after = it + 2 * BLOCK;
-    const Item nn = item_at(items, cfirst, after < total ? after : last);
-    flush_item<C, S, L, LY, LZ>(acc, base, a);
+    Item nn;
+    const int c = cur.cell;
+    const int cx = c / (TY * TZ), cy = (c / TZ) % TY, cz = c % TZ;
+    const int base = (cx * LY + cy) * LZ + cz;
+    const double cell[3] = {(double)(x0 + cx), (double)(y0 + cy), (double)(z0 + cz)};
+    const uint32_t first_count = cur.size < (uint32_t)ITEM ? cur.size : ITEM;
+    auto read_next = [&]() { nn = item_at(items, cfirst, after < total ? after : last); };
+    if (one)
+      deposit_markers<KIND, P, C, S, L, LY, LZ, true>(acc, base, now, first_count, cell, ar, g, num0, num1, num2,
+                                                      scale, read_next);
+    else
+      deposit_markers<KIND, P, C, S, L, LY, LZ, false>(acc, base, now, first_count, cell, ar, g, num0, num1, num2,
+                                                       scale, read_next);
+    // items of more than ITEM markers (cells of more than ITEM * SWEEPS markers), ITEM at a time
+    for (uint32_t r0 = ITEM; r0 < cur.size; r0 += ITEM) {
+      Ahead<T, C> more;
+      load_ahead<SMALL>(mrows, cur.begin + r0, cur.size - r0, more);
+      const uint32_t k = cur.size - r0 < (uint32_t)ITEM ? cur.size - r0 : ITEM;
+      if (one)
+        deposit_markers<KIND, P, C, S, L, LY, LZ, true>(acc, base, more, k, cell, ar, g, num0, num1, num2, scale,
+                                                        []() {});
+      else
+        deposit_markers<KIND, P, C, S, L, LY, LZ, false>(acc, base, more, k, cell, ar, g, num0, num1, num2, scale,
+                                                         []() {});
+    }
     cur = nxt;
     nxt = nn;
     now = next;
@@ -1120,23 +1211,6 @@ struct NodeSlot<1> {
   using type = float;
 };
 
-// The bounds of rel on one axis within which a marker's stencil needs neither a periodic
-// wrap nor a wall clamp: [0, num) on a periodic axis, [0, num - 1] on a bounded one; none
-// on a clamped axis (hi < lo).
-struct AxisRange {
-  double hi[3];
-};
-
-__device__ __forceinline__ AxisRange axis_ranges(const Grid &g) {
-  AxisRange r;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double nd = (double)g.num[d];
-    r.hi[d] = g.code[d] == 1 ? nd : (g.code[d] == 0 ? nd - 1.0 : -1.0);
-  }
-  return r;
-}
-
 // The float32 weights of rel on one axis for a marker binned into stencil cell `cell`
 // (in [0, num)).  Where the position needs neither a periodic wrap nor a wall clamp
 // (0 <= rel <= hi; rel < num holds on a periodic axis once |o| <= 1/2), the reference's
@@ -1147,20 +1221,14 @@ __device__ __forceinline__ AxisRange axis_ranges(const Grid &g) {
 template <int KIND, int P>
 __device__ __forceinline__ void cell_weights_f32(double rel, double cell, double hi, int num, int code, float *w) {
   const double o = rel - cell;
-  if (KIND == LINEAR) {
-    const bool fast = (code == 0) | ((rel >= 0.0) & (rel < hi) & (o >= 0.0) & (o < 1.0));
-    if (fast) {
-      const float f = (float)(o < 0.0 ? 0.0 : (o > 1.0 ? 1.0 : o));
-      w[0] = 1.0f - f;
-      w[1] = f;
-    } else {
-      weights_f32<KIND, P>(rel, num, code, w);
-    }
+  if (!offset_in_cell<KIND, P>(rel, o, hi, code)) {
+    weights_f32<KIND, P>(rel, num, code, w);
     return;
   }
-  const bool near = (P % 2 == 0) ? (fabs(o) <= 0.5) : ((o >= 0.0) & (o < 1.0));
-  if (!((rel >= 0.0) & (rel <= hi) & near)) {
-    weights_f32<KIND, P>(rel, num, code, w);
+  if (KIND == LINEAR) {
+    const float f = (float)(o < 0.0 ? 0.0 : (o > 1.0 ? 1.0 : o));
+    w[0] = 1.0f - f;
+    w[1] = f;
     return;
   }
   const float f = (float)o;
@@ -1221,20 +1289,6 @@ __device__ __forceinline__ void interpolate_marker(const typename NodeSlot<C>::t
   }
 #pragma unroll
   for (int q = 0; q < C; ++q) out[q] = s[q];
-}
-
-// A float32 marker row entry by binned index: with SMALL (every row of the binned arrays
-// below 4 GiB from its base) the byte offset is a 32-bit value added to the row's base,
-// which the compiler keeps in scalar registers (no 64-bit address arithmetic per load).
-template <bool SMALL>
-__device__ __forceinline__ const float *row_at(const float *row, int64_t m) {
-  if constexpr (SMALL) return (const float *)((const char *)row + (uint32_t)((uint32_t)m * 4u));
-  else return row + m;
-}
-template <bool SMALL>
-__device__ __forceinline__ float *row_at(float *row, int64_t m) {
-  if constexpr (SMALL) return (float *)((char *)row + (uint32_t)((uint32_t)m * 4u));
-  else return row + m;
 }
 
 struct PosAhead {
@@ -2124,9 +2178,13 @@ static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const 
         const int blocks = tl.tiles_x * tl.tiles_y * tl.tiles_z;
         if (blocks == 0) continue;
         ProfileScope prof("scatter_tiled", stream);
-        hipLaunchKernelGGL((scatter_tiled<T, KIND, P, C, BLOCK>), dim3(blocks),
-                           dim3(BLOCK), 0, stream,
-                           (const T *)pos, (const T *)val, n, start, g, tl, field, scalars, tables, table_counts);
+        // 32-bit row offsets while every row ends below 4 GiB from its base
+        if ((uint64_t)n * sizeof(T) < (1ull << 32))
+          hipLaunchKernelGGL((scatter_tiled<T, KIND, P, C, BLOCK, true>), dim3(blocks), dim3(BLOCK), 0, stream,
+                             (const T *)pos, (const T *)val, n, start, g, tl, field, scalars, tables, table_counts);
+        else
+          hipLaunchKernelGGL((scatter_tiled<T, KIND, P, C, BLOCK, false>), dim3(blocks), dim3(BLOCK), 0, stream,
+                             (const T *)pos, (const T *)val, n, start, g, tl, field, scalars, tables, table_counts);
         MPF_LAUNCH_CHECK();
       }
   return MPF_OK;
