@@ -72,7 +72,15 @@ const char *mpf_last_error(void);
 /* ABI version, bumped on any signature change. */
 int mpf_abi_version(void);
 /* A context on `device` (its scratch grows on demand); NULL on failure.  A NULL context
- * argument anywhere below means the library's default context of the current device. */
+ * argument anywhere below means the library's default context of the (current device,
+ * stream) pair, made on first use: calls on different streams never share scratch.
+ *
+ * One stream per context: a context's scratch carries state between the launches of a call
+ * (the fixed-point accumulator, the scale slot, bins made inside a call), so calls that
+ * share a context must be issued on one stream, or ordered by the caller.  A call that
+ * needs more scratch than its context holds synchronizes that context's stream and
+ * reallocates; such a call cannot be captured in a HIP graph, so run the work once on the
+ * capture stream before capturing it. */
 mpf_context *mpf_context_create(int device);
 void mpf_context_destroy(mpf_context *ctx);
 
@@ -87,13 +95,19 @@ int mpf_profile_query(const char *name, double *total_ms, int64_t *launches);
 /* ---- generic node kernels -------------------------------------------------------- */
 
 /* kind MPF_KIND_*, degree 0..3 (spline only), ndim 1..3; lower/cell_size/num_nodes/code
- * are host arrays of ndim entries.  Replaces every scatter_* kernel of catalog.py. */
+ * are host arrays of ndim entries.  Replaces every scatter_* kernel of catalog.py.
+ * A fixed-point deposit with the automatic scale of 2^18 to 2^31 - 1 markers, on a shape the
+ * tiled kernels take (2D; 3D with whole periodic tiles, 1 or 3 components), is binned by
+ * stencil cell inside the call (bins in the context) and deposited tile by tile: the same
+ * integers as the direct kernels. */
 int mpf_scatter(int kind, int degree, int ndim, const void *positions, int64_t pos_stride, const void *values,
                 int64_t val_stride, int dtype, int64_t n, int64_t ncomp, double *field, const double *lower,
                 const double *cell_size, const int64_t *num_nodes, const int64_t *code, int mode, int scale_exp,
                 mpf_context *ctx, mpf_stream stream);
 
-/* Replaces every gather_* kernel of catalog.py; result has the marker dtype. */
+/* Replaces every gather_* kernel of catalog.py; result has the marker dtype.  Batches of
+ * 2^18 to 2^31 - 1 markers on the tiled shapes are binned inside the call, gathered tile by
+ * tile and written back in the input order. */
 int mpf_gather(int kind, int degree, int ndim, const void *positions, int64_t pos_stride, int dtype, int64_t n,
                const double *field, int64_t ncomp, void *result, int64_t result_stride, const double *lower,
                const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,

@@ -10,15 +10,17 @@ The gather of a weighting is the transpose of its scatter (weighting.py:17-20), 
 Positions, values and fields are float64 or float32 GPU tensors; the field is float64.
 The deposits in the backward use the current deposit mode (kernels.deposit_mode).
 
-Large 3D batches on the tiled shapes (kernels.tiled_ok) bin their positions once: the bins
-of the last few position tensors are kept (by tensor identity and version), so that a
-gather of a scatter at the same positions, and both backward passes, sort the markers
-once and use the tiled deposit, gather and gradient kernels.
+Large 3D batches on the tiled shapes (kernels.binned_route) bin their positions and use the
+tiled deposit, gather and gradient kernels; a Function keeps its bins from the forward pass
+for its backward pass.  To share one binning between several Functions (a gather of a
+scatter at the same positions), make the bins once with ``bin_positions`` and pass them as
+``bins=``: the caller owns them and promises the positions are unchanged while they are in
+use.  Nothing is reused implicitly, so a position tensor written in place by any route (an
+in-place op, ``.data``, a pointer handed to a particle pusher) is binned afresh by the next
+call that is not given bins.
 """
 
 from __future__ import annotations
-
-import weakref
 
 import torch
 
@@ -27,26 +29,36 @@ from mpifield_amd import binning
 from mpifield_amd import kernels as K
 from mpifield_amd.binning import kind_of
 
-#: (weakref to the positions, their version, kind, degree, geometry, bins), newest last
-_BINS: list = []
-_BINS_KEPT = 2
 
-
-def positions_bins(positions: torch.Tensor, kind: int, degree: int, geometry, ncomp: int):
-    """Bins of ``positions`` (made from positions alone) for the tiled kernels, or None when
-    the batch is small or the shape has no tiled kernel.  Reused while the same tensor is
-    unchanged (same object, same version counter)."""
+def _positions_bins(positions: torch.Tensor, kind: int, degree: int, geometry, ncomp: int):
+    """Fresh bins of ``positions`` (made from positions alone, with their work lists) for the
+    tiled kernels, or None when the batch or the shape has no tiled kernel."""
     ndim, _, _, num, code, _ = K._split(geometry)
-    if ndim != 3 or positions.shape[1] < K.BINNED_MIN_MARKERS or not K.tiled_ok(kind, degree, ncomp, num, code):
+    if ndim != 3 or not K.binned_route(kind, degree, ncomp, num, code, positions.shape[1]):
         return None
-    key = (kind, degree, tuple(geometry))
-    for ref, version, k, bins in _BINS:
-        if ref() is positions and version == positions._version and k == key:
-            return bins
-    bins = binning.bin_markers(positions.detach(), None, (kind, degree), geometry).prepare_items()
-    _BINS.append((weakref.ref(positions), positions._version, key, bins))
-    del _BINS[:-_BINS_KEPT]
-    return bins
+    return binning.bin_markers(positions.detach(), None, (kind, degree), geometry).prepare_items()
+
+
+def bin_positions(positions: torch.Tensor, weighting, grid, ncomp: int = 1):
+    """Bins of ``positions`` for ``weighting`` on ``grid``, to share between the differentiable
+    gather and scatter at these positions (``bins=``), or None when the batch takes the
+    direct kernels.  The bins hold a sorted copy of the positions: they stay valid only while
+    the positions are unchanged."""
+    kind, degree = kind_of(weighting)
+    return _positions_bins(positions, kind, degree, tuple(grid.geometry), ncomp)
+
+
+def _check_bins(bins, positions: torch.Tensor, kind: int, degree: int, geometry, ncomp: int):
+    """The caller's bins where the tiled kernels take ``ncomp`` components, else None (the
+    direct kernels); errors for bins of other markers, weightings or grids."""
+    if bins is None:
+        return None
+    if not isinstance(bins, binning.MarkerBins) or bins.values is not None:
+        raise TypeError("bins must be MarkerBins made from positions alone (bin_positions)")
+    if (bins.kind, bins.degree, tuple(bins.geometry)) != (kind, degree, tuple(geometry)) or bins.n != positions.shape[1]:
+        raise ValueError("these bins were made for another weighting, grid or number of markers")
+    _, _, _, num, code, _ = K._split(geometry)
+    return bins if K.tiled_ok(kind, degree, ncomp, num, code) else None
 
 
 def _binned_deposit(bins, values: torch.Tensor, field: torch.Tensor) -> bool:
@@ -77,18 +89,22 @@ def gather_derivative(kind: int, degree: int, positions: torch.Tensor, field: to
     return out
 
 
-def _shape(geometry, ncomp: int) -> tuple[int, ...]:
-    ndim = len(geometry) // 4
-    return (*(int(v) for v in geometry[2 * ndim : 3 * ndim]), ncomp)
+def _shape(geometry, kind: int, degree: int, ncomp: int) -> tuple[int, ...]:
+    """The field a deposit of this weighting fills: num_nodes, plus degree - 1 on a clamped axis."""
+    _, _, _, num, code, _ = K._split(geometry)
+    deg = {_lib.KIND_LINEAR: 1, _lib.KIND_QUADRATIC: 2}.get(kind, degree)
+    return (*K._expected_extent(num, code, deg, kind), ncomp)
 
 
 class GatherFunction(torch.autograd.Function):
     """g = gather(positions, field) with the adjoint backward."""
 
     @staticmethod
-    def forward(ctx, positions, field, kind, degree, geometry):
+    def forward(ctx, positions, field, kind, degree, geometry, bins=None):
         field_c = field.contiguous()
-        bins = positions_bins(positions, kind, degree, geometry, field.shape[-1])
+        bins = _check_bins(bins, positions, kind, degree, geometry, field.shape[-1])
+        if bins is None:
+            bins = _positions_bins(positions, kind, degree, geometry, field.shape[-1])
         if bins is not None:
             result = _binned_gather(bins, field_c, positions.dtype)
         else:
@@ -116,16 +132,19 @@ class GatherFunction(torch.autograd.Function):
             else:
                 d = gather_derivative(kind, degree, positions.detach(), field.detach().contiguous(), geometry)
                 grad_pos = (d * grad.to(d.dtype).unsqueeze(0)).sum(dim=1)
-        return grad_pos, grad_field, None, None, None
+        return grad_pos, grad_field, None, None, None, None
 
 
 class ScatterFunction(torch.autograd.Function):
     """rho = scatter(positions, values) into a zero field, with the adjoint backward."""
 
     @staticmethod
-    def forward(ctx, positions, values, kind, degree, geometry):
-        field = torch.zeros(_shape(geometry, values.shape[0]), dtype=torch.float64, device=values.device)
-        bins = positions_bins(positions, kind, degree, geometry, values.shape[0])
+    def forward(ctx, positions, values, kind, degree, geometry, bins=None):
+        field = torch.zeros(_shape(geometry, kind, degree, values.shape[0]), dtype=torch.float64,
+                            device=values.device)
+        bins = _check_bins(bins, positions, kind, degree, geometry, values.shape[0])
+        if bins is None:
+            bins = _positions_bins(positions, kind, degree, geometry, values.shape[0])
         if not _binned_deposit(bins, values, field):
             K.scatter(kind, degree, positions, values, field, geometry)
         ctx.save_for_backward(positions, values)
@@ -153,17 +172,19 @@ class ScatterFunction(torch.autograd.Function):
             else:
                 d = gather_derivative(kind, degree, positions.detach(), grad, geometry)
                 grad_pos = (d * values.detach().to(d.dtype).unsqueeze(0)).sum(dim=1)
-        return grad_pos, grad_val, None, None, None
+        return grad_pos, grad_val, None, None, None, None
 
 
-def gather(positions: torch.Tensor, field: torch.Tensor, weighting, grid) -> torch.Tensor:
-    """Differentiable ``weighting`` gather of ``field`` (num_nodes + (ncomp,)) at ``positions`` (ndim, N)."""
+def gather(positions: torch.Tensor, field: torch.Tensor, weighting, grid, bins=None) -> torch.Tensor:
+    """Differentiable ``weighting`` gather of ``field`` (num_nodes + (ncomp,)) at ``positions`` (ndim, N);
+    ``bins`` from bin_positions of the same positions, or None to bin them here."""
     kind, degree = kind_of(weighting)
-    return GatherFunction.apply(positions, field, kind, degree, tuple(grid.geometry))
+    return GatherFunction.apply(positions, field, kind, degree, tuple(grid.geometry), bins)
 
 
-def scatter(positions: torch.Tensor, values: torch.Tensor, weighting, grid) -> torch.Tensor:
-    """Differentiable ``weighting`` deposit of ``values`` (ncomp, N) at ``positions`` into a new zero field."""
+def scatter(positions: torch.Tensor, values: torch.Tensor, weighting, grid, bins=None) -> torch.Tensor:
+    """Differentiable ``weighting`` deposit of ``values`` (ncomp, N) at ``positions`` into a new zero field
+    (num_nodes + degree - 1 on a clamped axis); ``bins`` as for gather."""
     kind, degree = kind_of(weighting)
-    return ScatterFunction.apply(positions, values, kind, degree, tuple(grid.geometry))
+    return ScatterFunction.apply(positions, values, kind, degree, tuple(grid.geometry), bins)
 

@@ -5,13 +5,13 @@ A particle code that keeps its markers sorted by cell between steps (what the re
 banded CPU buffers assume, scatter_linear_3d_pyccel.py:11-16) bins once and deposits and
 gathers many times::
 
-    bins = bin_markers(positions, values, QUADRATIC, grid)
+    bins = bin_markers(positions, values, kind_of(QUADRATIC), grid.geometry)
     scatter_binned(bins, field)              # adds into field, bit-reproducible
     gather_binned(bins, field, result)       # result (ncomp, N) in bins order
 
 The forms deposit (faces, cells) reads bins of the spline nodes of its degree::
 
-    bins = bin_markers(positions, values, (KIND_SPLINE, degree), grid)
+    bins = bin_markers(positions, values, (KIND_SPLINE, degree), grid.geometry)
     scatter_forms_binned(bins, faces, form=2)
     gather_forms_binned(bins, faces, result, form=2)   # result (ncomp, N) in bins order
 
@@ -35,11 +35,13 @@ _KINDS = {("linear", 1): (0, 1), ("quadratic", 2): (1, 2), ("cubic", 3): (2, 3)}
 
 def kind_of(weighting) -> tuple[int, int]:
     """The kernel kind and degree of a built-in weighting (ValueError for others)."""
+    # the clamping weightings run on the spline kernels (the only ones that take clamped
+    # axes), whatever their name
+    if weighting.degree is not None and weighting.clamps:
+        return (2, int(weighting.degree))
     key = (weighting.name, weighting.degree)
     if key in _KINDS:
         return _KINDS[key]
-    if weighting.degree is not None and weighting.clamps:
-        return (2, int(weighting.degree))
     raise ValueError(f"the {weighting.name} weighting has no binned kernels")
 
 
@@ -150,14 +152,30 @@ class ctypes_int64:
         return int(self.array[0])
 
 
+def _bins_field(bins: MarkerBins, field, ncomp: int | None = None, form: int | None = None) -> torch.Tensor:
+    """``field`` checked against the bins' grid and device before any launch: the tiled
+    kernels index it by the grid's node counts, so a field of another grid would be read or
+    written out of bounds.  ``ncomp`` None takes the field's own component count (a
+    gather); ``form`` checks a k-form field of the bins' degree instead of a node field."""
+    field = K._field(field)
+    _, _, _, num, code = _geometry_arrays(bins.geometry)
+    if form is not None:
+        field = K._form_field(field, bins.degree, form, num, code)
+    else:
+        want = (*(int(v) for v in num), field.shape[-1] if ncomp is None else ncomp)
+        if tuple(field.shape) != want:
+            raise ValueError(f"a field on this grid is {want}, got {tuple(field.shape)}")
+    if field.device != bins.positions.device:
+        raise ValueError(f"the field is on {field.device}, the bins on {bins.positions.device}")
+    return field
+
+
 def scatter_binned(bins: MarkerBins, field: torch.Tensor) -> None:
     """Add the binned markers' values into ``field`` (num_nodes + (ncomp,)) with the tiled kernels."""
     if bins.values is None:
         raise ValueError("these bins hold no values (binned for a gather)")
-    field = K._field(field)
+    field = _bins_field(bins, field, bins.ncomp)
     ndim, lower, cell, num, code = _geometry_arrays(bins.geometry)
-    if tuple(field.shape) != (*num, bins.ncomp):
-        raise ValueError(f"a field on this grid is {(*num, bins.ncomp)}, got {tuple(field.shape)}")
     with torch.cuda.device(field.device):
         _lib.call("mpf_scatter_binned", bins.kind, bins.degree, ndim, bins.positions.data_ptr(),
                   bins.values.data_ptr(), K._dtype_code(bins.positions), bins.n, bins.ncomp, bins.start.data_ptr(),
@@ -167,11 +185,10 @@ def scatter_binned(bins: MarkerBins, field: torch.Tensor) -> None:
 
 def gather_binned(bins: MarkerBins, field: torch.Tensor, result: torch.Tensor) -> None:
     """Interpolate ``field`` at the binned markers into ``result`` (ncomp, N) in bins order."""
-    field = K._field(field)
+    field = _bins_field(bins, field)
     ndim, lower, cell, num, code = _geometry_arrays(bins.geometry)
     ncomp = field.shape[-1]
-    if tuple(result.shape) != (ncomp, bins.n) or not result.is_contiguous() or result.dtype != bins.positions.dtype:
-        raise ValueError(f"result must be a contiguous ({ncomp}, {bins.n}) {bins.positions.dtype} tensor")
+    _check_result(bins, result, ncomp)
     with torch.cuda.device(field.device):
         # the gather builds its tiles' work lists itself: the build hides behind its footprint
         # load, and loading a list costs it registers (measured slower with the lists)
@@ -199,10 +216,17 @@ def with_values(bins: MarkerBins, values, scale_exp: int = _lib.SCALE_AUTO) -> M
                       bins.tables, bins.table_counts)
 
 
+def _check_result(bins: MarkerBins, result, ncomp: int) -> None:
+    if (not isinstance(result, torch.Tensor) or tuple(result.shape) != (ncomp, bins.n) or not result.is_contiguous()
+            or result.dtype != bins.positions.dtype or result.device != bins.positions.device):
+        raise ValueError(f"result must be a contiguous ({ncomp}, {bins.n}) {bins.positions.dtype} tensor on "
+                         f"{bins.positions.device}")
+
+
 def gather_derivative_binned(bins: MarkerBins, field: torch.Tensor, coef: torch.Tensor) -> torch.Tensor:
     """sum_c coef[c] * grad F_c at the binned markers: (ndim, N) in the input order, with
     ``coef`` (ncomp, N) in the input order and ``field`` num_nodes + (ncomp,)."""
-    field = K._field(field)
+    field = _bins_field(bins, field)
     ndim, lower, cell, num, code = _geometry_arrays(bins.geometry)
     ncomp = field.shape[-1]
     k, ks = K.marker_rows(coef, ncomp, bins.positions.dtype)
@@ -225,7 +249,9 @@ def scatter_forms_binned(bins: MarkerBins, field: torch.Tensor, form: int) -> No
         raise ValueError("these bins hold no values (binned for a gather)")
     if bins.kind != _lib.KIND_SPLINE:
         raise ValueError("the forms deposit reads bins of the spline nodes (kind 2)")
-    field = K._field(field)
+    field = _bins_field(bins, field, form=form)
+    if field.shape[-1] != bins.ncomp:
+        raise ValueError(f"a {form}-form field has {field.shape[-1]} components, the bins {bins.ncomp} values")
     ndim, lower, cell, num, code = _geometry_arrays(bins.geometry)
     with torch.cuda.device(field.device):
         _lib.call("mpf_scatter_forms_binned", bins.degree, form, ndim, bins.positions.data_ptr(),
@@ -239,11 +265,10 @@ def gather_forms_binned(bins: MarkerBins, field: torch.Tensor, result: torch.Ten
     (mpf_gather_forms_binned): gather_forms_spline_*d's values bit for bit."""
     if bins.kind != _lib.KIND_SPLINE:
         raise ValueError("the forms gather reads bins of the spline nodes (kind 2)")
-    field = K._field(field)
+    field = _bins_field(bins, field, form=form)
     ndim, lower, cell, num, code = _geometry_arrays(bins.geometry)
     ncomp = field.shape[-1]
-    if tuple(result.shape) != (ncomp, bins.n) or not result.is_contiguous() or result.dtype != bins.positions.dtype:
-        raise ValueError(f"result must be a contiguous ({ncomp}, {bins.n}) {bins.positions.dtype} tensor")
+    _check_result(bins, result, ncomp)
     with torch.cuda.device(field.device):
         _lib.call("mpf_gather_forms_binned", bins.degree, form, ndim, bins.positions.data_ptr(),
                   K._dtype_code(bins.positions), bins.n, bins.start.data_ptr(), field.data_ptr(), ncomp,

@@ -2092,7 +2092,7 @@ extern "C" int mpf_bin_markers(int kind, int degree, int ndim, const void *posit
   if (blocks > BIN_MAX_BLOCKS) blocks = BIN_MAX_BLOCKS;
   const int64_t chunk = n > 0 ? (n + blocks - 1) / blocks : 1;
   hipStream_t stream = (hipStream_t)stream_;
-  if (!ctx) ctx = default_context();
+  if (!ctx) ctx = default_context(stream);
   if (!ctx) return MPF_ERR_HIP;
   // scratch: the keys in input order, the markers as packed records in group order, the
   // per-(group, block) counts and their scan
@@ -2257,7 +2257,7 @@ extern "C" int mpf_scatter_binned(int kind, int degree, int ndim, const void *so
   if (n == 0) return MPF_OK;  // no markers: nothing to add, no results
   hipStream_t stream = (hipStream_t)stream_;
   if (tiled_2d_supported(kind, degree, ndim, ncomp, g)) {
-    if (!ctx) ctx = default_context();
+    if (!ctx) ctx = default_context(stream);
     if (!ctx) return MPF_ERR_HIP;
 #define SC2(T, K, P, C) run_scatter_2d<T, K, P, C>(sorted_positions, sorted_values, n, start, g, field, scalars, ctx, stream)
     if (dtype == F32) {
@@ -2389,7 +2389,7 @@ extern "C" int mpf_fixed_scale(const void *values, int64_t val_stride, int dtype
   }
   hipStream_t stream = (hipStream_t)stream_;
   if (n == 0) return empty_bins(nullptr, 0, scalars, stream);
-  if (!ctx) ctx = default_context();
+  if (!ctx) ctx = default_context(stream);
   if (!ctx) return MPF_ERR_HIP;
   int rc = prepare_scale(ctx, values, val_stride, dtype, ncomp, n, 1.0, scale_exp, stream);
   if (rc) return rc;
@@ -2423,11 +2423,11 @@ extern "C" int mpf_bin_items(int ndim, const int64_t *num_nodes, const uint32_t 
 // rows at once; 3 rows of 2^25: 5.2 against 5.6 the other way round).
 template <class T>
 __global__ void __launch_bounds__(256) unsort_rows_kernel(const T *__restrict__ sorted, T *__restrict__ out,
-                                                          const int64_t *__restrict__ order, int64_t rows,
-                                                          int64_t n) {
+                                                          int64_t out_stride, const int64_t *__restrict__ order,
+                                                          int64_t rows, int64_t n) {
   for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x) {
     const int64_t o = order[m];
-    for (int64_t c = 0; c < rows; ++c) out[c * n + o] = sorted[c * n + m];
+    for (int64_t c = 0; c < rows; ++c) out[c * out_stride + o] = sorted[c * n + m];
   }
 }
 
@@ -2437,6 +2437,103 @@ __global__ void __launch_bounds__(256) unsort_row_kernel(const T *__restrict__ s
   for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x)
     out[order[m]] = sorted[m];
 }
+
+namespace mpf {
+
+// Rows (rows, n) in bins order, contiguous, back to the input order into rows out_stride
+// elements apart.
+static int unsort_rows_strided(const void *sorted, void *out, int64_t out_stride, const int64_t *order, int64_t rows,
+                               int64_t n, int dtype, hipStream_t stream) {
+  if (rows == 0 || n == 0) return MPF_OK;
+  const int blocks = blocks_for(n, 256, 1 << 16);
+  const int64_t bytes = rows * n * (dtype == F32 ? 4 : 8);
+  if (bytes <= (int64_t)512 << 20) {
+    if (dtype == F32)
+      hipLaunchKernelGGL(unsort_rows_kernel<float>, dim3(blocks), dim3(256), 0, stream, (const float *)sorted,
+                         (float *)out, out_stride, order, rows, n);
+    else
+      hipLaunchKernelGGL(unsort_rows_kernel<double>, dim3(blocks), dim3(256), 0, stream, (const double *)sorted,
+                         (double *)out, out_stride, order, rows, n);
+    MPF_LAUNCH_CHECK();
+    return MPF_OK;
+  }
+  for (int64_t c = 0; c < rows; ++c) {
+    if (dtype == F32)
+      hipLaunchKernelGGL(unsort_row_kernel<float>, dim3(blocks), dim3(256), 0, stream, (const float *)sorted + c * n,
+                         (float *)out + c * out_stride, order, n);
+    else
+      hipLaunchKernelGGL(unsort_row_kernel<double>, dim3(blocks), dim3(256), 0, stream,
+                         (const double *)sorted + c * n, (double *)out + c * out_stride, order, n);
+    MPF_LAUNCH_CHECK();
+  }
+  return MPF_OK;
+}
+
+bool binned_route(int kind, int degree, int ndim, int64_t ncomp, const Grid &g, int64_t n) {
+  if (n < BINNED_MIN_MARKERS || n > 0x7FFFFFFFll) return false;  // 32-bit bin offsets
+  return tiled_supported(kind, degree, ndim, ncomp, g, tile_shape(kind, degree, ncomp)) ||
+         tiled_2d_supported(kind, degree, ndim, ncomp, g);
+}
+
+// The bins of one call, carved from ctx->bins: sorted positions and values, results in bins
+// order, the permutation, the bin offsets and the scale, each on a 256-byte boundary.
+struct CallBins {
+  void *pos, *val, *res;
+  int64_t *order;
+  uint32_t *start;
+  double *scalars;
+};
+
+static int call_bins(mpf_context *ctx, int ndim, const int64_t *num, int64_t n, int64_t val_rows, int64_t res_rows,
+                     size_t elem, CallBins &b, hipStream_t stream) {
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t cells = (size_t)bins_of(ndim, num);
+  const size_t pos_b = up((size_t)ndim * n * elem), val_b = up((size_t)val_rows * n * elem),
+               res_b = up((size_t)res_rows * n * elem), ord_b = up((size_t)n * 8), st_b = up((cells + 2) * 4);
+  int rc = ensure(ctx->bins, pos_b + val_b + res_b + ord_b + st_b + 256, false, stream);
+  if (rc) return rc;
+  char *p = (char *)ctx->bins.ptr;
+  b.pos = p;
+  b.val = p + pos_b;
+  b.res = p + pos_b + val_b;
+  b.order = (int64_t *)(p + pos_b + val_b + res_b);
+  b.start = (uint32_t *)(p + pos_b + val_b + res_b + ord_b);
+  b.scalars = (double *)(p + pos_b + val_b + res_b + ord_b + st_b);
+  return MPF_OK;
+}
+
+int scatter_via_bins(int kind, int degree, int ndim, const void *positions, int64_t pos_stride, const void *values,
+                     int64_t val_stride, int dtype, int64_t n, int64_t ncomp, double *field, const double *lower,
+                     const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
+                     hipStream_t stream) {
+  CallBins b;
+  int rc = call_bins(ctx, ndim, num_nodes, n, ncomp, 0, dtype == F32 ? 4 : 8, b, stream);
+  if (rc) return rc;
+  rc = mpf_bin_markers(kind, degree, ndim, positions, pos_stride, values, val_stride, dtype, n, ncomp, lower,
+                       cell_size, num_nodes, code, b.pos, b.val, b.order, b.start, b.scalars, MPF_SCALE_AUTO, ctx,
+                       stream);
+  if (rc) return rc;
+  return mpf_scatter_binned(kind, degree, ndim, b.pos, b.val, dtype, n, ncomp, b.start, nullptr, nullptr, b.scalars,
+                            field, lower, cell_size, num_nodes, code, ctx, stream);
+}
+
+int gather_via_bins(int kind, int degree, int ndim, const void *positions, int64_t pos_stride, int dtype, int64_t n,
+                    const double *field, int64_t ncomp, void *result, int64_t result_stride, const double *lower,
+                    const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
+                    hipStream_t stream) {
+  CallBins b;
+  int rc = call_bins(ctx, ndim, num_nodes, n, 0, ncomp, dtype == F32 ? 4 : 8, b, stream);
+  if (rc) return rc;
+  rc = mpf_bin_markers(kind, degree, ndim, positions, pos_stride, nullptr, 0, dtype, n, 0, lower, cell_size,
+                       num_nodes, code, b.pos, nullptr, b.order, b.start, b.scalars, MPF_SCALE_AUTO, ctx, stream);
+  if (rc) return rc;
+  rc = mpf_gather_binned(kind, degree, ndim, b.pos, dtype, n, b.start, nullptr, nullptr, field, ncomp, b.res, lower,
+                         cell_size, num_nodes, code, ctx, stream);
+  if (rc) return rc;
+  return unsort_rows_strided(b.res, result, result_stride, b.order, ncomp, n, dtype, stream);
+}
+
+}  // namespace mpf
 
 /* Rows (rows, n) in bins order back to the input order: out[c * n + order[m]] =
  * sorted[c * n + m], order a permutation of 0..n-1 (mpf_bin_markers'). */
@@ -2450,28 +2547,5 @@ extern "C" int mpf_unsort_rows(const void *sorted, void *out, const int64_t *ord
     set_error("unsort_rows: need rows >= 0 and n >= 0");
     return MPF_ERR_ARGUMENT;
   }
-  if (rows == 0 || n == 0) return MPF_OK;
-  hipStream_t stream = (hipStream_t)stream_;
-  const int blocks = blocks_for(n, 256, 1 << 16);
-  const int64_t bytes = rows * n * (dtype == F32 ? 4 : 8);
-  if (bytes <= (int64_t)512 << 20) {
-    if (dtype == F32)
-      hipLaunchKernelGGL(unsort_rows_kernel<float>, dim3(blocks), dim3(256), 0, stream, (const float *)sorted,
-                         (float *)out, order, rows, n);
-    else
-      hipLaunchKernelGGL(unsort_rows_kernel<double>, dim3(blocks), dim3(256), 0, stream, (const double *)sorted,
-                         (double *)out, order, rows, n);
-    MPF_LAUNCH_CHECK();
-    return MPF_OK;
-  }
-  for (int64_t c = 0; c < rows; ++c) {
-    if (dtype == F32)
-      hipLaunchKernelGGL(unsort_row_kernel<float>, dim3(blocks), dim3(256), 0, stream, (const float *)sorted + c * n,
-                         (float *)out + c * n, order, n);
-    else
-      hipLaunchKernelGGL(unsort_row_kernel<double>, dim3(blocks), dim3(256), 0, stream,
-                         (const double *)sorted + c * n, (double *)out + c * n, order, n);
-    MPF_LAUNCH_CHECK();
-  }
-  return MPF_OK;
+  return unsort_rows_strided(sorted, out, n, order, rows, n, dtype, (hipStream_t)stream_);
 }

@@ -61,14 +61,20 @@ int ensure(Buffer &b, size_t bytes, bool zero, hipStream_t stream) {
 }
 
 static std::mutex g_ctx_mutex;
-static mpf_context *g_default[64] = {nullptr};
+static std::map<std::pair<int, hipStream_t>, mpf_context *> g_default;
 
-mpf_context *default_context() {
+// One default context per (device, stream): calls on different streams never share scratch,
+// and a context's buffers are reallocated only after its own stream has drained.
+mpf_context *default_context(hipStream_t stream) {
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    set_error("no HIP device");
+    return nullptr;
+  }
   std::lock_guard<std::mutex> lock(g_ctx_mutex);
-  if (!g_default[dev]) g_default[dev] = mpf_context_create(dev);
-  return g_default[dev];
+  mpf_context *&ctx = g_default[{dev, stream}];
+  if (!ctx) ctx = mpf_context_create(dev);
+  return ctx;
 }
 
 int make_grid(Grid &g, int ndim, int kind, int degree, const double *lower, const double *cell, const int64_t *num,
@@ -312,7 +318,8 @@ mpf_context *mpf_context_create(int device) {
 void mpf_context_destroy(mpf_context *ctx) {
   if (!ctx) return;
   (void)hipDeviceSynchronize();
-  mpf::Buffer *all[] = {&ctx->accum, &ctx->scalars, &ctx->sort_keys, &ctx->sort_tmp, &ctx->offsets, &ctx->staging, &ctx->path_in, &ctx->path_sorted};
+  mpf::Buffer *all[] = {&ctx->accum,   &ctx->scalars, &ctx->sort_keys, &ctx->sort_tmp,    &ctx->offsets,
+                        &ctx->staging, &ctx->path_in, &ctx->bins,      &ctx->path_sorted};
   for (mpf::Buffer *b : all)
     if (b->ptr) (void)hipFree(b->ptr);
   delete ctx;

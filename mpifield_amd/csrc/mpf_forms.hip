@@ -647,7 +647,7 @@ extern "C" int mpf_scatter_forms(int degree, int form, int ndim, const void *pos
   if (rc) return rc;
   if (n == 0) return MPF_OK;
   hipStream_t stream = (hipStream_t)stream_;
-  if (!ctx) ctx = default_context();
+  if (!ctx) ctx = default_context(stream);
   if (!ctx) return MPF_ERR_HIP;
   const int64_t count = g.extent[0] * (ndim > 1 ? g.extent[1] : 1) * (ndim > 2 ? g.extent[2] : 1) * ncomp;
   unsigned long long *accum = nullptr;
@@ -719,7 +719,7 @@ extern "C" int mpf_scatter_forms_binned(int degree, int form, int ndim, const vo
   }
   if (n == 0) return MPF_OK;
   hipStream_t stream = (hipStream_t)stream_;
-  if (!ctx) ctx = default_context();
+  if (!ctx) ctx = default_context(stream);
   if (!ctx) return MPF_ERR_HIP;
   const int64_t count = g.extent[0] * g.extent[1] * (ndim > 2 ? g.extent[2] : 1) * ncomp;
   rc = ensure(ctx->accum, (size_t)count * 8, true, stream);

@@ -20,6 +20,9 @@ struct Buffer {
 
 }  // namespace mpf
 
+// A context is used by one stream at a time: its scratch carries state from one launch of a
+// call to the next (the accumulator, the scale slot, bins made inside a call).  The default
+// context of a call with ctx == NULL is the one of its (device, stream) pair.
 struct mpf_context {
   int device = 0;
   mpf::Buffer accum;     // int64 fixed-point accumulator, zero between calls
@@ -30,6 +33,8 @@ struct mpf_context {
   mpf::Buffer staging;   // markers in group order while binning
   mpf::Buffer path_in;   // a path deposit's charge and end rows, to be binned with the start
   mpf::Buffer path_sorted;  // the binned paths: start, rows, order, bin offsets, scalars
+  mpf::Buffer bins;      // node deposits and gathers binned inside the call: sorted rows, order,
+                         // bin offsets, scale, results in bins order
 };
 
 namespace mpf {
@@ -49,7 +54,24 @@ int zero_async(void *p, size_t bytes, hipStream_t stream);
 // Grow a buffer to at least `bytes`; `zero` clears new memory.
 int ensure(Buffer &b, size_t bytes, bool zero, hipStream_t stream);
 
-mpf_context *default_context();
+// The library's context for (current device, stream), made on first use.
+mpf_context *default_context(hipStream_t stream);
+
+// Node deposits and gathers at least this large whose shape the tiled kernels take are
+// binned inside mpf_scatter / mpf_gather (the bins in the context) and run tiled.
+constexpr int64_t BINNED_MIN_MARKERS = 1 << 18;
+// Whether mpf_scatter / mpf_gather route a call of this shape through bins and tiles.
+bool binned_route(int kind, int degree, int ndim, int64_t ncomp, const Grid &g, int64_t n);
+// The node deposit (fixed point, automatic scale) and the node gather through bins made in
+// ctx->bins: the same results as mpf_scatter_binned / mpf_gather_binned on the caller's bins.
+int scatter_via_bins(int kind, int degree, int ndim, const void *positions, int64_t pos_stride, const void *values,
+                     int64_t val_stride, int dtype, int64_t n, int64_t ncomp, double *field, const double *lower,
+                     const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
+                     hipStream_t stream);
+int gather_via_bins(int kind, int degree, int ndim, const void *positions, int64_t pos_stride, int dtype, int64_t n,
+                    const double *field, int64_t ncomp, void *result, int64_t result_stride, const double *lower,
+                    const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
+                    hipStream_t stream);
 
 inline int blocks_for(int64_t n, int threads, int64_t cap = 1 << 20) {
   int64_t b = (n + threads - 1) / threads;

@@ -225,8 +225,13 @@ extern "C" int mpf_scatter(int kind, int degree, int ndim, const void *positions
   if (rc) return rc;
   if (n == 0) return MPF_OK;
   hipStream_t stream = (hipStream_t)stream_;
-  if (!ctx) ctx = default_context();
+  if (!ctx) ctx = default_context(stream);
   if (!ctx) return MPF_ERR_HIP;
+  // large fixed-point deposits on shapes the tiled kernels take: binned by stencil cell
+  // inside the call and deposited tile by tile (the same integers, bit for bit)
+  if (mode == MODE_FIXED && scale_exp == MPF_SCALE_AUTO && binned_route(kind, degree, ndim, ncomp, g, n))
+    return scatter_via_bins(kind, degree, ndim, positions, pos_stride, values, val_stride, dtype, n, ncomp, field,
+                            lower, cell_size, num_nodes, code, ctx, stream);
   const int64_t count = g.extent[0] * (ndim > 1 ? g.extent[1] : 1) * (ndim > 2 ? g.extent[2] : 1) * ncomp;
   unsigned long long *accum = nullptr;
   const double *slot = nullptr;
@@ -256,7 +261,6 @@ extern "C" int mpf_gather(int kind, int degree, int ndim, const void *positions,
                           int64_t n, const double *field, int64_t ncomp, void *result, int64_t result_stride,
                           const double *lower, const double *cell_size, const int64_t *num_nodes, const int64_t *code,
                           mpf_context *ctx, mpf_stream stream_) {
-  (void)ctx;
   int rc = check_common(kind, degree, dtype, n, ncomp);
   if (rc) return rc;
   Grid g;
@@ -264,6 +268,14 @@ extern "C" int mpf_gather(int kind, int degree, int ndim, const void *positions,
   if (rc) return rc;
   if (n == 0) return MPF_OK;
   hipStream_t stream = (hipStream_t)stream_;
+  // large gathers on shapes the tiled kernels take: binned inside the call, gathered tile by
+  // tile, and the results put back in the input order
+  if (binned_route(kind, degree, ndim, ncomp, g, n)) {
+    if (!ctx) ctx = default_context(stream);
+    if (!ctx) return MPF_ERR_HIP;
+    return gather_via_bins(kind, degree, ndim, positions, pos_stride, dtype, n, field, ncomp, result, result_stride,
+                           lower, cell_size, num_nodes, code, ctx, stream);
+  }
   ProfileScope prof("gather_direct", stream);
   if (dtype == F32)
     dispatch_gather<float>(kind, ndim, positions, pos_stride, n, (int)ncomp, g, kind_degree(kind, degree), field,

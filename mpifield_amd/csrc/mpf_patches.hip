@@ -330,7 +330,7 @@ extern "C" int mpf_patches_scatter(const mpf_patch_plan *plan, const void *posit
     return MPF_ERR_ARGUMENT;
   }
   hipStream_t stream = (hipStream_t)stream_;
-  if (!ctx) ctx = default_context();
+  if (!ctx) ctx = default_context(stream);
   if (!ctx) return MPF_ERR_HIP;
   int rc = ensure(ctx->accum, (size_t)(plan->field_values > 0 ? plan->field_values : 1) * 8, true, stream);
   if (rc) return rc;

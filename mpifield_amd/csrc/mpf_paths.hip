@@ -1025,7 +1025,7 @@ extern "C" int mpf_deposit_paths(int kind, int degree, int ndim, const void *sta
   if (rc) return rc;
   if (n == 0) return MPF_OK;
   hipStream_t stream = (hipStream_t)stream_;
-  if (!ctx) ctx = default_context();
+  if (!ctx) ctx = default_context(stream);
   if (!ctx) return MPF_ERR_HIP;
   const int64_t count = g.extent[0] * (ndim > 1 ? g.extent[1] : 1) * (ndim > 2 ? g.extent[2] : 1) * ndim;
   unsigned long long *accum = nullptr;

@@ -21,7 +21,7 @@ import torch
 
 from mpifield_amd import kernels as K
 from mpifield_amd._types import BlockGrid, Bounds
-from mpifield_amd.weighting import LINEAR, Weighting
+from mpifield_amd.weighting import LINEAR, Weighting, _is_bins
 
 __all__ = ["Field"]
 
@@ -246,31 +246,43 @@ class Field:
             raise ValueError(f"Expected {self.num_components} components, got {t.shape[0]}")
         return t
 
+    def bin_markers(self, positions, weighting: Weighting = LINEAR):
+        """The positions binned for ``weighting`` on this field's block (`Weighting.bin`): pass
+        them as ``positions`` to `gather` and `scatter` to skip the sort while the markers
+        stay where they are."""
+        return weighting.bin(self._positions(positions), self.block_grid(inside_domain=True))
+
     def gather(self, positions, weighting: Weighting = LINEAR):
         """Interpolate the field at marker positions (field.py:1097-1126); ``(ncomp, N)``.
 
         The result is a GPU tensor of the positions' float dtype, or a NumPy array when the
-        positions were given on the host.
+        positions were given on the host.  ``positions`` may be the bins of `bin_markers`.
         """
         self._check_halo(weighting)
-        host = not isinstance(positions, torch.Tensor)
-        pos = self._positions(positions)
+        host = not isinstance(positions, torch.Tensor) and not _is_bins(positions)
+        pos = positions if _is_bins(positions) else self._positions(positions)
         grid = self.block_grid(inside_domain=True)
         stored = self._storage[grid.index]
         if not stored.is_contiguous():
             stored = stored.contiguous()
-        result = torch.zeros((self.num_components, pos.shape[1]), dtype=pos.dtype, device=self.device)
+        n, dtype = (pos.n, pos.positions.dtype) if _is_bins(pos) else (pos.shape[1], pos.dtype)
+        result = torch.zeros((self.num_components, n), dtype=dtype, device=self.device)
         weighting.gather_block(pos, stored, result, grid)
         return result.cpu().numpy().astype(np.float64) if host else result
 
     def scatter(self, positions, values, weighting: Weighting = LINEAR) -> None:
         """Add marker values to the stored nodes, halos included (field.py:1128-1180).
 
-        On a field with halos call `exchange_halos` afterwards.
+        On a field with halos call `exchange_halos` afterwards.  ``positions`` may be the
+        bins of `bin_markers` (the values in the markers' order).
         """
         self._check_halo(weighting)
-        pos = self._positions(positions)
-        val = self._values(values, pos.shape[1], pos.dtype)
+        if _is_bins(positions):
+            pos = positions
+            val = self._values(values, pos.n, pos.positions.dtype)
+        else:
+            pos = self._positions(positions)
+            val = self._values(values, pos.shape[1], pos.dtype)
         grid = self.block_grid(inside_domain=True)
         block = self._storage[grid.index]
         weighting.scatter_block(pos, val, block, grid)

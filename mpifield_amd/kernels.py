@@ -137,9 +137,18 @@ def _expected_extent(num, code, degree, kind: int = _lib.KIND_SPLINE):
     return tuple(int(n) + (degree - 1 if c == 2 else 0) for n, c in zip(num, code))
 
 
-#: batches at least this large go through the binned (tiled) kernels where they apply
+#: batches at least this large go through the binned (tiled) kernels where they apply (the
+#: library's own threshold for mpf_scatter / mpf_gather, BINNED_MIN_MARKERS in mpf_internal.h)
 BINNED_MIN_MARKERS = 1 << 18
+#: bins hold 32-bit offsets: larger batches take the direct kernels
+BINNED_MAX_MARKERS = (1 << 31) - 1
 _TILE = (8, 8, 16)
+
+
+def binned_route(kind: int, degree: int, ncomp: int, num, code, n: int) -> bool:
+    """Whether a batch of ``n`` markers goes through bins and tiles (mpf_binned.hip's
+    binned_route): at least BINNED_MIN_MARKERS, below 2^31, and a shape tiled_ok takes."""
+    return BINNED_MIN_MARKERS <= n <= BINNED_MAX_MARKERS and tiled_ok(kind, degree, ncomp, num, code)
 
 
 def tiled_ok(kind: int, degree: int, ncomp: int, num, code) -> bool:
@@ -162,7 +171,8 @@ def scatter(kind: int, degree: int, positions, values, field: torch.Tensor, geom
             scale_exp: int = _lib.SCALE_AUTO) -> None:
     """The generic node deposit (mpf_scatter): adds into ``field`` num_nodes + (ncomp,).
 
-    Large batches in fixed-point mode are binned and deposited with the tiled kernels."""
+    Large batches in fixed-point mode are binned and deposited with the tiled kernels inside
+    the library (binned_route)."""
     ndim, lower, cell, num, code, _ = _split(geometry)
     field = _field(field)
     pos, ps = marker_rows(positions, ndim)
@@ -175,13 +185,6 @@ def scatter(kind: int, degree: int, positions, values, field: torch.Tensor, geom
     if tuple(field.shape) != (*extent, ncomp):
         raise ValueError(f"a field on this grid is {(*extent, ncomp)}, got {tuple(field.shape)}")
     mode_code = _mode_code() if mode is None else mode
-    if (mode_code == _lib.MODE_FIXED and pos.shape[1] >= BINNED_MIN_MARKERS and scale_exp == _lib.SCALE_AUTO
-            and tiled_ok(kind, degree, ncomp, num, code)):
-        from mpifield_amd import binning
-
-        bins = binning.bin_markers(pos, val, (kind, degree), geometry)
-        binning.scatter_binned(bins, field)
-        return
     with torch.cuda.device(field.device):
         _lib.call("mpf_scatter", kind, degree, ndim, pos.data_ptr(), ps, val.data_ptr(), vs, _dtype_code(pos),
                   pos.shape[1], ncomp, field.data_ptr(), lower.ctypes.data, cell.ctypes.data, num.ctypes.data,
@@ -206,15 +209,6 @@ def gather(kind: int, degree: int, positions, field: torch.Tensor, result: torch
         result2 = result
     if tuple(result2.shape) != (ncomp, pos.shape[1]):
         raise ValueError(f"result must be ({ncomp}, {pos.shape[1]}), got {tuple(result.shape)}")
-    if pos.shape[1] >= BINNED_MIN_MARKERS and tiled_ok(kind, degree, ncomp, num, code):
-        from mpifield_amd import binning
-
-        bins = binning.bin_markers(pos, None, (kind, degree), geometry)
-        sorted_out = torch.empty((ncomp, pos.shape[1]), dtype=pos.dtype, device=field.device)
-        binning.gather_binned(bins, field, sorted_out)
-        unsorted = bins.unsort(sorted_out)
-        result2.copy_(unsorted)
-        return
     direct = result2.dtype == pos.dtype and (result2.shape[1] <= 1 or result2.stride(1) == 1)
     out = result2 if direct else torch.empty((ncomp, pos.shape[1]), dtype=pos.dtype, device=field.device)
     rs = out.stride(0) if out.shape[0] > 1 else out.shape[1]

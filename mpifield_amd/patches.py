@@ -134,14 +134,17 @@ class PatchBatch:
                       self.ncomp, result.data_ptr(), rs, 0, None, K.stream_handle(fields.device))
 
     def capture(self, positions, values, fields) -> torch.cuda.CUDAGraph:
-        """Record one deposit of the batch in a HIP graph (run it once first so that the
-        library's scratch exists); replay with ``graph.replay()``."""
-        self.scatter(positions, values, fields)
-        torch.cuda.synchronize()
+        """Record one deposit of the batch in a HIP graph; replay with ``graph.replay()``.
+
+        The library's scratch belongs to the (device, stream) pair, so the deposit runs once
+        on the capture stream first: the scratch then exists and the captured call allocates
+        nothing.  ``fields`` gets that deposit as well as the captured one."""
         graph = torch.cuda.CUDAGraph()
         stream = torch.cuda.Stream()
         stream.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(stream):
+            self.scatter(positions, values, fields)
+            stream.synchronize()
             with torch.cuda.graph(graph, stream=stream):
                 self.scatter(positions, values, fields)
         torch.cuda.current_stream().wait_stream(stream)

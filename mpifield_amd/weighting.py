@@ -68,6 +68,12 @@ class _DeviceArray:
                 self.array[...] = self.tensor.cpu().numpy()
 
 
+def _is_bins(a) -> bool:
+    from mpifield_amd.binning import MarkerBins
+
+    return isinstance(a, MarkerBins)
+
+
 def _markers(a) -> torch.Tensor:
     """Marker rows on the GPU, float32 kept as float32, anything else as float64."""
     t = K.as_device(a)
@@ -113,13 +119,43 @@ class Weighting:
     def gather_kernel(self, ndim: int) -> KernelFunction:
         return self._kernel(self.gather, ndim, "gather")
 
+    def bin(self, positions, grid: BlockGrid):
+        """The positions sorted by this weighting's stencil cell on ``grid`` (a `MarkerBins`),
+        to pass as ``positions`` to `scatter_block` and `gather_block` while the markers stay
+        where they are: those calls then skip the sort.  The built-in weightings only; the
+        caller keeps the bins and bins again after the markers move."""
+        from mpifield_amd import binning
+
+        _check_clamps(self, grid)
+        return binning.bin_markers(_markers(positions), None, binning.kind_of(self), tuple(grid.geometry))
+
+    def _bins(self, bins, grid: BlockGrid):
+        from mpifield_amd import binning
+
+        if bins.values is not None:
+            raise ValueError("bins passed as positions must be made from positions alone (Weighting.bin)")
+        if (bins.kind, bins.degree) != binning.kind_of(self) or tuple(bins.geometry) != tuple(grid.geometry):
+            raise ValueError(f"these bins were made for another weighting or grid than {self.name} on this block")
+        return binning
+
     def scatter_block(self, positions, values, field, grid: BlockGrid) -> None:
         """Deposit ``values`` at ``positions`` into ``field``, a block on ``grid`` (weighting.py:195-213).
 
         ``field`` is ``num_nodes + (ncomp,)`` on the GPU (a torch tensor) or the host (a
         NumPy array, computed on the GPU and copied back); a 1-D ``values`` is one component.
+        ``positions`` may be the `MarkerBins` of `bin` (values still in the markers' order).
         """
         _check_clamps(self, grid)
+        if _is_bins(positions):
+            binning = self._bins(positions, grid)
+            val = _markers(values).to(positions.positions.dtype).reshape(-1, positions.n)
+            with _DeviceArray(field) as buffer:
+                if K.current_mode() == "fixed":
+                    binning.scatter_binned(binning.with_values(positions, val), buffer)
+                else:  # float64 atomics: the direct kernels on the markers in bins order
+                    sval = torch.index_select(val, 1, positions.order)
+                    self.scatter_kernel(len(grid.periodic))(positions.positions, sval, buffer, *grid.geometry)
+            return
         kernel = self.scatter_kernel(len(grid.periodic))
         pos = _markers(positions)
         val = _markers(values).reshape(-1, pos.shape[-1])
@@ -127,8 +163,19 @@ class Weighting:
             kernel(pos, val, buffer, *grid.geometry)
 
     def gather_block(self, positions, field, result, grid: BlockGrid) -> None:
-        """Interpolate ``field`` at ``positions`` into ``result`` ``(ncomp, N)`` (weighting.py:233-250)."""
+        """Interpolate ``field`` at ``positions`` into ``result`` ``(ncomp, N)`` (weighting.py:233-250);
+        ``positions`` may be the `MarkerBins` of `bin` (the result is in the markers' order)."""
         _check_clamps(self, grid)
+        if _is_bins(positions):
+            binning = self._bins(positions, grid)
+            values = K.as_device(field, torch.float64).contiguous()
+            sorted_out = torch.empty((values.shape[-1], positions.n), dtype=positions.positions.dtype,
+                                     device=values.device)
+            binning.gather_binned(positions, values, sorted_out)
+            with _DeviceArray(result, dtype=result.dtype if isinstance(result, torch.Tensor) and result.dtype in (
+                    torch.float32, torch.float64) else torch.float64, contiguous=False) as out:
+                out.copy_(positions.unsort(sorted_out).reshape(out.shape))
+            return
         kernel = self.gather_kernel(len(grid.periodic))
         pos = _markers(positions)
         values = K.as_device(field, torch.float64).contiguous()

@@ -120,3 +120,18 @@ def test_missing_library_is_an_error(tmp_path):
 
     with pytest.raises(_lib.MpfError, match="no CPU fallback"):
         _lib.load(tmp_path / "absent.so")
+
+
+def test_binned_route_stops_below_two_to_the_31_markers():
+    """Bins hold 32-bit offsets: a batch of 2^31 markers or more takes the direct kernels (the
+    rule of mpf_binned.hip's binned_route, which the autograd Functions share)."""
+    from mpifield_amd import kernels as K
+
+    num, code = (256, 256, 256), (1, 1, 1)
+    assert K.binned_route(1, 2, 3, num, code, (1 << 31) - 1)
+    assert not K.binned_route(1, 2, 3, num, code, 1 << 31)
+    assert not K.binned_route(1, 2, 3, num, code, K.BINNED_MIN_MARKERS - 1)
+    assert K.binned_route(1, 2, 3, num, code, K.BINNED_MIN_MARKERS)
+    # shapes without tiles never bin: clamped axes, cubic with three components
+    assert not K.binned_route(2, 3, 3, num, (2, 1, 1), 1 << 20)
+    assert not K.binned_route(2, 3, 3, num, code, 1 << 20)

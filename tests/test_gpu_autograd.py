@@ -158,9 +158,10 @@ def test_binned_gradient_matches_the_direct_derivative(name, ncomp, periodic, dt
 
 
 @pytest.mark.parametrize("mode", ["fixed", "atomic"])
-def test_large_batches_bin_once_and_match_the_direct_gradients(monkeypatch, mode):
-    """gather(scatter) above the binning threshold (tiled kernels, positions binned once) gives
-    the gradients of the direct kernels."""
+def test_shared_bins_give_the_gradients_of_separate_calls(mode):
+    """gather(scatter) above the binning threshold with the positions binned once
+    (bin_positions, passed as bins=) gives the values and gradients of the same calls each
+    binning by itself."""
     nodes, n = 32, 300000
     cell = 1.0 / nodes
     grid = m.BlockGrid.from_geometry([0.0] * 3, [cell] * 3, [nodes] * 3, [True] * 3)
@@ -170,19 +171,67 @@ def test_large_batches_bin_once_and_match_the_direct_gradients(monkeypatch, mode
     field0 = gaussians(nodes, cell)
     a = torch.as_tensor(rng.normal(size=(1, n)), device="cuda")
 
-    def run():
+    def run(shared):
         pos, w, field = (t.clone().requires_grad_(True) for t in (pos0, w0, field0))
+        bins = AG.bin_positions(pos, m.CUBIC, grid) if shared else None
+        assert (bins is not None) == shared
         with K.deposit_mode(mode):
-            rho = AG.scatter(pos, w, m.CUBIC, grid)
-            g = AG.gather(pos, rho + field, m.CUBIC, grid)
+            rho = AG.scatter(pos, w, m.CUBIC, grid, bins=bins)
+            g = AG.gather(pos, rho + field, m.CUBIC, grid, bins=bins)
             (g * a).sum().backward()
         return g.detach(), pos.grad, w.grad, field.grad
 
-    AG._BINS.clear()
-    tiled = run()
-    assert len(AG._BINS) == 1  # one binning for the scatter, the gather and both backward passes
-    monkeypatch.setattr(K, "BINNED_MIN_MARKERS", 1 << 40)
-    direct = run()
-    for got, want in zip(tiled, direct):
+    shared, separate = run(True), run(False)
+    for got, want in zip(shared, separate):
         scale = want.abs().max().item()
         assert (got - want).abs().max().item() <= 1e-9 * scale
+
+
+def test_positions_written_in_place_are_binned_afresh():
+    """Nothing is reused implicitly: after the positions change through .data (which bumps no
+    version counter), the next gather follows the new positions."""
+    nodes, n = 32, 300000
+    cell = 1.0 / nodes
+    grid = m.BlockGrid.from_geometry([0.0] * 3, [cell] * 3, [nodes] * 3, [True] * 3)
+    rng = np.random.default_rng(22)
+    pos = torch.as_tensor(rng.uniform(0, 1, (3, n)), device="cuda")
+    field = gaussians(nodes, cell)
+    first = AG.gather(pos, field, m.QUADRATIC, grid)
+    pos.data.add_(0.37 * cell)
+    moved = AG.gather(pos, field, m.QUADRATIC, grid)
+    want = torch.empty_like(moved)
+    K.gather(1, 0, pos.detach().clone(), field, want, tuple(grid.geometry))
+    assert torch.equal(moved, want)
+    assert not torch.equal(moved, first)
+    with pytest.raises(ValueError, match="another weighting"):
+        AG.gather(pos, field, m.LINEAR, grid, bins=AG.bin_positions(pos, m.QUADRATIC, grid))
+
+
+@pytest.mark.parametrize("degree", [2, 3])
+def test_clamped_scatter_is_differentiable_in_its_values(degree):
+    """autograd.scatter on clamped axes sizes the field num + degree - 1 per clamped axis: the
+    deposit equals the oracle's clamped spline deposit, and the gradient with respect to the
+    values is the clamped gather of the upstream gradient (the adjoint)."""
+    from oracle import oracle as orc
+
+    shape = m.splines(degree, clamped=True).nodes
+    num = (9, 8, 7)
+    grid = m.BlockGrid.from_geometry([0.0] * 3, [0.5, 0.4, 0.3], list(num), [False, True, False],
+                                     [True, False, True])
+    g = tuple(grid.geometry)
+    extent = (num[0] + degree - 1, num[1], num[2] + degree - 1)
+    rng = np.random.default_rng(degree)
+    upper = np.array([0.5, 0.4, 0.3]) * (np.array(num) - 1)
+    pos = np.ascontiguousarray(rng.uniform(-0.1, upper + 0.1, (500, 3)).T)
+    vals = rng.normal(size=(1, 500))
+    w = torch.as_tensor(vals, device="cuda").requires_grad_(True)
+    rho = AG.scatter(torch.as_tensor(pos, device="cuda"), w, shape, grid)
+    assert tuple(rho.shape) == (*extent, 1)
+    want = np.zeros((*extent, 1))
+    orc.scatter("spline", pos, vals, want, *g, degree=degree)
+    np.testing.assert_allclose(rho.detach().cpu().numpy(), want, rtol=0, atol=1e-10 * np.abs(want).max())
+    up = rng.normal(size=(*extent, 1))
+    (rho * torch.as_tensor(up, device="cuda")).sum().backward()
+    adj = np.zeros((1, 500))
+    orc.gather("spline", pos, up, adj, *g, degree=degree)
+    np.testing.assert_allclose(w.grad.cpu().numpy(), adj, rtol=1e-12, atol=1e-12 * np.abs(adj).max())
