@@ -105,6 +105,24 @@ int mpf_scatter(int kind, int degree, int ndim, const void *positions, int64_t p
                 const double *cell_size, const int64_t *num_nodes, const int64_t *code, int mode, int scale_exp,
                 mpf_context *ctx, mpf_stream stream);
 
+/* The node deposit of mpf_scatter that allocates nothing.  It has no counterpart in the
+ * reference; it is for callers that cannot allocate: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  Same arguments and results as
+ * mpf_scatter (any kind, degree, dimension, boundary code, stride, dtype and mode; always
+ * the direct kernels, never the in-call binning), with no context: a fixed-point call keeps
+ * its int64 accumulator and scale slot in a static device array of the library, one per GPU
+ * and held until the process ends, and takes at most MPF_NOALLOC_MAX_VALUES field values
+ * (extent product x ncomp; MPF_ERR_ARGUMENT beyond, before any device work).  Atomic mode
+ * uses no scratch.  Every call on one GPU shares that GPU's array, which each call clears
+ * before use: fixed-point calls that overlap on one GPU (on two streams at once) race.  The
+ * first call on a device looks the array's address up (hipGetSymbolAddress); make it outside
+ * graph capture. */
+#define MPF_NOALLOC_MAX_VALUES (1 << 23)
+int mpf_scatter_noalloc(int kind, int degree, int ndim, const void *positions, int64_t pos_stride,
+                        const void *values, int64_t val_stride, int dtype, int64_t n, int64_t ncomp, double *field,
+                        const double *lower, const double *cell_size, const int64_t *num_nodes, const int64_t *code,
+                        int mode, int scale_exp, mpf_stream stream);
+
 /* Replaces every gather_* kernel of catalog.py; result has the marker dtype.  Batches of
  * 2^18 to 2^31 - 1 markers on the tiled shapes are binned inside the call, gathered tile by
  * tile and written back in the input order. */

@@ -49,6 +49,8 @@ SIGNATURES: dict[str, list] = {
     "mpf_profile_reset": [],
     "mpf_profile_query": [ctypes.c_char_p, _P, _P],
     "mpf_scatter": [_INT, _INT, _INT, _P, _I64, _P, _I64, _INT, _I64, _I64, _P, _P, _P, _P, _P, _INT, _INT, _P, _P],
+    "mpf_scatter_noalloc": [_INT, _INT, _INT, _P, _I64, _P, _I64, _INT, _I64, _I64, _P, _P, _P, _P, _P, _INT, _INT,
+                            _P],
     "mpf_gather": [_INT, _INT, _INT, _P, _I64, _INT, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P],
     "mpf_gather_derivative": [_INT, _INT, _INT, _P, _I64, _INT, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P],
     "mpf_deposit_paths": [_INT, _INT, _INT, _P, _I64, _P, _I64, _P, _INT, _I64, _P, _P, _P, _P, _P, _INT, _INT, _P,

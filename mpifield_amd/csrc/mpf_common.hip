@@ -177,7 +177,13 @@ int prepare_scale(mpf_context *ctx, const void *values, int64_t vstride, int dty
                   double bound_factor, int scale_exp, hipStream_t stream) {
   int rc = ensure(ctx->scalars, 64, true, stream);
   if (rc) return rc;
-  unsigned long long *maxbits = (unsigned long long *)ctx->scalars.ptr;
+  return scale_into(values, vstride, dtype, rows, n, bound_factor, scale_exp, (unsigned long long *)ctx->scalars.ptr,
+                    scale_slot(ctx), stream);
+}
+
+int scale_into(const void *values, int64_t vstride, int dtype, int64_t rows, int64_t n, double bound_factor,
+               int scale_exp, unsigned long long *maxbits, double *slot, hipStream_t stream) {
+  int rc;
   if (scale_exp == MPF_SCALE_AUTO && values) {  // values == nullptr: the caller filled the bound slot
     rc = zero_async(maxbits, 8, stream);
     if (rc) return rc;
@@ -193,8 +199,7 @@ int prepare_scale(mpf_context *ctx, const void *values, int64_t vstride, int dty
       MPF_LAUNCH_CHECK();
     }
   }
-  hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(1), 0, stream, maxbits, (double)n, bound_factor, scale_exp,
-                     scale_slot(ctx));
+  hipLaunchKernelGGL(scale_kernel, dim3(1), dim3(1), 0, stream, maxbits, (double)n, bound_factor, scale_exp, slot);
   MPF_LAUNCH_CHECK();
   return MPF_OK;
 }
@@ -211,22 +216,21 @@ __global__ void finalize_kernel(double *__restrict__ field, unsigned long long *
   }
 }
 
-int finalize_fixed_with(mpf_context *ctx, double *field, int64_t count, const double *slot, hipStream_t stream) {
+int finalize_into(double *field, unsigned long long *accum, int64_t count, const double *slot, hipStream_t stream) {
   if (count == 0) return MPF_OK;
   const int threads = 256;
   hipLaunchKernelGGL(finalize_kernel, dim3(blocks_for(count, threads, 1 << 16)), dim3(threads), 0, stream, field,
-                     (unsigned long long *)ctx->accum.ptr, count, slot);
+                     accum, count, slot);
   MPF_LAUNCH_CHECK();
   return MPF_OK;
 }
 
+int finalize_fixed_with(mpf_context *ctx, double *field, int64_t count, const double *slot, hipStream_t stream) {
+  return finalize_into(field, (unsigned long long *)ctx->accum.ptr, count, slot, stream);
+}
+
 int finalize_fixed(mpf_context *ctx, double *field, int64_t count, hipStream_t stream) {
-  if (count == 0) return MPF_OK;
-  const int threads = 256;
-  hipLaunchKernelGGL(finalize_kernel, dim3(blocks_for(count, threads, 1 << 16)), dim3(threads), 0, stream, field,
-                     (unsigned long long *)ctx->accum.ptr, count, scale_slot(ctx));
-  MPF_LAUNCH_CHECK();
-  return MPF_OK;
+  return finalize_into(field, (unsigned long long *)ctx->accum.ptr, count, scale_slot(ctx), stream);
 }
 
 // ---- kernel timing ---------------------------------------------------------------------

@@ -90,6 +90,12 @@ int make_grid(Grid &g, int ndim, int kind, int degree, const double *lower, cons
 int prepare_scale(mpf_context *ctx, const void *values, int64_t vstride, int dtype, int64_t rows, int64_t n,
                   double bound_factor, int scale_exp, hipStream_t stream);
 double *scale_slot(mpf_context *ctx);  // device pointer: [0] scale, [1] 1/scale
+// The same into caller-given device memory: maxbits (8 bytes) and slot ([scale, 1/scale,
+// markers per register pass]); launches only, no allocation.
+int scale_into(const void *values, int64_t vstride, int dtype, int64_t rows, int64_t n, double bound_factor,
+               int scale_exp, unsigned long long *maxbits, double *slot, hipStream_t stream);
+// field[i] += accum[i] * slot[1]; accum[i] = 0 (launches only)
+int finalize_into(double *field, unsigned long long *accum, int64_t count, const double *slot, hipStream_t stream);
 
 // field[i] += accum[i] / scale; accum[i] = 0
 int finalize_fixed(mpf_context *ctx, double *field, int64_t count, hipStream_t stream);

@@ -9,6 +9,9 @@
 // accumulator that a finalize pass adds into the field (MODE_FIXED).
 // Gather: gather_*_pyccel.py, one marker per work-item, the weight products and the sum in
 // the reference's order.
+#include <map>
+#include <mutex>
+
 #include "mpf_internal.h"
 
 namespace mpf {
@@ -254,6 +257,89 @@ extern "C" int mpf_scatter(int kind, int degree, int ndim, const void *positions
   }
   MPF_LAUNCH_CHECK();
   if (mode == MODE_FIXED) return finalize_fixed(ctx, field, count, stream);
+  return MPF_OK;
+}
+
+// ---- the deposit that allocates nothing ------------------------------------------------
+//
+// mpf_scatter_noalloc keeps its fixed-point accumulator and scale slot in `scratch`, a
+// file-scope device array (one copy per GPU, held until the process ends): [0, 256) the
+// max |value| bits and the scale slot, then MPF_NOALLOC_MAX_VALUES int64 accumulator words.
+// Host code never passes the array's name to a kernel (on the host it is a stand-in, not a
+// device address): the device address comes from hipGetSymbolAddress, once per device.
+
+namespace mpf {
+
+constexpr size_t NOALLOC_HEAD = 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_HEAD + (size_t)MPF_NOALLOC_MAX_VALUES * 8;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+static std::mutex g_scratch_mutex;
+static std::map<int, unsigned char *> g_scratch_ptr;  // device -> the address of its copy
+
+// The device address of this device's scratch, looked up on the first call on the device.
+static int noalloc_scratch(unsigned char **out) {
+  int dev = 0;
+  MPF_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    void *ptr = nullptr;
+    MPF_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    it = g_scratch_ptr.emplace(dev, (unsigned char *)ptr).first;
+  }
+  *out = it->second;
+  return MPF_OK;
+}
+
+}  // namespace mpf
+
+extern "C" int mpf_scatter_noalloc(int kind, int degree, int ndim, const void *positions, int64_t pos_stride,
+                                   const void *values, int64_t val_stride, int dtype, int64_t n, int64_t ncomp,
+                                   double *field, const double *lower, const double *cell_size,
+                                   const int64_t *num_nodes, const int64_t *code, int mode, int scale_exp,
+                                   mpf_stream stream_) {
+  int rc = check_common(kind, degree, dtype, n, ncomp);
+  if (rc) return rc;
+  if (mode != MODE_ATOMIC && mode != MODE_FIXED) {
+    set_error("mode must be MPF_MODE_ATOMIC or MPF_MODE_FIXED, got %d", mode);
+    return MPF_ERR_ARGUMENT;
+  }
+  Grid g;
+  rc = make_grid(g, ndim, kind, kind_degree(kind, degree), lower, cell_size, num_nodes, code);
+  if (rc) return rc;
+  const int64_t count = g.extent[0] * (ndim > 1 ? g.extent[1] : 1) * (ndim > 2 ? g.extent[2] : 1) * ncomp;
+  if (mode == MODE_FIXED && count > (int64_t)MPF_NOALLOC_MAX_VALUES) {
+    set_error("scatter_noalloc: a fixed-point deposit takes at most %lld field values (MPF_NOALLOC_MAX_VALUES), "
+              "this field has %lld",
+              (long long)MPF_NOALLOC_MAX_VALUES, (long long)count);
+    return MPF_ERR_ARGUMENT;
+  }
+  if (n == 0) return MPF_OK;
+  hipStream_t stream = (hipStream_t)stream_;
+  unsigned long long *accum = nullptr;
+  double *slot = nullptr;
+  if (mode == MODE_FIXED) {
+    unsigned char *base = nullptr;
+    rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    unsigned long long *maxbits = (unsigned long long *)base;
+    slot = (double *)(base + 64);
+    accum = (unsigned long long *)(base + NOALLOC_HEAD);
+    // the array keeps the last call's contents: the accumulator starts at zero
+    rc = zero_async(accum, (size_t)count * 8, stream);
+    if (rc) return rc;
+    rc = scale_into(values, val_stride, dtype, ncomp, n, 1.0, scale_exp, maxbits, slot, stream);
+    if (rc) return rc;
+  }
+  if (dtype == F32)
+    dispatch_scatter<float>(kind, ndim, mode, positions, pos_stride, values, val_stride, n, (int)ncomp, g,
+                            kind_degree(kind, degree), field, accum, slot, stream);
+  else
+    dispatch_scatter<double>(kind, ndim, mode, positions, pos_stride, values, val_stride, n, (int)ncomp, g,
+                             kind_degree(kind, degree), field, accum, slot, stream);
+  MPF_LAUNCH_CHECK();
+  if (mode == MODE_FIXED) return finalize_into(field, accum, count, slot, stream);
   return MPF_OK;
 }
 
