@@ -90,8 +90,9 @@ def log(msg: str) -> None:
     print(f"# {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(pos: np.ndarray, val: np.ndarray, nodes: int, sample: int) -> dict:
-    """The oracle (C, float64, the reference's OpenMP policy) on `sample` markers."""
+def cpu_baseline(pos: np.ndarray, val: np.ndarray, nodes: int, sample: int, guard_lifted: bool = True) -> dict:
+    """The oracle (C, float64, the reference's OpenMP policy) on `sample` markers; with
+    guard_lifted, also the whole batch with the policy's private-copy bound lifted."""
     from oracle import oracle as orc
 
     threads = os.cpu_count() or 1
@@ -105,7 +106,7 @@ def cpu_baseline(pos: np.ndarray, val: np.ndarray, nodes: int, sample: int) -> d
     out = np.zeros_like(v)
     orc.gather("quadratic", p, field, out, *geometry, threads=threads)
     t2 = time.perf_counter()
-    return {
+    line = {
         "value": sample / (t2 - t0),
         "unit": "markers/s",
         "cores": threads,
@@ -113,6 +114,29 @@ def cpu_baseline(pos: np.ndarray, val: np.ndarray, nodes: int, sample: int) -> d
         "sample": f"first {sample} of the same 64M markers (input order) on the 256^3 x 3 grid: scatter {t1 - t0:.2f} s "
         f"(serial, the reference's MAX_BUFFER_VALUES policy) + gather {t2 - t1:.2f} s on {threads} threads",
     }
+    if guard_lifted:
+        # the same kernels on every marker with the private-copy bound lifted: each thread
+        # deposits its chunk into its own 256^3 x 3 copy, the copies summed in thread order
+        # (threads x 403 MB of host memory); a labelled variant, not the reference's policy
+        del p, v, out
+        n = pos.shape[1]
+        p = pos.astype(np.float64)
+        v = val.astype(np.float64)
+        field = np.zeros((nodes, nodes, nodes, v.shape[0]))
+        t0 = time.perf_counter()
+        orc.scatter("quadratic", p, v, field, *geometry, threads=threads, guard=False)
+        t1 = time.perf_counter()
+        out = np.zeros_like(v)
+        orc.gather("quadratic", p, field, out, *geometry, threads=threads)
+        t2 = time.perf_counter()
+        line["guard_lifted"] = {
+            "value": n / (t2 - t0),
+            "unit": "markers/s",
+            "cores": threads,
+            "sample": f"all {n} markers, the reference's MAX_BUFFER_VALUES guard lifted (not its policy): scatter "
+            f"{t1 - t0:.2f} s into {threads} private copies + gather {t2 - t1:.2f} s on {threads} threads",
+        }
+    return line
 
 
 class Profile:
@@ -738,27 +762,47 @@ def main() -> None:
 
         sh_ms, _ = time_steps(torch, shuffled, max(3, args.steps // 4), 1, stream)
 
-        # ---- the plain float64-atomic one-thread-per-marker form, same markers: the direct
-        # entry points (mpf_scatter in atomic mode, mpf_gather), not the binned route ----
-        def direct_gather(p):
-            ndim, lower, cell, num, code, _ = K._split(geometry)
-            _lib.call("mpf_gather", 1, 0, 3, p.data_ptr(), p.stride(0), _lib.F32, n, field.data_ptr(), comps,
-                      result.data_ptr(), result.stride(0), lower.ctypes.data, cell.ctypes.data, num.ctypes.data,
-                      code.ctypes.data, None, K.stream_handle(dev))
+        # ---- the drop-in API (Field.scatter + Field.gather, the reference's field.py:1097-1180):
+        # positions given each call (the library bins inside both calls), and the positions'
+        # bins made once with Field.bin_markers and passed instead ----
+        log("Field.scatter + Field.gather (drop-in route)")
+        import mpifield_amd as m
 
+        fld = m.Field(3, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), nodes, num_components=comps, periodic=True)
+
+        def api_positions():
+            fld.scatter(pos, val, m.QUADRATIC)
+            fld.gather(pos, m.QUADRATIC)
+
+        api_ms, _ = time_steps(torch, api_positions, max(3, args.steps // 4), 1, stream)
+        fbins = fld.bin_markers(pos, m.QUADRATIC)
+
+        def api_bins():
+            fld.scatter(fbins, val, m.QUADRATIC)
+            fld.gather(fbins, m.QUADRATIC)
+
+        apib_ms, _ = time_steps(torch, api_bins, max(3, args.steps // 4), 1, stream)
+        del fld, fbins
+
+        # ---- the plain float64-atomic one-thread-per-marker form, same markers: the direct
+        # entry points (mpf_scatter in atomic mode, mpf_gather with the in-call binning off) ----
         def atomic_sorted_direct():
-            K.scatter(1, 0, bins.positions, bins.values, field, geometry, mode=_lib.MODE_ATOMIC)
-            direct_gather(bins.positions)
+            with K.direct_kernels():
+                K.scatter(1, 0, bins.positions, bins.values, field, geometry, mode=_lib.MODE_ATOMIC)
+                K.gather(1, 0, bins.positions, field, result, geometry)
 
         def atomic_input_direct():
-            K.scatter(1, 0, pos, val, field, geometry, mode=_lib.MODE_ATOMIC)
-            direct_gather(pos)
+            with K.direct_kernels():
+                K.scatter(1, 0, pos, val, field, geometry, mode=_lib.MODE_ATOMIC)
+                K.gather(1, 0, pos, field, result, geometry)
 
         at_s, _ = time_steps(torch, atomic_sorted_direct, max(3, args.steps // 4), 1, stream)
         at_u, _ = time_steps(torch, atomic_input_direct, 3, 1, stream)
         line["variants"] = {
             "binned_sort_inside_step_markers_per_s": n / (sh_ms * 1e-3),
             "binned_sort_inside_step_ms": sh_ms,
+            "field_api_positions_step_ms": api_ms,
+            "field_api_bins_kept_step_ms": apib_ms,
             "fp64_atomic_cell_sorted_markers_per_s": n / (at_s * 1e-3),
             "fp64_atomic_cell_sorted_ms": at_s,
             "fp64_atomic_input_order_markers_per_s": n / (at_u * 1e-3),

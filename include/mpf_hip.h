@@ -123,6 +123,12 @@ int mpf_scatter_noalloc(int kind, int degree, int ndim, const void *positions, i
                         const double *lower, const double *cell_size, const int64_t *num_nodes, const int64_t *code,
                         int mode, int scale_exp, mpf_stream stream);
 
+/* The smallest batch that mpf_scatter and mpf_gather bin inside the call (2^18 markers by
+ * default; the per-name catalogue symbols forward to them); returns the previous value, and
+ * a negative argument only queries.  A process-wide setting: a huge value keeps every call
+ * on the direct, one-marker-per-work-item kernels. */
+int64_t mpf_set_binned_min(int64_t markers);
+
 /* Replaces every gather_* kernel of catalog.py; result has the marker dtype.  Batches of
  * 2^18 to 2^31 - 1 markers on the tiled shapes are binned inside the call, gathered tile by
  * tile and written back in the input order. */

@@ -52,6 +52,7 @@ SIGNATURES: dict[str, list] = {
     "mpf_scatter_noalloc": [_INT, _INT, _INT, _P, _I64, _P, _I64, _INT, _I64, _I64, _P, _P, _P, _P, _P, _INT, _INT,
                             _P],
     "mpf_gather": [_INT, _INT, _INT, _P, _I64, _INT, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P],
+    "mpf_set_binned_min": [_I64],
     "mpf_gather_derivative": [_INT, _INT, _INT, _P, _I64, _INT, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P],
     "mpf_deposit_paths": [_INT, _INT, _INT, _P, _I64, _P, _I64, _P, _INT, _I64, _P, _P, _P, _P, _P, _INT, _INT, _P,
                           _P],
@@ -96,6 +97,7 @@ for _d in (1, 2, 3):
     SIGNATURES[f"mpf_gather_forms_spline_{_d}d"] = _GATHER_IN + _geom(_d) + [_INT, _INT] + _TAIL
 
 _RESTYPES = {"mpf_last_error": ctypes.c_char_p, "mpf_context_create": _P, "mpf_context_destroy": None,
+             "mpf_set_binned_min": _I64,
              "mpf_patch_plan_create": _P, "mpf_patch_plan_destroy": None, "mpf_patch_plan_items": _I64}
 
 

@@ -27,6 +27,8 @@
 // order (gather_quadratic_3d_pyccel.py:103-114).
 #include <hipcub/hipcub.hpp>
 
+#include <atomic>
+
 #include "mpf_internal.h"
 
 namespace mpf {
@@ -2469,8 +2471,12 @@ static int unsort_rows_strided(const void *sorted, void *out, int64_t out_stride
   return MPF_OK;
 }
 
+static std::atomic<int64_t> g_binned_min{BINNED_MIN_MARKERS};
+
+int64_t binned_min() { return g_binned_min.load(); }
+
 bool binned_route(int kind, int degree, int ndim, int64_t ncomp, const Grid &g, int64_t n) {
-  if (n < BINNED_MIN_MARKERS || n > 0x7FFFFFFFll) return false;  // 32-bit bin offsets
+  if (n < binned_min() || n > 0x7FFFFFFFll) return false;  // 32-bit bin offsets
   return tiled_supported(kind, degree, ndim, ncomp, g, tile_shape(kind, degree, ncomp)) ||
          tiled_2d_supported(kind, degree, ndim, ncomp, g);
 }
@@ -2534,6 +2540,13 @@ int gather_via_bins(int kind, int degree, int ndim, const void *positions, int64
 }
 
 }  // namespace mpf
+
+/* The smallest batch that mpf_scatter / mpf_gather bin inside the call; returns the previous
+ * value (a negative argument only queries). */
+extern "C" int64_t mpf_set_binned_min(int64_t markers) {
+  if (markers < 0) return g_binned_min.load();
+  return g_binned_min.exchange(markers);
+}
 
 /* Rows (rows, n) in bins order back to the input order: out[c * n + order[m]] =
  * sorted[c * n + m], order a permutation of 0..n-1 (mpf_bin_markers'). */

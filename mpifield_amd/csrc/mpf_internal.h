@@ -57,9 +57,11 @@ int ensure(Buffer &b, size_t bytes, bool zero, hipStream_t stream);
 // The library's context for (current device, stream), made on first use.
 mpf_context *default_context(hipStream_t stream);
 
-// Node deposits and gathers at least this large whose shape the tiled kernels take are
-// binned inside mpf_scatter / mpf_gather (the bins in the context) and run tiled.
+// Node deposits and gathers at least this large (mpf_set_binned_min; 2^18 by default) whose
+// shape the tiled kernels take are binned inside mpf_scatter / mpf_gather (the bins in the
+// context) and run tiled.
 constexpr int64_t BINNED_MIN_MARKERS = 1 << 18;
+int64_t binned_min();
 // Whether mpf_scatter / mpf_gather route a call of this shape through bins and tiles.
 bool binned_route(int kind, int degree, int ndim, int64_t ncomp, const Grid &g, int64_t n);
 // The node deposit (fixed point, automatic scale) and the node gather through bins made in

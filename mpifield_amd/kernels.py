@@ -145,6 +145,18 @@ BINNED_MAX_MARKERS = (1 << 31) - 1
 _TILE = (8, 8, 16)
 
 
+@contextlib.contextmanager
+def direct_kernels():
+    """Within the block, mpf_scatter / mpf_gather keep every batch on the direct kernels
+    (mpf_set_binned_min; process-wide, restored on exit)."""
+    lib = _lib.load()
+    old = lib.mpf_set_binned_min(1 << 62)
+    try:
+        yield
+    finally:
+        lib.mpf_set_binned_min(old)
+
+
 def binned_route(kind: int, degree: int, ncomp: int, num, code, n: int) -> bool:
     """Whether a batch of ``n`` markers goes through bins and tiles (mpf_binned.hip's
     binned_route): at least BINNED_MIN_MARKERS, below 2^31, and a shape tiled_ok takes."""

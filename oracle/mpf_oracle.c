@@ -327,16 +327,18 @@ static void scatter_chunk(int kind, int degree, const grid_t *g, const double *p
 
 static int64_t grid_nodes(const grid_t *g) { return g->extent[0] * g->extent[1] * g->extent[2]; }
 
-/* scatter: adds.  The thread policy of scatter_quadratic_3d_pyccel.py:147-218. */
-void orc_scatter(int kind, int degree, int ndim, const double *pos, int64_t pos_stride, const double *val,
-                 int64_t val_stride, int64_t n, int64_t ncomp, double *field, const double *lower, const double *cell,
-                 const int64_t *num, const int64_t *code, int threads) {
+/* scatter: adds.  The thread policy of scatter_quadratic_3d_pyccel.py:147-218; with guard
+ * = 0 the MAX_BUFFER_VALUES bound on the private copies is lifted (a labelled variant of the
+ * CPU baseline, not the reference's policy). */
+void orc_scatter_policy(int kind, int degree, int ndim, const double *pos, int64_t pos_stride, const double *val,
+                        int64_t val_stride, int64_t n, int64_t ncomp, double *field, const double *lower,
+                        const double *cell, const int64_t *num, const int64_t *code, int threads, int guard) {
     grid_t g;
     make_grid(&g, ndim, kind, degree, lower, cell, num, code);
     const int64_t values = grid_nodes(&g) * ncomp;
     if (threads < 1) threads = 1;
     int serial = threads == 1 || n < MIN_PARALLEL_SCATTER || n < values;
-    if (kind != KIND_LINEAR && (int64_t)threads * values > MAX_BUFFER_VALUES) serial = 1;
+    if (guard && kind != KIND_LINEAR && (int64_t)threads * values > MAX_BUFFER_VALUES) serial = 1;
     if (kind == KIND_LINEAR && ndim == 1 && n < g.extent[0]) serial = 1;
     if (serial) {
         scatter_chunk(kind, degree, &g, pos, pos_stride, val, val_stride, ncomp, field, 0, n);
@@ -367,6 +369,13 @@ void orc_scatter(int kind, int degree, int ndim, const double *pos, int64_t pos_
         }
     }
     free(buffer);
+}
+
+void orc_scatter(int kind, int degree, int ndim, const double *pos, int64_t pos_stride, const double *val,
+                 int64_t val_stride, int64_t n, int64_t ncomp, double *field, const double *lower, const double *cell,
+                 const int64_t *num, const int64_t *code, int threads) {
+    orc_scatter_policy(kind, degree, ndim, pos, pos_stride, val, val_stride, n, ncomp, field, lower, cell, num, code,
+                       threads, 1);
 }
 
 /* The gather of markers [first, last).  gather_linear_{1,2,3}d_pyccel.py, gather_quadratic_3d_pyccel.py:81-114,

@@ -40,6 +40,8 @@ def _load() -> ctypes.CDLL:
         build()
     lib = ctypes.CDLL(str(LIB_PATH))
     lib.orc_scatter.argtypes = [_INT, _INT, _INT, _P, _I64, _P, _I64, _I64, _I64, _P, _P, _P, _P, _P, _INT]
+    lib.orc_scatter_policy.argtypes = [_INT, _INT, _INT, _P, _I64, _P, _I64, _I64, _I64, _P, _P, _P, _P, _P, _INT,
+                                       _INT]
     lib.orc_gather.argtypes = [_INT, _INT, _INT, _P, _I64, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _INT]
     lib.orc_deposit_whitney.argtypes = [_INT, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _INT]
     lib.orc_deposit_spline.argtypes = [_INT, _INT, _P, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P]
@@ -98,15 +100,17 @@ def threads_default() -> int:
     return int(os.environ.get("OMP_NUM_THREADS", "1"))
 
 
-def scatter(kind: str, positions, values, field: np.ndarray, *geometry, degree: int = 0, threads: int = 1) -> None:
-    """Add the deposit of ``values`` at ``positions`` into ``field`` (in place)."""
+def scatter(kind: str, positions, values, field: np.ndarray, *geometry, degree: int = 0, threads: int = 1,
+            guard: bool = True) -> None:
+    """Add the deposit of ``values`` at ``positions`` into ``field`` (in place); ``guard=False``
+    lifts the reference's MAX_BUFFER_VALUES bound on the threads' private copies."""
     ndim = len(geometry) // 4
     pos, ps = _rows(positions)
     val, vs = _rows(values)
     assert field.dtype == np.float64 and field.flags.c_contiguous
     lower, cell, num, code = _geometry(geometry, ndim)
-    lib().orc_scatter(KIND[kind], degree, ndim, _ptr(pos), ps, _ptr(val), vs, pos.shape[1], val.shape[0],
-                      _ptr(field), _ptr(lower), _ptr(cell), _ptr(num), _ptr(code), threads)
+    lib().orc_scatter_policy(KIND[kind], degree, ndim, _ptr(pos), ps, _ptr(val), vs, pos.shape[1], val.shape[0],
+                             _ptr(field), _ptr(lower), _ptr(cell), _ptr(num), _ptr(code), threads, 1 if guard else 0)
 
 
 def gather(kind: str, positions, field: np.ndarray, result: np.ndarray, *geometry, degree: int = 0,
