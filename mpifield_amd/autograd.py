@@ -34,7 +34,8 @@ def _positions_bins(positions: torch.Tensor, kind: int, degree: int, geometry, n
     """Fresh bins of ``positions`` (made from positions alone, with their work lists) for the
     tiled kernels, or None when the batch or the shape has no tiled kernel."""
     ndim, _, _, num, code, _ = K._split(geometry)
-    if ndim != 3 or not K.binned_route(kind, degree, ncomp, num, code, positions.shape[1]):
+    if (ndim != 3 or not K.binned_route(kind, degree, ncomp, num, code, positions.shape[1])
+            or not K.derivative_tiled_ok(kind, degree, ncomp, num, code)):
         return None
     return binning.bin_markers(positions.detach(), None, (kind, degree), geometry).prepare_items()
 
@@ -58,7 +59,7 @@ def _check_bins(bins, positions: torch.Tensor, kind: int, degree: int, geometry,
     if (bins.kind, bins.degree, tuple(bins.geometry)) != (kind, degree, tuple(geometry)) or bins.n != positions.shape[1]:
         raise ValueError("these bins were made for another weighting, grid or number of markers")
     _, _, _, num, code, _ = K._split(geometry)
-    return bins if K.tiled_ok(kind, degree, ncomp, num, code) else None
+    return bins if K.derivative_tiled_ok(kind, degree, ncomp, num, code) else None
 
 
 def _binned_deposit(bins, values: torch.Tensor, field: torch.Tensor) -> bool:

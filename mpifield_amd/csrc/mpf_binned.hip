@@ -881,8 +881,9 @@ __device__ __forceinline__ void deposit_markers(unsigned long long *acc, int bas
 // The field index (times C) of node (gx, gy, z0 - OFF) for each footprint row (ix, iy) of a
 // tile, wrapped on periodic axes; -1 for a row beyond a bounded wall (clamp = false) or the
 // clamped row (clamp = true, the gather's view).  A row is LZ nodes along z.
-template <int LX, int LY, int OFF, int C, int BLOCK>
-__device__ __forceinline__ void footprint_rows(const Grid &g, int x0, int y0, int z0, bool clamp, long long *rows) {
+template <int LX, int LY, int OFF, int BLOCK>
+__device__ __forceinline__ void footprint_rows(const Grid &g, int x0, int y0, int z0, bool clamp, int fc,
+                                               long long *rows) {
   const int num0 = (int)g.num[0], num1 = (int)g.num[1], num2 = (int)g.num[2];
   for (int r = threadIdx.x; r < LX * LY; r += BLOCK) {
     int gx = x0 - OFF + r / LY, gy = y0 - OFF + r % LY;
@@ -891,7 +892,7 @@ __device__ __forceinline__ void footprint_rows(const Grid &g, int x0, int y0, in
     else if (gx < 0 || gx >= num0) { ok = clamp; gx = gx < 0 ? 0 : num0 - 1; }
     if (g.code[1] == 1) gy = gy < 0 ? gy + num1 : (gy >= num1 ? gy - num1 : gy);
     else if (gy < 0 || gy >= num1) { ok = ok && clamp; gy = gy < 0 ? 0 : num1 - 1; }
-    rows[r] = ok ? ((long long)gx * num1 + gy) * num2 * C : -1ll;
+    rows[r] = ok ? ((long long)gx * num1 + gy) * num2 * fc : -1ll;
   }
 }
 
@@ -914,7 +915,7 @@ __device__ __forceinline__ int footprint_z(const Grid &g, int gz, bool clamp) {
 template <class T, int KIND, int P, int C, int BLOCK, bool SMALL>
 __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ pos, const T *__restrict__ val,
                                                           int64_t n, const uint32_t *__restrict__ start, Grid g,
-                                                          Tiling tl, double *__restrict__ field,
+                                                          Tiling tl, double *__restrict__ field, int fc,
                                                           const double *__restrict__ slot,
                                                           const uint16_t *__restrict__ tables,
                                                           const uint32_t *__restrict__ table_counts) {
@@ -946,7 +947,7 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
   else
     load_offsets<BLOCK, TC>(start + tile_key, cfirst);
   for (int i = t; i < C * L; i += BLOCK) acc[i] = 0ull;
-  footprint_rows<LX, LY, OFF, C, BLOCK>(g, x0, y0, z0, false, rows);
+  footprint_rows<LX, LY, OFF, BLOCK>(g, x0, y0, z0, false, fc, rows);
   __syncthreads();
   const uint32_t total = tables ? table_counts[tile_id] : build_items<BLOCK, TC>(cfirst, items, wave_sums);
 
@@ -1049,7 +1050,7 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
     const long long rb = rows[r];
     const int gz = plain_z ? z0 - OFF + iz : footprint_z(g, z0 - OFF + iz, false);
     const long long v = (long long)acc[q * L + r * LZ + iz];
-    if (rb >= 0 && gz >= 0 && v != 0) unsafeAtomicAdd(field + rb + (long long)gz * C + q, (double)v * inv);
+    if (rb >= 0 && gz >= 0 && v != 0) unsafeAtomicAdd(field + rb + (long long)gz * fc + q, (double)v * inv);
   }
 }
 
@@ -1067,7 +1068,7 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
 template <class T, int KIND, int P, int C, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) gather_tiled(const T *__restrict__ pos, int64_t n,
                                                       const uint32_t *__restrict__ start, Grid g, int ntx, int nty,
-                                                      int ntz, const double *__restrict__ field,
+                                                      int ntz, const double *__restrict__ field, int fc,
                                                       T *__restrict__ res) {
   using Sh = StencilShape<KIND, P>;
   constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, TC = TILE_CELLS;
@@ -1087,7 +1088,7 @@ __global__ void __launch_bounds__(BLOCK) gather_tiled(const T *__restrict__ pos,
     const int64_t gx = wrap_index(x0 - OFF + ix, num0, g.code[0]);
     const int64_t gy = wrap_index(y0 - OFF + iy, num1, g.code[1]);
     const int64_t gz = wrap_index(z0 - OFF + iz, num2, g.code[2]);
-    const int64_t at = ((gx * num1 + gy) * num2 + gz) * C;
+    const int64_t at = ((gx * num1 + gy) * num2 + gz) * fc;
 #pragma unroll
     for (int c = 0; c < C; ++c) tile[i * C + c] = field[at + c];
   }
@@ -1319,7 +1320,7 @@ __device__ __forceinline__ void load_pos_ahead(const float *__restrict__ px, con
 template <int KIND, int P, int C, int BLOCK, bool SMALL>
 __global__ void __launch_bounds__(BLOCK, 3) gather_items(const float *__restrict__ pos, int64_t n,
                                                       const uint32_t *__restrict__ start, Grid g, int ntx, int nty,
-                                                      int ntz, const double *__restrict__ field,
+                                                      int ntz, const double *__restrict__ field, int fc,
                                                       float *__restrict__ res, const uint16_t *__restrict__ tables,
                                                       const uint32_t *__restrict__ table_counts) {
   using Sh = StencilShape<KIND, P>;
@@ -1345,7 +1346,7 @@ __global__ void __launch_bounds__(BLOCK, 3) gather_items(const float *__restrict
   const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
   const int t = threadIdx.x;
   const int64_t tile_key = (int64_t)id * TC;
-  footprint_rows<LX, LY, OFF, C, BLOCK>(g, x0, y0, z0, true, rows);
+  footprint_rows<LX, LY, OFF, BLOCK>(g, x0, y0, z0, true, fc, rows);
   __syncthreads();
   {
     // the cell offsets and the footprint (one (node, component) per work-item and pass; a
@@ -1372,7 +1373,7 @@ __global__ void __launch_bounds__(BLOCK, 3) gather_items(const float *__restrict
       const int e = t + u * BLOCK < L * C ? t + u * BLOCK : L * C - 1;  // unconditional loads
       const int r = e / ROW, k = e - r * ROW, iz = k / C, q = k - iz * C;
       const int gz = plain_z ? z0 - OFF + iz : footprint_z(g, z0 - OFF + iz, true);
-      v[u] = field[rows[r] + (long long)gz * C + q];
+      v[u] = field[rows[r] + (long long)gz * fc + q];
     }
 #pragma unroll
     for (int u = 0; u < EO; ++u) {
@@ -1465,7 +1466,8 @@ __global__ void __launch_bounds__(BLOCK, 3) gather_items(const float *__restrict
 template <class T, int KIND, int C, int NDIM = 3>
 __global__ void __launch_bounds__(256) gather_dead_kernel(const T *__restrict__ pos, int64_t n, int degree,
                                                           const uint32_t *__restrict__ start, int64_t cells, Grid g,
-                                                          const double *__restrict__ field, T *__restrict__ res) {
+                                                          const double *__restrict__ field, int fc,
+                                                          T *__restrict__ res) {
   const int64_t first = start[cells], last = start[cells + 1];
   for (int64_t m = first + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < last;
        m += (int64_t)gridDim.x * blockDim.x) {
@@ -1482,7 +1484,7 @@ __global__ void __launch_bounds__(256) gather_dead_kernel(const T *__restrict__ 
           const double wab = NDIM >= 2 ? w[0][a] * w[1][b] : w[0][a];
           for (int e = 0; e < size[2]; ++e) {
             const double weight = NDIM == 3 ? wab * w[2][e] : wab;
-            total += weight * field[(node[0][a] * g.stride[0] + node[1][b] * g.stride[1] + node[2][e] * g.stride[2]) * C + c];
+            total += weight * field[(node[0][a] * g.stride[0] + node[1][b] * g.stride[1] + node[2][e] * g.stride[2]) * fc + c];
           }
         }
       res[c * n + m] = (T)total;
@@ -1848,23 +1850,37 @@ static int colors_for(int nt, int code) {
   return 2;
 }
 
-// Which (kind, degree, ncomp, dims, codes) the tiled kernels take; others use the direct path.
+// Which (kind, degree, ncomp, dims, codes) the 3D tiled deposit and gather take; others use
+// the direct path.  Every degree and component count (groups of three components where a
+// kernel holds three, p <= 2; one at a time otherwise); bounded axes of at least a tile,
+// periodic axes of at least two tiles, whole or with a partial last tile, and at least a
+// footprint long (a tile's footprint never meets itself across the seam).  Same-colour
+// footprints stay apart across a periodic seam with a partial last tile too: what that tile
+// deposits past the seam reaches nodes 0 .. S - 2 - OFF only, below the footprint of tile 1.
 static bool tiled_supported(int kind, int degree, int ndim, int64_t ncomp, const Grid &g, const TileShape &ts) {
-  if (ndim != 3) return false;
-  const int p = kind == LINEAR ? 1 : (kind == QUADRATIC ? 2 : degree);
-  // 1 or 3 components up to degree 2; the cubic's 64-node stencil holds one component in
-  // registers
-  if (!((ncomp == 1 || ncomp == 3) && (p <= 2 || (p == 3 && ncomp == 1)))) return false;
+  if (ndim != 3 || ncomp < 1 || ncomp > 64) return false;
   if (kind == SPLINE && (degree < 0 || degree > 3)) return false;
+  const int p = kind == LINEAR ? 1 : (kind == QUADRATIC ? 2 : degree);
+  const int span = kind == LINEAR ? 2 : p + 1;  // stencil nodes per axis
   const int t[3] = {ts.tx, ts.ty, ts.tz};
   for (int d = 0; d < 3; ++d) {
     if (g.code[d] == 2) return false;
-    // a periodic axis needs at least two tiles so that a footprint never meets itself
-    if (g.code[d] == 1 && (g.num[d] + t[d] - 1) / t[d] < 2) return false;
     if (g.num[d] < t[d]) return false;
-    // and whole tiles, so that same-colour footprints stay apart across the seam
-    if (g.code[d] == 1 && g.num[d] % t[d] != 0) return false;
+    if (g.code[d] == 1 && ((g.num[d] + t[d] - 1) / t[d] < 2 || g.num[d] < t[d] + span - 1)) return false;
   }
+  return true;
+}
+
+// The tiled gradient (derivative_tiled) holds 1 or 3 components up to degree 2, the cubic
+// with one, on whole periodic tiles.
+static bool derivative_tiled_supported(int kind, int degree, int ndim, int64_t ncomp, const Grid &g,
+                                       const TileShape &ts) {
+  if (!tiled_supported(kind, degree, ndim, ncomp, g, ts)) return false;
+  const int p = kind == LINEAR ? 1 : (kind == QUADRATIC ? 2 : degree);
+  if (!((ncomp == 1 || ncomp == 3) && (p <= 2 || (p == 3 && ncomp == 1)))) return false;
+  const int t[3] = {ts.tx, ts.ty, ts.tz};
+  for (int d = 0; d < 3; ++d)
+    if (g.code[d] == 1 && g.num[d] % t[d] != 0) return false;
   return true;
 }
 
@@ -1904,7 +1920,7 @@ static int run_gather_2d(const void *pos, int64_t n, const uint32_t *start, cons
   MPF_LAUNCH_CHECK();
   const int64_t cells = (int64_t)ntx * nty * TILE2_CELLS;
   hipLaunchKernelGGL((gather_dead_kernel<T, KIND, C, 2>), dim3(64), dim3(256), 0, stream, (const T *)pos, n, P, start,
-                     cells, g, field, (T *)res);
+                     cells, g, field, C, (T *)res);
   MPF_LAUNCH_CHECK();
   return MPF_OK;
 }
@@ -2150,7 +2166,7 @@ extern "C" int mpf_bin_markers(int kind, int degree, int ndim, const void *posit
 
 template <class T, int KIND, int P, int C>
 static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const uint32_t *start, const Grid &g,
-                             double *field, const double *scalars, const uint16_t *tables,
+                             double *field, int fc, const double *scalars, const uint16_t *tables,
                              const uint32_t *table_counts, hipStream_t stream) {
   constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, BLOCK = 256;
   Tiling tl;
@@ -2183,10 +2199,12 @@ static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const 
         // 32-bit row offsets while every row ends below 4 GiB from its base
         if ((uint64_t)n * sizeof(T) < (1ull << 32))
           hipLaunchKernelGGL((scatter_tiled<T, KIND, P, C, BLOCK, true>), dim3(blocks), dim3(BLOCK), 0, stream,
-                             (const T *)pos, (const T *)val, n, start, g, tl, field, scalars, tables, table_counts);
+                             (const T *)pos, (const T *)val, n, start, g, tl, field, fc, scalars, tables,
+                             table_counts);
         else
           hipLaunchKernelGGL((scatter_tiled<T, KIND, P, C, BLOCK, false>), dim3(blocks), dim3(BLOCK), 0, stream,
-                             (const T *)pos, (const T *)val, n, start, g, tl, field, scalars, tables, table_counts);
+                             (const T *)pos, (const T *)val, n, start, g, tl, field, fc, scalars, tables,
+                             table_counts);
         MPF_LAUNCH_CHECK();
       }
   return MPF_OK;
@@ -2194,7 +2212,8 @@ static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const 
 
 template <class T, int KIND, int P, int C>
 static int run_gather_tiled(const void *pos, int64_t n, const uint32_t *start, const Grid &g, const double *field,
-                            void *res, const uint16_t *tables, const uint32_t *table_counts, hipStream_t stream) {
+                            int fc, void *res, const uint16_t *tables, const uint32_t *table_counts,
+                            hipStream_t stream) {
   constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, BLOCK = 256;
   const int ntx = (int)((g.num[0] + TX - 1) / TX), nty = (int)((g.num[1] + TY - 1) / TY),
             ntz = (int)((g.num[2] + TZ - 1) / TZ);
@@ -2205,35 +2224,103 @@ static int run_gather_tiled(const void *pos, int64_t n, const uint32_t *start, c
     if constexpr (sizeof(T) == 4) {
       if (small)
         hipLaunchKernelGGL((gather_items<KIND, P, C, BLOCK, true>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
-                           (const float *)pos, n, start, g, ntx, nty, ntz, field, (float *)res, tables, table_counts);
+                           (const float *)pos, n, start, g, ntx, nty, ntz, field, fc, (float *)res, tables,
+                           table_counts);
       else
         hipLaunchKernelGGL((gather_items<KIND, P, C, BLOCK, false>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
-                           (const float *)pos, n, start, g, ntx, nty, ntz, field, (float *)res, tables, table_counts);
+                           (const float *)pos, n, start, g, ntx, nty, ntz, field, fc, (float *)res, tables,
+                           table_counts);
     }
     else
       hipLaunchKernelGGL((gather_tiled<T, KIND, P, C, BLOCK>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
-                         (const T *)pos, n, start, g, ntx, nty, ntz, field, (T *)res);
+                         (const T *)pos, n, start, g, ntx, nty, ntz, field, fc, (T *)res);
   }
   MPF_LAUNCH_CHECK();
   const int64_t cells = (int64_t)ntx * nty * ntz * TILE_CELLS;  // the dead bin follows the tiles' bins
   hipLaunchKernelGGL((gather_dead_kernel<T, KIND, C>), dim3(64), dim3(256), 0, stream, (const T *)pos, n, P, start,
-                     cells, g, field, (T *)res);
+                     cells, g, field, fc, (T *)res);
   MPF_LAUNCH_CHECK();
   return MPF_OK;
 }
 
-#define TILED_CASES(X, T)                                \
-  if (kind == LINEAR && ncomp == 1) return X(T, LINEAR, 1, 1);       \
-  if (kind == LINEAR && ncomp == 3) return X(T, LINEAR, 1, 3);       \
-  if (kind == QUADRATIC && ncomp == 1) return X(T, QUADRATIC, 2, 1); \
-  if (kind == QUADRATIC && ncomp == 3) return X(T, QUADRATIC, 2, 3); \
-  if (kind == SPLINE && degree == 3 && ncomp == 1) return X(T, SPLINE, 3, 1);  \
-  if (kind == SPLINE && degree == 2 && ncomp == 1) return X(T, SPLINE, 2, 1);  \
-  if (kind == SPLINE && degree == 2 && ncomp == 3) return X(T, SPLINE, 2, 3);  \
-  if (kind == SPLINE && degree == 1 && ncomp == 1) return X(T, SPLINE, 1, 1);  \
-  if (kind == SPLINE && degree == 1 && ncomp == 3) return X(T, SPLINE, 1, 3);  \
-  if (kind == SPLINE && degree == 0 && ncomp == 1) return X(T, SPLINE, 0, 1);  \
-  if (kind == SPLINE && degree == 0 && ncomp == 3) return X(T, SPLINE, 0, 3);
+// One group of C components (C = 3 for p <= 2, else 1) of a 3D tiled deposit or gather:
+// the group's values start at val (rows n apart), its field values at field (fc per node).
+template <class T, int C>
+static int scatter_tiled_group(int kind, int degree, const void *pos, const void *val, int64_t n,
+                               const uint32_t *start, const Grid &g, double *field, int fc, const double *scalars,
+                               const uint16_t *tables, const uint32_t *counts, hipStream_t stream) {
+#define SC(K, P) run_scatter_tiled<T, K, P, C>(pos, val, n, start, g, field, fc, scalars, tables, counts, stream)
+  if (kind == LINEAR) return SC(LINEAR, 1);
+  if (kind == QUADRATIC) return SC(QUADRATIC, 2);
+  if (degree == 0) return SC(SPLINE, 0);
+  if (degree == 1) return SC(SPLINE, 1);
+  if (degree == 2) return SC(SPLINE, 2);
+  if constexpr (C == 1)
+    if (degree == 3) return SC(SPLINE, 3);
+#undef SC
+  set_error("no tiled kernel for this case");
+  return MPF_ERR_UNSUPPORTED;
+}
+
+template <class T, int C>
+static int gather_tiled_group(int kind, int degree, const void *pos, int64_t n, const uint32_t *start, const Grid &g,
+                              const double *field, int fc, void *res, const uint16_t *tables, const uint32_t *counts,
+                              hipStream_t stream) {
+#define GA(K, P) run_gather_tiled<T, K, P, C>(pos, n, start, g, field, fc, res, tables, counts, stream)
+  if (kind == LINEAR) return GA(LINEAR, 1);
+  if (kind == QUADRATIC) return GA(QUADRATIC, 2);
+  if (degree == 0) return GA(SPLINE, 0);
+  if (degree == 1) return GA(SPLINE, 1);
+  if (degree == 2) return GA(SPLINE, 2);
+  if constexpr (C == 1)
+    if (degree == 3) return GA(SPLINE, 3);
+#undef GA
+  set_error("no tiled kernel for this case");
+  return MPF_ERR_UNSUPPORTED;
+}
+
+// The component groups of a 3D tiled call: groups of three components while three remain
+// and the kernels hold three (p <= 2), single components otherwise; each group reads the
+// same bins.  visit(c0, size) runs one group.
+template <class Visit>
+static int component_groups(int kind, int degree, int64_t ncomp, Visit &&visit) {
+  const int p = kind == LINEAR ? 1 : (kind == QUADRATIC ? 2 : degree);
+  for (int64_t c0 = 0; c0 < ncomp;) {
+    const int size = (p <= 2 && ncomp - c0 >= 3) ? 3 : 1;
+    const int rc = visit(c0, size);
+    if (rc) return rc;
+    c0 += size;
+  }
+  return MPF_OK;
+}
+
+template <class T>
+static int scatter_tiled_3d(int kind, int degree, const void *pos, const void *val, int64_t n, int64_t ncomp,
+                            const uint32_t *start, const Grid &g, double *field, const double *scalars,
+                            const uint16_t *tables, const uint32_t *counts, hipStream_t stream) {
+  return component_groups(kind, degree, ncomp, [&](int64_t c0, int size) {
+    const T *v = (const T *)val + c0 * n;
+    if (size == 3)
+      return scatter_tiled_group<T, 3>(kind, degree, pos, v, n, start, g, field + c0, (int)ncomp, scalars, tables,
+                                       counts, stream);
+    return scatter_tiled_group<T, 1>(kind, degree, pos, v, n, start, g, field + c0, (int)ncomp, scalars, tables,
+                                     counts, stream);
+  });
+}
+
+template <class T>
+static int gather_tiled_3d(int kind, int degree, const void *pos, int64_t n, int64_t ncomp, const uint32_t *start,
+                           const Grid &g, const double *field, void *res, const uint16_t *tables,
+                           const uint32_t *counts, hipStream_t stream) {
+  return component_groups(kind, degree, ncomp, [&](int64_t c0, int size) {
+    T *r = (T *)res + c0 * n;
+    if (size == 3)
+      return gather_tiled_group<T, 3>(kind, degree, pos, n, start, g, field + c0, (int)ncomp, r, tables, counts,
+                                      stream);
+    return gather_tiled_group<T, 1>(kind, degree, pos, n, start, g, field + c0, (int)ncomp, r, tables, counts,
+                                    stream);
+  });
+}
 
 /* Deposit binned markers into field with the tiled kernels (int64 fixed point through
  * float64 exact accumulators; bit-reproducible).  sorted_* and start come from
@@ -2269,16 +2356,11 @@ extern "C" int mpf_scatter_binned(int kind, int degree, int ndim, const void *so
     }
 #undef SC2
   }
-#define SC(T, K, P, C) \
-  run_scatter_tiled<T, K, P, C>(sorted_positions, sorted_values, n, start, g, field, scalars, item_tables, item_counts, stream)
-  if (dtype == F32) {
-    TILED_CASES(SC, float)
-  } else {
-    TILED_CASES(SC, double)
-  }
-#undef SC
-  set_error("no tiled kernel for this case");
-  return MPF_ERR_UNSUPPORTED;
+  if (dtype == F32)
+    return scatter_tiled_3d<float>(kind, degree, sorted_positions, sorted_values, n, ncomp, start, g, field, scalars,
+                                   item_tables, item_counts, stream);
+  return scatter_tiled_3d<double>(kind, degree, sorted_positions, sorted_values, n, ncomp, start, g, field, scalars,
+                                  item_tables, item_counts, stream);
 }
 
 /* Gather at binned markers with the tiled kernels: result (ncomp, n) in the sorted order. */
@@ -2309,16 +2391,11 @@ extern "C" int mpf_gather_binned(int kind, int degree, int ndim, const void *sor
     }
 #undef GA2
   }
-#define GA(T, K, P, C) \
-  run_gather_tiled<T, K, P, C>(sorted_positions, n, start, g, field, result, item_tables, item_counts, stream)
-  if (dtype == F32) {
-    TILED_CASES(GA, float)
-  } else {
-    TILED_CASES(GA, double)
-  }
-#undef GA
-  set_error("no tiled kernel for this case");
-  return MPF_ERR_UNSUPPORTED;
+  if (dtype == F32)
+    return gather_tiled_3d<float>(kind, degree, sorted_positions, n, ncomp, start, g, field, result, item_tables,
+                                  item_counts, stream);
+  return gather_tiled_3d<double>(kind, degree, sorted_positions, n, ncomp, start, g, field, result, item_tables,
+                                 item_counts, stream);
 }
 
 template <class T, int KIND, int P, int C>
@@ -2357,13 +2434,25 @@ extern "C" int mpf_gather_derivative_binned(int kind, int degree, int ndim, cons
                      num_nodes, code);
   if (rc) return rc;
   const TileShape ts = tile_shape(kind, degree, ncomp);
-  if (!tiled_supported(kind, degree, ndim, ncomp, g, ts)) {
+  if (!derivative_tiled_supported(kind, degree, ndim, ncomp, g, ts)) {
     set_error("no tiled kernel for kind %d degree %d, %dD, %lld components on this grid", kind, degree, ndim,
               (long long)ncomp);
     return MPF_ERR_UNSUPPORTED;
   }
   if (n == 0) return MPF_OK;  // no markers: nothing to add, no results
   hipStream_t stream = (hipStream_t)stream_;
+#define TILED_CASES(X, T)                                                       \
+  if (kind == LINEAR && ncomp == 1) return X(T, LINEAR, 1, 1);                  \
+  if (kind == LINEAR && ncomp == 3) return X(T, LINEAR, 1, 3);                  \
+  if (kind == QUADRATIC && ncomp == 1) return X(T, QUADRATIC, 2, 1);            \
+  if (kind == QUADRATIC && ncomp == 3) return X(T, QUADRATIC, 2, 3);            \
+  if (kind == SPLINE && degree == 3 && ncomp == 1) return X(T, SPLINE, 3, 1);  \
+  if (kind == SPLINE && degree == 2 && ncomp == 1) return X(T, SPLINE, 2, 1);  \
+  if (kind == SPLINE && degree == 2 && ncomp == 3) return X(T, SPLINE, 2, 3);  \
+  if (kind == SPLINE && degree == 1 && ncomp == 1) return X(T, SPLINE, 1, 1);  \
+  if (kind == SPLINE && degree == 1 && ncomp == 3) return X(T, SPLINE, 1, 3);  \
+  if (kind == SPLINE && degree == 0 && ncomp == 1) return X(T, SPLINE, 0, 1);  \
+  if (kind == SPLINE && degree == 0 && ncomp == 3) return X(T, SPLINE, 0, 3);
 #define DV(T, K, P, C) \
   run_derivative_tiled<T, K, P, C>(sorted_positions, n, start, g, field, coef, coef_stride, order, out, out_stride, stream)
   if (dtype == F32) {
@@ -2372,6 +2461,7 @@ extern "C" int mpf_gather_derivative_binned(int kind, int degree, int ndim, cons
     TILED_CASES(DV, double)
   }
 #undef DV
+#undef TILED_CASES
   set_error("no tiled kernel for this case");
   return MPF_ERR_UNSUPPORTED;
 }

@@ -164,19 +164,37 @@ def binned_route(kind: int, degree: int, ncomp: int, num, code, n: int) -> bool:
 
 
 def tiled_ok(kind: int, degree: int, ncomp: int, num, code) -> bool:
-    """Whether the tiled kernels take this case (the rules of mpf_binned.hip's tiled_supported
-    and tiled_2d_supported)."""
+    """Whether the tiled deposit and gather take this case (the rules of mpf_binned.hip's
+    tiled_supported and tiled_2d_supported): 2D with 1-3 components on bounded or periodic
+    axes; 3D with any degree and 1-64 components (groups of three where p <= 2), bounded axes
+    of at least a tile, periodic axes of at least two tiles and a footprint (partial last
+    tiles allowed)."""
     if len(num) not in (2, 3) or any(int(c) == 2 for c in code):
         return False
-    if len(num) == 2:
-        return 1 <= ncomp <= 3 and (kind in (0, 1) or 0 <= degree <= 3)
-    p = {0: 1, 1: 2}.get(kind, degree)
-    if not ((ncomp in (1, 3) and p in (0, 1, 2)) or (p == 3 and ncomp == 1)):
+    if kind == _lib.KIND_SPLINE and not 0 <= degree <= 3:
         return False
+    if len(num) == 2:
+        return 1 <= ncomp <= 3
+    if not 1 <= ncomp <= 64:
+        return False
+    p = {0: 1, 1: 2}.get(kind, degree)
+    span = 2 if kind == _lib.KIND_LINEAR else p + 1
     for n, c, t in zip(num, code, _TILE):
-        if n < t or (int(c) == 1 and (n % t != 0 or n // t < 2)):
+        n = int(n)
+        if n < t or (int(c) == 1 and ((n + t - 1) // t < 2 or n < t + span - 1)):
             return False
     return True
+
+
+def derivative_tiled_ok(kind: int, degree: int, ncomp: int, num, code) -> bool:
+    """Whether the tiled gradient (mpf_gather_derivative_binned) takes this 3D case: 1 or 3
+    components up to degree 2, the cubic with one, whole periodic tiles."""
+    if len(num) != 3 or not tiled_ok(kind, degree, ncomp, num, code):
+        return False
+    p = {0: 1, 1: 2}.get(kind, degree)
+    if not ((ncomp in (1, 3) and p <= 2) or (p == 3 and ncomp == 1)):
+        return False
+    return all(int(c) != 1 or int(n) % t == 0 for n, c, t in zip(num, code, _TILE))
 
 
 def scatter(kind: int, degree: int, positions, values, field: torch.Tensor, geometry, *, mode: int | None = None,

@@ -132,6 +132,15 @@ def test_binned_route_stops_below_two_to_the_31_markers():
     assert not K.binned_route(1, 2, 3, num, code, 1 << 31)
     assert not K.binned_route(1, 2, 3, num, code, K.BINNED_MIN_MARKERS - 1)
     assert K.binned_route(1, 2, 3, num, code, K.BINNED_MIN_MARKERS)
-    # shapes without tiles never bin: clamped axes, cubic with three components
+    # clamped axes never bin; any degree and component count does on tiles
     assert not K.binned_route(2, 3, 3, num, (2, 1, 1), 1 << 20)
-    assert not K.binned_route(2, 3, 3, num, code, 1 << 20)
+    assert K.binned_route(2, 3, 3, num, code, 1 << 20)
+    assert K.binned_route(1, 2, 5, (100, 36, 52), (1, 1, 1), 1 << 20)  # partial periodic tiles
+    # a periodic axis needs two tiles and a footprint: 8 x 8 x 16 tiles, quadratic stencils
+    assert not K.binned_route(1, 2, 3, (9, 64, 64), (1, 1, 1), 1 << 20)
+    assert K.binned_route(1, 2, 3, (10, 64, 64), (1, 1, 1), 1 << 20)
+    assert K.binned_route(1, 2, 3, (9, 64, 64), (0, 1, 1), 1 << 20)
+    # the tiled gradient keeps its narrower rule
+    assert not K.derivative_tiled_ok(2, 3, 3, num, code)
+    assert not K.derivative_tiled_ok(1, 2, 3, (100, 36, 52), (1, 1, 1))
+    assert K.derivative_tiled_ok(1, 2, 3, num, code)
