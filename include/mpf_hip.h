@@ -123,6 +123,18 @@ int mpf_scatter_noalloc(int kind, int degree, int ndim, const void *positions, i
                         const double *lower, const double *cell_size, const int64_t *num_nodes, const int64_t *code,
                         int mode, int scale_exp, mpf_stream stream);
 
+/* Launch variants of the float32 3D tiled deposit and gather (no counterpart in the
+ * reference; its kernels have one fixed thread policy, scatter_quadratic_3d_pyccel.py:22-25).
+ * A variant is a workgroup tile of 1 or 2 bin tiles along z (8 x 8 x 16 or 8 x 8 x 32 cells:
+ * the bins stay the same) and a workgroup of 256 or 512 work-items; the results are the same
+ * in every variant.  mpf_tiled_variant sets the variant of the deposit (op 0) or the gather
+ * (op 1) for a kind, degree and group of 1 or 3 components (the autotuner's pick) and
+ * returns the previous one (-1 on a bad argument; variant < 0 only queries);
+ * mpf_tiled_variant_shape describes variant `variant`.  A variant that does not fit a grid
+ * (a z axis shorter than its tile) falls back to variant 0. */
+int mpf_tiled_variant(int op, int kind, int degree, int ncomp_group, int variant);
+int mpf_tiled_variant_shape(int variant, int *tile, int *block);
+
 /* The smallest batch that mpf_scatter and mpf_gather bin inside the call (2^18 markers by
  * default; the per-name catalogue symbols forward to them); returns the previous value, and
  * a negative argument only queries.  A process-wide setting: a huge value keeps every call

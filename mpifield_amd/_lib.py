@@ -53,6 +53,8 @@ SIGNATURES: dict[str, list] = {
                             _P],
     "mpf_gather": [_INT, _INT, _INT, _P, _I64, _INT, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P],
     "mpf_set_binned_min": [_I64],
+    "mpf_tiled_variant": [_INT, _INT, _INT, _INT, _INT],
+    "mpf_tiled_variant_shape": [_INT, _P, _P],
     "mpf_gather_derivative": [_INT, _INT, _INT, _P, _I64, _INT, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P],
     "mpf_deposit_paths": [_INT, _INT, _INT, _P, _I64, _P, _I64, _P, _INT, _I64, _P, _P, _P, _P, _P, _INT, _INT, _P,
                           _P],
@@ -137,6 +139,17 @@ def load(path: Path | str | None = None) -> ctypes.CDLL:
         fn.restype = _RESTYPES.get(name, _INT)
     if path is None:
         _LIB = lib
+        # the launch variants and binning threshold tuned on the MI355X (autotune.py)
+        from mpifield_amd import autotune
+
+        table = autotune.TABLE
+        if table.exists():
+            import json
+
+            data = json.loads(table.read_text())
+            autotune.apply_table(lib, data)
+            if "binned" in data:
+                lib.mpf_set_binned_min(int(data["binned"]["binned_min"]))
     return lib
 
 

@@ -28,6 +28,9 @@
 #include <hipcub/hipcub.hpp>
 
 #include <atomic>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "mpf_internal.h"
 
@@ -580,23 +583,19 @@ constexpr double BIAS = 6755399441055744.0;  // 1.5 * 2^52
 // ---- tiled deposit --------------------------------------------------------------------
 
 struct Tiling {
-  int tx, ty, tz;          // tile extent in anchor cells
-  int ntx, nty, ntz;       // tiles per axis
+  int tx, ty, tz;          // tile extent in anchor cells (a workgroup's: ZT tiles along z)
+  int ntx, nty, ntz;       // workgroup tiles per axis
+  int ntz_bins;            // 8 x 8 x 16 bin tiles along z (the bins' key layout)
   int cx, cy, cz;          // colours per axis
   int color_x, color_y, color_z;  // the colour of this launch
   int tiles_x, tiles_y, tiles_z;  // tiles of this colour per axis
 };
 
-__device__ __forceinline__ int color_of(int t, int nt, int colors) {
-  if (colors == 2) return t & 1;
-  if (colors == 3) return t == nt - 1 ? 2 : (t & 1);
-  return 0;
-}
-
-// The k-th tile of colour `color` along an axis of nt tiles.
-__device__ __forceinline__ int tile_of_color(int k, int nt, int colors, int color) {
+// The k-th tile of colour `color` along an axis of nt tiles.  Two colours alternate; with
+// three, tile 0 has colour 2 to itself and tiles 1 .. nt - 1 alternate (colours_for).
+__device__ __forceinline__ int tile_of_color(int k, int colors, int color) {
   if (colors == 1) return k;
-  if (colors == 3 && color == 2) return nt - 1;
+  if (colors == 3) return color == 2 ? 0 : 1 + 2 * k + color;
   return 2 * k + color;
 }
 
@@ -670,6 +669,24 @@ __device__ __forceinline__ void load_offsets(const uint32_t *__restrict__ start,
   for (int u = 0; u < E; ++u) {
     const int i = threadIdx.x + u * BLOCK;
     if (i <= TC) v[u] = start[i];
+  }
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
+    const int i = threadIdx.x + u * BLOCK;
+    if (i <= TC) cfirst[i] = v[u];
+  }
+}
+
+// The offsets of a workgroup tile's first `valid` cells (valid <= TC); the entries past them
+// repeat the last valid one (empty cells), so no offset beyond the bins is read.
+template <int BLOCK, int TC>
+__device__ __forceinline__ void load_offsets_valid(const uint32_t *__restrict__ start, int valid, uint32_t *cfirst) {
+  constexpr int E = (TC + 1 + BLOCK - 1) / BLOCK;
+  uint32_t v[E];
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
+    const int i = threadIdx.x + u * BLOCK;
+    v[u] = start[i <= valid ? i : valid];
   }
 #pragma unroll
   for (int u = 0; u < E; ++u) {
@@ -912,18 +929,22 @@ __device__ __forceinline__ int footprint_z(const Grid &g, int gz, bool clamp) {
 // accumulator, which spans the tile's footprint before any wrap or clamp.  The markers of
 // the next item are loaded while the current one is deposited.  Nodes beyond a bounded wall
 // are folded onto the wall node, and the tile is then added into the field once.
-template <class T, int KIND, int P, int C, int BLOCK, bool SMALL>
-__global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ pos, const T *__restrict__ val,
+template <class T, int KIND, int P, int C, int BLOCK, bool SMALL, int ZT = 1>
+__global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 2 : 1) scatter_tiled(const T *__restrict__ pos,
+                                                                             const T *__restrict__ val,
                                                           int64_t n, const uint32_t *__restrict__ start, Grid g,
                                                           Tiling tl, double *__restrict__ field, int fc,
                                                           const double *__restrict__ slot,
                                                           const uint16_t *__restrict__ tables,
                                                           const uint32_t *__restrict__ table_counts) {
   using Sh = StencilShape<KIND, P>;
-  constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, TC = TILE_CELLS;
+  // a workgroup tile of ZT bin tiles along z: consecutive tiles in the bins' key order, so
+  // its cells are one contiguous run of bins (bin tile zt's cell k is cell zt * 1024 + k)
+  constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z * ZT, TC = TILE_CELLS * ZT;
   constexpr int S = Sh::S, OFF = Sh::OFF;
   constexpr int LX = TX + S - 1, LY = TY + S - 1, LZ = TZ + S - 1, L = LX * LY * LZ;
   constexpr int PARTS = TC / BLOCK;
+  static_assert(TC <= 4096 && TC % BLOCK == 0, "an item holds its cell in 12 bits");
   __shared__ unsigned long long acc[C * L];
   __shared__ uint32_t cfirst[TC + 1];
   __shared__ __align__(16) uint16_t items[SWEEPS * TC];
@@ -934,22 +955,25 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
   const int kx = blockIdx.x % tl.tiles_x;
   const int ky = (blockIdx.x / tl.tiles_x) % tl.tiles_y;
   const int kz = blockIdx.x / (tl.tiles_x * tl.tiles_y);
-  const int tx = tile_of_color(kx, tl.ntx, tl.cx, tl.color_x);
-  const int ty = tile_of_color(ky, tl.nty, tl.cy, tl.color_y);
-  const int tz = tile_of_color(kz, tl.ntz, tl.cz, tl.color_z);
+  const int tx = tile_of_color(kx, tl.cx, tl.color_x);
+  const int ty = tile_of_color(ky, tl.cy, tl.color_y);
+  const int tz = tile_of_color(kz, tl.cz, tl.color_z);
   const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
-  const int64_t tile_key = (((int64_t)tx * tl.nty + ty) * tl.ntz + tz) * TC;
+  const int64_t tile_key = (((int64_t)tx * tl.nty + ty) * tl.ntz_bins + (int64_t)tz * ZT) * TILE_CELLS;
   const int t = threadIdx.x;
+  // bin tiles of this workgroup tile that exist (the last one along z may hold fewer)
+  const int zt_valid = tl.ntz_bins - tz * ZT < ZT ? tl.ntz_bins - tz * ZT : ZT;
 
-  const int64_t tile_id = tile_key / TC;
-  if (tables)
+  const int64_t tile_id = tile_key / TILE_CELLS;
+  if (ZT == 1 && tables)
     load_offsets_and_items<BLOCK, TC>(start + tile_key, tables + tile_id * SWEEPS * TC, cfirst, items);
   else
-    load_offsets<BLOCK, TC>(start + tile_key, cfirst);
+    load_offsets_valid<BLOCK, TC>(start + tile_key, zt_valid * TILE_CELLS, cfirst);
   for (int i = t; i < C * L; i += BLOCK) acc[i] = 0ull;
   footprint_rows<LX, LY, OFF, BLOCK>(g, x0, y0, z0, false, fc, rows);
   __syncthreads();
-  const uint32_t total = tables ? table_counts[tile_id] : build_items<BLOCK, TC>(cfirst, items, wave_sums);
+  const uint32_t total =
+      (ZT == 1 && tables) ? table_counts[tile_id] : build_items<BLOCK, TC>(cfirst, items, wave_sums);
 
   const double scale = slot[0];
   // markers per register pass (the exact sums stay below 2^51 quanta): below ITEM only for a
@@ -983,8 +1007,8 @@ __global__ void __launch_bounds__(BLOCK, 2) scatter_tiled(const T *__restrict__ 
     load_ahead<SMALL>(mrows, nxt.begin, nxt.size, next);
     const uint32_t after = it + 2 * BLOCK;
     Item nn;
-    const int c = cur.cell;
-    const int cx = c / (TY * TZ), cy = (c / TZ) % TY, cz = c % TZ;
+    const int c = cur.cell, sub = c / TILE_CELLS, loc = c % TILE_CELLS;
+    const int cx = loc / (TY * TILE_Z), cy = (loc / TILE_Z) % TY, cz = sub * TILE_Z + loc % TILE_Z;
     const int base = (cx * LY + cy) * LZ + cz;
     const double cell[3] = {(double)(x0 + cx), (double)(y0 + cy), (double)(z0 + cz)};
     const uint32_t first_count = cur.size < (uint32_t)ITEM ? cur.size : ITEM;
@@ -1317,19 +1341,21 @@ __device__ __forceinline__ void load_pos_ahead(const float *__restrict__ px, con
 // node slots from LDS into registers once and interpolates the item's markers, whose
 // positions were loaded while the previous item was computed.  Tiles are numbered so that
 // each XCD (workgroups go round-robin over the eight) takes a contiguous range of them.
-template <int KIND, int P, int C, int BLOCK, bool SMALL>
-__global__ void __launch_bounds__(BLOCK, 3) gather_items(const float *__restrict__ pos, int64_t n,
+template <int KIND, int P, int C, int BLOCK, bool SMALL, int ZT = 1>
+__global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 3 : 1) gather_items(const float *__restrict__ pos, int64_t n,
                                                       const uint32_t *__restrict__ start, Grid g, int ntx, int nty,
                                                       int ntz, const double *__restrict__ field, int fc,
                                                       float *__restrict__ res, const uint16_t *__restrict__ tables,
                                                       const uint32_t *__restrict__ table_counts) {
   using Sh = StencilShape<KIND, P>;
   using Slot = typename NodeSlot<C>::type;
-  constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, TC = TILE_CELLS;
+  // a workgroup tile of ZT bin tiles along z (consecutive in the bins' key order)
+  constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z * ZT, TC = TILE_CELLS * ZT;
   constexpr int S = Sh::S, OFF = Sh::OFF;
   constexpr int LX = TX + S - 1, LY = TY + S - 1, LZ = TZ + S - 1, L = LX * LY * LZ;
   constexpr int PARTS = TC / BLOCK;
   static_assert(C == 1 || C == 3, "float32 gather tiles hold 1 or 3 components");
+  static_assert(TC <= 4096 && TC % BLOCK == 0, "an item holds its cell in 12 bits");
   // the footprint: component 0 (C = 1), or the (x, y) pairs and the z values (C = 3)
   __shared__ float2 tile_xy[C == 3 ? L : 1];
   __shared__ float tile_z[L];
@@ -1345,7 +1371,10 @@ __global__ void __launch_bounds__(BLOCK, 3) gather_items(const float *__restrict
   const int num0 = (int)g.num[0], num1 = (int)g.num[1], num2 = (int)g.num[2];
   const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
   const int t = threadIdx.x;
-  const int64_t tile_key = (int64_t)id * TC;
+  // ntz counts workgroup tiles; the bins' key layout has ntz_bins 8 x 8 x 16 tiles along z
+  const int ntz_bins = (num2 + TILE_Z - 1) / TILE_Z;
+  const int64_t tile_key = (((int64_t)tx * nty + ty) * ntz_bins + (int64_t)tz * ZT) * TILE_CELLS;
+  const int valid = (ntz_bins - tz * ZT < ZT ? ntz_bins - tz * ZT : ZT) * TILE_CELLS;
   footprint_rows<LX, LY, OFF, BLOCK>(g, x0, y0, z0, true, fc, rows);
   __syncthreads();
   {
@@ -1360,12 +1389,7 @@ __global__ void __launch_bounds__(BLOCK, 3) gather_items(const float *__restrict
 #pragma unroll
     for (int u = 0; u < EO; ++u) {
       const int i = t + u * BLOCK;
-      o[u] = start[tile_key + (i <= TC ? i : TC)];
-    }
-    uint4 tq[Q];
-    if (tables) {
-#pragma unroll
-      for (int u = 0; u < Q; ++u) tq[u] = reinterpret_cast<const uint4 *>(tables + (int64_t)id * SWEEPS * TC)[t + u * BLOCK];
+      o[u] = start[tile_key + (i <= valid ? i : valid)];
     }
     double v[E];
 #pragma unroll
@@ -1374,6 +1398,13 @@ __global__ void __launch_bounds__(BLOCK, 3) gather_items(const float *__restrict
       const int r = e / ROW, k = e - r * ROW, iz = k / C, q = k - iz * C;
       const int gz = plain_z ? z0 - OFF + iz : footprint_z(g, z0 - OFF + iz, true);
       v[u] = field[rows[r] + (long long)gz * fc + q];
+    }
+    // the item table goes straight into LDS: its loads, issued after the footprint's, return
+    // after them, so waiting for them costs no second latency
+    if (ZT == 1 && tables) {
+      const uint4 *tq = reinterpret_cast<const uint4 *>(tables + (int64_t)id * SWEEPS * TC);
+#pragma unroll
+      for (int u = 0; u < Q; ++u) reinterpret_cast<uint4 *>(items)[t + u * BLOCK] = tq[t + u * BLOCK];
     }
 #pragma unroll
     for (int u = 0; u < EO; ++u) {
@@ -1390,13 +1421,10 @@ __global__ void __launch_bounds__(BLOCK, 3) gather_items(const float *__restrict
         else flat_xy[2 * i + q] = (float)v[u];
       }
     }
-    if (tables) {
-#pragma unroll
-      for (int u = 0; u < Q; ++u) reinterpret_cast<uint4 *>(items)[t + u * BLOCK] = tq[u];
-    }
   }
   __syncthreads();
-  const uint32_t total = tables ? table_counts[id] : build_items<BLOCK, TC>(cfirst, items, wave_sums);
+  const uint32_t total =
+      (ZT == 1 && tables) ? table_counts[id] : build_items<BLOCK, TC>(cfirst, items, wave_sums);
 
   // lookahead as in scatter_tiled: unconditional loads, the first ones drained before the loop
   const float *px = pos, *py = pos + n, *pz = pos + 2 * n;
@@ -1417,8 +1445,8 @@ __global__ void __launch_bounds__(BLOCK, 3) gather_items(const float *__restrict
     const Item nxt = item_at(items, cfirst, nit < total ? nit : total - 1);
     PosAhead next;
     load_pos_ahead<SMALL>(px, py, pz, nxt, next);
-    const int c = cur.cell;
-    const int cx = c / (TY * TZ), cy = (c / TZ) % TY, cz = c % TZ;
+    const int c = cur.cell, sub = c / TILE_CELLS, loc = c % TILE_CELLS;
+    const int cx = loc / (TY * TILE_Z), cy = (loc / TILE_Z) % TY, cz = sub * TILE_Z + loc % TILE_Z;
     const int base = (cx * LY + cy) * LZ + cz;
     const double cell[3] = {(double)(x0 + cx), (double)(y0 + cy), (double)(z0 + cz)};
     Slot nd[S][S][S];
@@ -1844,19 +1872,28 @@ static TileShape tile_shape(int kind, int degree, int ncomp) {
   return TileShape{TILE_X, TILE_Y, TILE_Z, 256};
 }
 
-static int colors_for(int nt, int code) {
+// Colours along an axis of nt tiles of `tile` cells, `num` cells long, for a stencil of
+// `span` nodes: tiles of one colour must have disjoint footprints (nodes they deposit to).
+// Neighbouring tiles' footprints overlap, so two colours alternate.  A periodic axis also
+// joins tile nt - 1 to tile 0 across the seam and, when the last tile holds at most
+// span - 2 cells, tile 0 to tile nt - 2 (tile 0 deposits below node 0, which wraps to the
+// last span - 1 nodes and reaches past the short last tile into tile nt - 2's footprint).
+// An odd cycle, or that extra join with an even nt, takes three colours: tile 0 alone, then
+// tiles 1 .. nt - 1 alternating (tile nt - 1 deposits past the seam onto nodes
+// 0 .. span - 2 - OFF only, below the footprint of tile 1).
+static int colors_for(int nt, int code, int64_t num, int tile, int span) {
   if (nt == 1) return 1;
-  if (code == 1 && (nt % 2) == 1) return 3;
-  return 2;
+  if (code != 1) return 2;
+  const int64_t last = num - (int64_t)(nt - 1) * tile;
+  return (nt % 2 == 1 || last <= span - 2) ? 3 : 2;
 }
 
 // Which (kind, degree, ncomp, dims, codes) the 3D tiled deposit and gather take; others use
 // the direct path.  Every degree and component count (groups of three components where a
 // kernel holds three, p <= 2; one at a time otherwise); bounded axes of at least a tile,
 // periodic axes of at least two tiles, whole or with a partial last tile, and at least a
-// footprint long (a tile's footprint never meets itself across the seam).  Same-colour
-// footprints stay apart across a periodic seam with a partial last tile too: what that tile
-// deposits past the seam reaches nodes 0 .. S - 2 - OFF only, below the footprint of tile 1.
+// footprint long (a tile's footprint never meets itself across the seam); colors_for keeps
+// same-colour footprints apart across the seam, a partial last tile included.
 static bool tiled_supported(int kind, int degree, int ndim, int64_t ncomp, const Grid &g, const TileShape &ts) {
   if (ndim != 3 || ncomp < 1 || ncomp > 64) return false;
   if (kind == SPLINE && (degree < 0 || degree > 3)) return false;
@@ -2164,11 +2201,41 @@ extern "C" int mpf_bin_markers(int kind, int degree, int ndim, const void *posit
   return MPF_OK;
 }
 
-template <class T, int KIND, int P, int C>
-static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const uint32_t *start, const Grid &g,
-                             double *field, int fc, const double *scalars, const uint16_t *tables,
-                             const uint32_t *table_counts, hipStream_t stream) {
-  constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, BLOCK = 256;
+// ---- launch variants of the 3D tiled deposit and gather ---------------------------------
+//
+// A variant is a workgroup tile of ZT bin tiles along z (8 x 8 x 16 ZT cells; the bins are
+// unchanged) and a workgroup size.  The float32 kernels on 32-bit row offsets come in every
+// variant; the table below (mpf_tiled_variant, filled by the autotuner) picks one per
+// (operation, kind, degree, components per group); float64 markers take variant 0.
+struct Variant {
+  int zt, block;
+};
+constexpr Variant VARIANTS[] = {{1, 256}, {2, 256}, {1, 512}, {2, 512}};
+constexpr int NUM_VARIANTS = 4;
+
+static std::mutex g_variant_mutex;
+static std::map<std::tuple<int, int, int, int>, int> g_variants;  // (op, kind, degree, C) -> variant
+
+static int variant_for(int op, int kind, int degree, int c) {
+  std::lock_guard<std::mutex> lock(g_variant_mutex);
+  auto it = g_variants.find({op, kind, kind == SPLINE ? degree : 0, c});
+  return it == g_variants.end() ? 0 : it->second;
+}
+
+// Whether a workgroup tile of zt bin tiles along z fits this grid: a periodic z axis needs
+// two workgroup tiles and a footprint's length, a bounded one a workgroup tile.
+static bool variant_fits(int zt, const Grid &g, int span) {
+  if (zt == 1) return true;
+  const int64_t tz = (int64_t)TILE_Z * zt, nt = (g.num[2] + tz - 1) / tz;
+  if (g.code[2] == 1) return nt >= 2 && g.num[2] >= tz + span - 1;
+  return g.num[2] >= tz;
+}
+
+template <class T, int KIND, int P, int C, int BLOCK, bool SMALL, int ZT>
+static int launch_scatter_tiled(const void *pos, const void *val, int64_t n, const uint32_t *start, const Grid &g,
+                                double *field, int fc, const double *scalars, const uint16_t *tables,
+                                const uint32_t *table_counts, hipStream_t stream) {
+  constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z * ZT;
   Tiling tl;
   tl.tx = TX;
   tl.ty = TY;
@@ -2176,12 +2243,14 @@ static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const 
   tl.ntx = (int)((g.num[0] + TX - 1) / TX);
   tl.nty = (int)((g.num[1] + TY - 1) / TY);
   tl.ntz = (int)((g.num[2] + TZ - 1) / TZ);
-  tl.cx = colors_for(tl.ntx, g.code[0]);
-  tl.cy = colors_for(tl.nty, g.code[1]);
-  tl.cz = colors_for(tl.ntz, g.code[2]);
+  tl.ntz_bins = (int)((g.num[2] + TILE_Z - 1) / TILE_Z);
+  constexpr int SPAN = KIND == LINEAR ? 2 : P + 1;
+  tl.cx = colors_for(tl.ntx, g.code[0], g.num[0], TX, SPAN);
+  tl.cy = colors_for(tl.nty, g.code[1], g.num[1], TY, SPAN);
+  tl.cz = colors_for(tl.ntz, g.code[2], g.num[2], TZ, SPAN);
   auto count_of = [](int nt, int colors, int color) {
     if (colors == 1) return nt;
-    if (colors == 3) return color == 2 ? 1 : (nt - 1 + 1 - color) / 2;
+    if (colors == 3) return color == 2 ? 1 : (nt - color) / 2;  // tiles 1 + color, 3 + color, ...
     return (nt + 1 - color) / 2;
   };
   for (int cx = 0; cx < tl.cx; ++cx)
@@ -2196,18 +2265,48 @@ static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const 
         const int blocks = tl.tiles_x * tl.tiles_y * tl.tiles_z;
         if (blocks == 0) continue;
         ProfileScope prof("scatter_tiled", stream);
-        // 32-bit row offsets while every row ends below 4 GiB from its base
-        if ((uint64_t)n * sizeof(T) < (1ull << 32))
-          hipLaunchKernelGGL((scatter_tiled<T, KIND, P, C, BLOCK, true>), dim3(blocks), dim3(BLOCK), 0, stream,
-                             (const T *)pos, (const T *)val, n, start, g, tl, field, fc, scalars, tables,
-                             table_counts);
-        else
-          hipLaunchKernelGGL((scatter_tiled<T, KIND, P, C, BLOCK, false>), dim3(blocks), dim3(BLOCK), 0, stream,
-                             (const T *)pos, (const T *)val, n, start, g, tl, field, fc, scalars, tables,
-                             table_counts);
+        hipLaunchKernelGGL((scatter_tiled<T, KIND, P, C, BLOCK, SMALL, ZT>), dim3(blocks), dim3(BLOCK), 0, stream,
+                           (const T *)pos, (const T *)val, n, start, g, tl, field, fc, scalars, tables, table_counts);
         MPF_LAUNCH_CHECK();
       }
   return MPF_OK;
+}
+
+template <class T, int KIND, int P, int C>
+static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const uint32_t *start, const Grid &g,
+                             double *field, int fc, const double *scalars, const uint16_t *tables,
+                             const uint32_t *table_counts, hipStream_t stream) {
+  // 32-bit row offsets while every row ends below 4 GiB from its base
+  const bool small = (uint64_t)n * sizeof(T) < (1ull << 32);
+  constexpr int SPAN = KIND == LINEAR ? 2 : P + 1;
+  if constexpr (sizeof(T) == 4) {
+    if (small) {
+      int v = variant_for(0, KIND, P, C);
+      if (!variant_fits(VARIANTS[v].zt, g, SPAN)) v = 0;
+#define SV(B, Z) \
+  launch_scatter_tiled<T, KIND, P, C, B, true, Z>(pos, val, n, start, g, field, fc, scalars, tables, table_counts, stream)
+      if (v == 1) return SV(256, 2);
+      if (v == 2) return SV(512, 1);
+      if (v == 3) return SV(512, 2);
+      return SV(256, 1);
+#undef SV
+    }
+  }
+  if (small)
+    return launch_scatter_tiled<T, KIND, P, C, 256, true, 1>(pos, val, n, start, g, field, fc, scalars, tables,
+                                                             table_counts, stream);
+  return launch_scatter_tiled<T, KIND, P, C, 256, false, 1>(pos, val, n, start, g, field, fc, scalars, tables,
+                                                            table_counts, stream);
+}
+
+template <int KIND, int P, int C, int BLOCK, bool SMALL, int ZT>
+static void launch_gather_items(const void *pos, int64_t n, const uint32_t *start, const Grid &g, const double *field,
+                                int fc, void *res, const uint16_t *tables, const uint32_t *table_counts,
+                                hipStream_t stream) {
+  const int ntx = (int)((g.num[0] + TILE_X - 1) / TILE_X), nty = (int)((g.num[1] + TILE_Y - 1) / TILE_Y),
+            ntz = (int)((g.num[2] + TILE_Z * ZT - 1) / (TILE_Z * ZT));
+  hipLaunchKernelGGL((gather_items<KIND, P, C, BLOCK, SMALL, ZT>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
+                     (const float *)pos, n, start, g, ntx, nty, ntz, field, fc, (float *)res, tables, table_counts);
 }
 
 template <class T, int KIND, int P, int C>
@@ -2215,6 +2314,7 @@ static int run_gather_tiled(const void *pos, int64_t n, const uint32_t *start, c
                             int fc, void *res, const uint16_t *tables, const uint32_t *table_counts,
                             hipStream_t stream) {
   constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, BLOCK = 256;
+  constexpr int SPAN = KIND == LINEAR ? 2 : P + 1;
   const int ntx = (int)((g.num[0] + TX - 1) / TX), nty = (int)((g.num[1] + TY - 1) / TY),
             ntz = (int)((g.num[2] + TZ - 1) / TZ);
   {
@@ -2222,18 +2322,23 @@ static int run_gather_tiled(const void *pos, int64_t n, const uint32_t *start, c
     // 32-bit row offsets while every row ends below 4 GiB from its base
     const bool small = (uint64_t)n * 4u < (1ull << 32);
     if constexpr (sizeof(T) == 4) {
-      if (small)
-        hipLaunchKernelGGL((gather_items<KIND, P, C, BLOCK, true>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
-                           (const float *)pos, n, start, g, ntx, nty, ntz, field, fc, (float *)res, tables,
-                           table_counts);
-      else
-        hipLaunchKernelGGL((gather_items<KIND, P, C, BLOCK, false>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
-                           (const float *)pos, n, start, g, ntx, nty, ntz, field, fc, (float *)res, tables,
-                           table_counts);
-    }
-    else
+      if (small) {
+        int v = variant_for(1, KIND, P, C);
+        if (!variant_fits(VARIANTS[v].zt, g, SPAN)) v = 0;
+#define GV(B, Z) launch_gather_items<KIND, P, C, B, true, Z>(pos, n, start, g, field, fc, res, tables, table_counts, stream)
+        if (v == 1) GV(256, 2);
+        else if (v == 2) GV(512, 1);
+        else if (v == 3) GV(512, 2);
+        else GV(256, 1);
+#undef GV
+      } else {
+        launch_gather_items<KIND, P, C, BLOCK, false, 1>(pos, n, start, g, field, fc, res, tables, table_counts,
+                                                         stream);
+      }
+    } else {
       hipLaunchKernelGGL((gather_tiled<T, KIND, P, C, BLOCK>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
                          (const T *)pos, n, start, g, ntx, nty, ntz, field, fc, (T *)res);
+    }
   }
   MPF_LAUNCH_CHECK();
   const int64_t cells = (int64_t)ntx * nty * ntz * TILE_CELLS;  // the dead bin follows the tiles' bins
@@ -2630,6 +2735,35 @@ int gather_via_bins(int kind, int degree, int ndim, const void *positions, int64
 }
 
 }  // namespace mpf
+
+/* The launch variant of the float32 3D tiled deposit (op 0) or gather (op 1) for a (kind,
+ * degree, components per group): returns the previous one, variant < 0 only queries. */
+extern "C" int mpf_tiled_variant(int op, int kind, int degree, int ncomp_group, int variant) {
+  if (op < 0 || op > 1 || kind < LINEAR || kind > SPLINE || (ncomp_group != 1 && ncomp_group != 3) ||
+      variant >= NUM_VARIANTS) {
+    set_error("tiled_variant: op 0/1, kind 0..2, 1 or 3 components per group, variant below %d", NUM_VARIANTS);
+    return -1;
+  }
+  std::lock_guard<std::mutex> lock(g_variant_mutex);
+  const auto key = std::make_tuple(op, kind, kind == SPLINE ? degree : 0, ncomp_group);
+  auto it = g_variants.find(key);
+  const int old = it == g_variants.end() ? 0 : it->second;
+  if (variant >= 0) g_variants[key] = variant;
+  return old;
+}
+
+/* The shape of launch variant `variant`: tile[3] cells per workgroup tile and block[0] work-items. */
+extern "C" int mpf_tiled_variant_shape(int variant, int *tile, int *block) {
+  if (variant < 0 || variant >= NUM_VARIANTS) {
+    set_error("tiled_variant_shape: variants are 0..%d", NUM_VARIANTS - 1);
+    return MPF_ERR_ARGUMENT;
+  }
+  tile[0] = TILE_X;
+  tile[1] = TILE_Y;
+  tile[2] = TILE_Z * VARIANTS[variant].zt;
+  *block = VARIANTS[variant].block;
+  return MPF_OK;
+}
 
 /* The smallest batch that mpf_scatter / mpf_gather bin inside the call; returns the previous
  * value (a negative argument only queries). */

@@ -145,22 +145,33 @@ BINNED_MAX_MARKERS = (1 << 31) - 1
 _TILE = (8, 8, 16)
 
 
+def binned_min() -> int:
+    """The smallest batch that mpf_scatter / mpf_gather bin inside the call (the library's
+    current setting: BINNED_MIN_MARKERS, or the tuned value the library loaded)."""
+    return int(_lib.load().mpf_set_binned_min(-1))
+
+
 @contextlib.contextmanager
-def direct_kernels():
-    """Within the block, mpf_scatter / mpf_gather keep every batch on the direct kernels
+def binned_from(markers: int):
+    """Within the block, mpf_scatter / mpf_gather bin batches of at least ``markers``
     (mpf_set_binned_min; process-wide, restored on exit)."""
     lib = _lib.load()
-    old = lib.mpf_set_binned_min(1 << 62)
+    old = lib.mpf_set_binned_min(int(markers))
     try:
         yield
     finally:
         lib.mpf_set_binned_min(old)
 
 
+def direct_kernels():
+    """Within the block, mpf_scatter / mpf_gather keep every batch on the direct kernels."""
+    return binned_from(1 << 62)
+
+
 def binned_route(kind: int, degree: int, ncomp: int, num, code, n: int) -> bool:
     """Whether a batch of ``n`` markers goes through bins and tiles (mpf_binned.hip's
-    binned_route): at least BINNED_MIN_MARKERS, below 2^31, and a shape tiled_ok takes."""
-    return BINNED_MIN_MARKERS <= n <= BINNED_MAX_MARKERS and tiled_ok(kind, degree, ncomp, num, code)
+    binned_route): at least binned_min(), below 2^31, and a shape tiled_ok takes."""
+    return binned_min() <= n <= BINNED_MAX_MARKERS and tiled_ok(kind, degree, ncomp, num, code)
 
 
 def tiled_ok(kind: int, degree: int, ncomp: int, num, code) -> bool:
@@ -325,7 +336,7 @@ def scatter_forms(degree: int, form: int, positions, values, field: torch.Tensor
     if val.shape[1] != pos.shape[1]:
         raise ValueError(f"{pos.shape[1]} positions for {val.shape[1]} values")
     mode_code = _mode_code() if mode is None else mode
-    if (mode_code == _lib.MODE_FIXED and BINNED_MIN_MARKERS <= pos.shape[1] < 1 << 31
+    if (mode_code == _lib.MODE_FIXED and binned_min() <= pos.shape[1] < 1 << 31
             and forms_tiled_ok(degree, num, code)):
         from mpifield_amd import binning
 
@@ -358,7 +369,7 @@ def gather_forms(degree: int, form: int, positions, field: torch.Tensor, result:
     result2 = result.reshape(1, -1) if result.dim() == 1 else result
     if tuple(result2.shape) != (ncomp, pos.shape[1]):
         raise ValueError(f"result must be ({ncomp}, {pos.shape[1]}), got {tuple(result.shape)}")
-    if BINNED_MIN_MARKERS <= pos.shape[1] < 1 << 31 and forms_tiled_ok(degree, num, code):
+    if binned_min() <= pos.shape[1] < 1 << 31 and forms_tiled_ok(degree, num, code):
         from mpifield_amd import binning
 
         bins = binning.bin_markers(pos, None, (_lib.KIND_SPLINE, degree), geometry)

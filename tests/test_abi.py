@@ -6,6 +6,8 @@ from __future__ import annotations
 
 import ctypes
 import re
+import subprocess
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -38,7 +40,11 @@ def test_abi_version_and_error_slot_work_without_a_gpu():
 
     lib = _lib.load()
     assert lib.mpf_abi_version() == 1
-    assert lib.mpf_last_error() == b""
+    # the slot keeps a thread's last error, so it is empty only in a fresh process
+    fresh = subprocess.run([sys.executable, "-c", "import ctypes, sys; lib = ctypes.CDLL(sys.argv[1]); "
+                            "lib.mpf_last_error.restype = ctypes.c_char_p; print(repr(lib.mpf_last_error()))",
+                            str(_lib.LIB_PATH)], capture_output=True, text=True, check=True)
+    assert fresh.stdout.strip() == "b''"
 
 
 def test_argument_errors_need_no_gpu():

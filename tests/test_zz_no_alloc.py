@@ -15,6 +15,7 @@ pytestmark = pytest.mark.gpu
 
 if torch.cuda.is_available():
     from mpifield_amd import _lib
+    from mpifield_amd import kernels as K
 
 
 def _arrays(ndim):
@@ -57,14 +58,15 @@ def test_noalloc_deposit_equals_mpf_scatter_twice(case, mode):
     lower = [0.0] * ndim
     extent = tuple(n + (degree - 1 if c == 2 else 0) for n, c in zip(nodes, code))
     rng = np.random.default_rng(case)
-    # below the in-call binning threshold (2^18 markers), where mpf_scatter runs the same
-    # direct kernels; the second call smaller than the first
+    # mpf_scatter on the direct kernels (the ones the no-alloc entry point runs) at every
+    # size; the second call smaller than the first
     for n in (250000, 7000):
         pos = torch.as_tensor(rng.uniform(-0.05, 1.05, (ndim, n)), device="cuda", dtype=dtype)
         val = torch.as_tensor(rng.normal(1.0, 0.3, (ncomp, n)), device="cuda", dtype=dtype)
         want = torch.zeros((*extent, ncomp), dtype=torch.float64, device="cuda")
         got = torch.zeros_like(want)
-        deposit("mpf_scatter", kind, degree, pos, val, want, lower, cell, nodes, code, mode_code)
+        with K.direct_kernels():
+            deposit("mpf_scatter", kind, degree, pos, val, want, lower, cell, nodes, code, mode_code)
         deposit("mpf_scatter_noalloc", kind, degree, pos, val, got, lower, cell, nodes, code, mode_code)
         torch.cuda.synchronize()
         if mode == "fixed":
