@@ -701,8 +701,8 @@ template <int BLOCK, int TC>
 __device__ __forceinline__ void load_offsets_and_items(const uint32_t *__restrict__ start,
                                                        const uint16_t *__restrict__ table, uint32_t *cfirst,
                                                        uint16_t *items) {
-  constexpr int E = (TC + 1 + BLOCK - 1) / BLOCK, Q = SWEEPS * TC * 2 / 16 / BLOCK;
-  static_assert(SWEEPS * TC * 2 % (16 * BLOCK) == 0, "table copy");
+  constexpr int E = (TC + 1 + BLOCK - 1) / BLOCK, Q16 = SWEEPS * TC * 2 / 16, Q = (Q16 + BLOCK - 1) / BLOCK;
+  static_assert(SWEEPS * TC * 2 % 16 == 0, "table copy");
   uint32_t v[E];
   uint4 q[Q];
 #pragma unroll
@@ -711,14 +711,20 @@ __device__ __forceinline__ void load_offsets_and_items(const uint32_t *__restric
     v[u] = start[i <= TC ? i : TC];
   }
 #pragma unroll
-  for (int u = 0; u < Q; ++u) q[u] = reinterpret_cast<const uint4 *>(table)[threadIdx.x + u * BLOCK];
+  for (int u = 0; u < Q; ++u) {
+    const int i = threadIdx.x + u * BLOCK;
+    q[u] = reinterpret_cast<const uint4 *>(table)[i < Q16 ? i : Q16 - 1];
+  }
 #pragma unroll
   for (int u = 0; u < E; ++u) {
     const int i = threadIdx.x + u * BLOCK;
     if (i <= TC) cfirst[i] = v[u];
   }
 #pragma unroll
-  for (int u = 0; u < Q; ++u) reinterpret_cast<uint4 *>(items)[threadIdx.x + u * BLOCK] = q[u];
+  for (int u = 0; u < Q; ++u) {
+    const int i = threadIdx.x + u * BLOCK;
+    if (i < Q16) reinterpret_cast<uint4 *>(items)[i] = q[u];
+  }
 }
 
 // Item tables of every 3D tile, in the order build_items lists them for workgroups of
@@ -922,6 +928,24 @@ __device__ __forceinline__ int footprint_z(const Grid &g, int gz, bool clamp) {
   return clamp ? (gz < 0 ? 0 : num2 - 1) : -1;
 }
 
+// A walk over a tile's footprint, NROWS rows of ROW words (a row: LZ nodes along z times C
+// components, consecutive words of the field), in steps of whole rows: work-item t takes
+// word k = t % ROW of rows t / ROW, t / ROW + RPS, ... (RPS rows per step), so that its node
+// along z, its component and their wrap or clamp are computed once, and an access costs an
+// add.  Work-items past RPS * ROW sit out.
+template <int BLOCK, int ROW, int NROWS>
+struct RowWalk {
+  static_assert(ROW <= BLOCK, "a footprint row fits one step");
+  static constexpr int RPS = BLOCK / ROW, STEPS = (NROWS + RPS - 1) / RPS;
+  int r0, k;
+  bool on;
+  __device__ explicit RowWalk(int t) : r0(t / ROW), k(t % ROW), on(t < RPS * ROW) {}
+  __device__ int row(int u) const { return r0 + u * RPS; }
+  __device__ bool has(int u) const { return on && row(u) < NROWS; }
+  // a row that exists, for loads issued unconditionally
+  __device__ int row_in(int u) const { return has(u) ? row(u) : NROWS - 1; }
+};
+
 // The deposit of one tile of TILE_X x TILE_Y x TILE_Z stencil cells.  The bins are in tile
 // order, so the tile's cells are consecutive bins; their offsets go to LDS and the markers
 // are cut into items (build_items).  A work-item runs over items: it sums an item's
@@ -929,6 +953,8 @@ __device__ __forceinline__ int footprint_z(const Grid &g, int gz, bool clamp) {
 // accumulator, which spans the tile's footprint before any wrap or clamp.  The markers of
 // the next item are loaded while the current one is deposited.  Nodes beyond a bounded wall
 // are folded onto the wall node, and the tile is then added into the field once.
+// Two waves per SIMD: the item loop needs about 200 registers (capped at 168, for three
+// waves, it spills inside the loop and runs at half the speed).
 template <class T, int KIND, int P, int C, int BLOCK, bool SMALL, int ZT = 1>
 __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 2 : 1) scatter_tiled(const T *__restrict__ pos,
                                                                              const T *__restrict__ val,
@@ -1068,13 +1094,16 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 2 : 1) scatter_tiled(con
   // same as a read-modify-write, bit for bit, whatever the timing.
   const double inv = slot[1];
   const bool plain_z = g.code[2] != 2 && z0 - OFF >= 0 && z0 - OFF + LZ <= num2;  // rows neither wrap nor clamp
-  constexpr int ROW = LZ * C;
-  for (int e = t; e < L * C; e += BLOCK) {
-    const int r = e / ROW, k = e - r * ROW, iz = k / C, q = k - iz * C;
+  using Walk = RowWalk<BLOCK, LZ * C, LX * LY>;
+  const Walk walk(t);
+  const int iz = walk.k / C, q = walk.k - iz * C;
+  const int gz = plain_z ? z0 - OFF + iz : footprint_z(g, z0 - OFF + iz, false);
+  const long long zq = (long long)gz * fc + q;
+  for (int u = 0; u < Walk::STEPS && walk.has(u); ++u) {
+    const int r = walk.row(u);
     const long long rb = rows[r];
-    const int gz = plain_z ? z0 - OFF + iz : footprint_z(g, z0 - OFF + iz, false);
     const long long v = (long long)acc[q * L + r * LZ + iz];
-    if (rb >= 0 && gz >= 0 && v != 0) unsafeAtomicAdd(field + rb + (long long)gz * fc + q, (double)v * inv);
+    if (rb >= 0 && gz >= 0 && v != 0) unsafeAtomicAdd(field + rb + zq, (double)v * inv);
   }
 }
 
@@ -1382,29 +1411,28 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 3 : 1) gather_items(cons
     // row is LZ * C consecutive words of the field), every load of the work-item issued
     // before the first one is used: one memory latency for both
     constexpr int EO = (TC + 1 + BLOCK - 1) / BLOCK;
-    constexpr int E = (L * C + BLOCK - 1) / BLOCK, ROW = LZ * C;
     const bool plain_z = g.code[2] != 2 && z0 - OFF >= 0 && z0 - OFF + LZ <= num2;
-    constexpr int Q = SWEEPS * TC * 2 / 16 / BLOCK;
+    constexpr int Q16 = SWEEPS * TC * 2 / 16, Q = (Q16 + BLOCK - 1) / BLOCK;
     uint32_t o[EO];
 #pragma unroll
     for (int u = 0; u < EO; ++u) {
       const int i = t + u * BLOCK;
       o[u] = start[tile_key + (i <= valid ? i : valid)];
     }
-    double v[E];
+    using Walk = RowWalk<BLOCK, LZ * C, LX * LY>;
+    const Walk walk(t);
+    const int iz = walk.k / C, q = walk.k - iz * C;
+    const long long zq = (long long)(plain_z ? z0 - OFF + iz : footprint_z(g, z0 - OFF + iz, true)) * fc + q;
+    double v[Walk::STEPS];
 #pragma unroll
-    for (int u = 0; u < E; ++u) {
-      const int e = t + u * BLOCK < L * C ? t + u * BLOCK : L * C - 1;  // unconditional loads
-      const int r = e / ROW, k = e - r * ROW, iz = k / C, q = k - iz * C;
-      const int gz = plain_z ? z0 - OFF + iz : footprint_z(g, z0 - OFF + iz, true);
-      v[u] = field[rows[r] + (long long)gz * fc + q];
-    }
+    for (int u = 0; u < Walk::STEPS; ++u) v[u] = field[rows[walk.row_in(u)] + zq];  // unconditional loads
     // the item table goes straight into LDS: its loads, issued after the footprint's, return
     // after them, so waiting for them costs no second latency
     if (ZT == 1 && tables) {
       const uint4 *tq = reinterpret_cast<const uint4 *>(tables + (int64_t)id * SWEEPS * TC);
 #pragma unroll
-      for (int u = 0; u < Q; ++u) reinterpret_cast<uint4 *>(items)[t + u * BLOCK] = tq[t + u * BLOCK];
+      for (int u = 0; u < Q; ++u)
+        if (t + u * BLOCK < Q16) reinterpret_cast<uint4 *>(items)[t + u * BLOCK] = tq[t + u * BLOCK];
     }
 #pragma unroll
     for (int u = 0; u < EO; ++u) {
@@ -1413,10 +1441,9 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 3 : 1) gather_items(cons
     }
     float *flat_xy = reinterpret_cast<float *>(tile_xy);
 #pragma unroll
-    for (int u = 0; u < E; ++u) {
-      const int e = t + u * BLOCK;
-      if (e < L * C) {
-        const int i = e / C, q = e - (e / C) * C;
+    for (int u = 0; u < Walk::STEPS; ++u) {
+      if (walk.has(u)) {
+        const int i = walk.row(u) * LZ + iz;
         if (q == C - 1) tile_z[i] = (float)v[u];
         else flat_xy[2 * i + q] = (float)v[u];
       }
