@@ -12,7 +12,9 @@ Headline (``value``): markers kept sorted by stencil cell between steps, the way
 code keeps them -- the on-device counting sort (mpf_bin_markers) runs once before timing;
 the step is the tiled deposit (int64 fixed point, bit-reproducible) + the tiled gather.
 Also reported: the same step with the sort inside it (shuffled markers, results returned in
-input order) and the plain float64-atomic one-thread-per-marker form on the same markers.
+input order), with an incremental re-binning inside it (the binned markers moved by up to
+half a cell, results kept in bins order) and the plain float64-atomic
+one-thread-per-marker form on the same markers.
 
 Prints ONE JSON line: throughput (marker scatter+gather pairs per second), the roofline of
 the dominant kernel (algorithmic bytes / its HIP-event-timed launches), the error against
@@ -762,6 +764,32 @@ def main() -> None:
 
         sh_ms, _ = time_steps(torch, shuffled, max(3, args.steps // 4), 1, stream)
 
+        # ---- re-binning inside the step: the bins' sorted markers moved by up to half a cell
+        # per axis since they were binned (a pusher's step), binned again incrementally
+        # (mpf_rebin_markers), results kept in bins order ----
+        log("re-binning moved markers (incremental) inside the step")
+        gen = torch.Generator(device=dev).manual_seed(5)
+        moved = torch.remainder(bins.positions + (torch.rand(bins.positions.shape, generator=gen, device=dev) - 0.5)
+                                / nodes, 1.0)
+
+        def rebinned():
+            b = bins.rebin(moved)
+            binning.scatter_binned(b, field)
+            binning.gather_binned(b, field, result)
+
+        rb_ms, _ = time_steps(torch, rebinned, max(3, args.steps // 4), 1, stream)
+        rbo_ms, _ = time_steps(torch, lambda: bins.rebin(moved), max(3, args.steps // 4), 1, stream)
+        full_ms, _ = time_steps(torch, lambda: binning.bin_markers(moved, bins.values, (1, 2), geometry),
+                                max(3, args.steps // 4), 1, stream)
+        # the deposit of the re-binned markers against that of a fresh binning of them
+        fresh = binning.bin_markers(moved, bins.values, (1, 2), geometry)
+        again = bins.rebin(moved)
+        f1, f2 = torch.zeros_like(field), torch.zeros_like(field)
+        binning.scatter_binned(again, f1)
+        binning.scatter_binned(fresh, f2)
+        rebin_equal = bool(torch.equal(f1, f2) and torch.equal(again.start, fresh.start))
+        del fresh, again, f1, f2, moved
+
         # ---- the drop-in API (Field.scatter + Field.gather, the reference's field.py:1097-1180):
         # positions given each call (the library bins inside both calls), and the positions'
         # bins made once with Field.bin_markers and passed instead ----
@@ -801,6 +829,12 @@ def main() -> None:
         line["variants"] = {
             "binned_sort_inside_step_markers_per_s": n / (sh_ms * 1e-3),
             "binned_sort_inside_step_ms": sh_ms,
+            "rebinned_inside_step_ms": rb_ms,
+            "rebinned_inside_step_markers_per_s": n / (rb_ms * 1e-3),
+            "rebin_only_ms": rbo_ms,
+            "full_sort_of_moved_markers_ms": full_ms,
+            "rebin_deposit_equals_fresh_binning": rebin_equal,
+            "rebin_workload": "the binned markers moved by uniform [-0.5, 0.5] cell per axis (periodic wrap)",
             "field_api_positions_step_ms": api_ms,
             "field_api_bins_kept_step_ms": apib_ms,
             "fp64_atomic_cell_sorted_markers_per_s": n / (at_s * 1e-3),

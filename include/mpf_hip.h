@@ -339,6 +339,23 @@ int mpf_bin_markers(int kind, int degree, int ndim, const void *positions, int64
                     void *sorted_values, int64_t *order, uint32_t *start, double *scalars, int scale_exp,
                     mpf_context *ctx, mpf_stream stream);
 
+/* Bin again markers kept in the order of earlier bins, after they moved (a pusher updating
+ * the sorted arrays in place): positions/values are in the old bins' order, old_order and
+ * old_start the old bins' permutation and offsets (same kind, degree and grid).  Writes the
+ * outputs of mpf_bin_markers for the same markers -- the same offsets and scale, each bin's
+ * markers in some order -- into arrays that must not overlap the inputs, with order[]
+ * relative to the markers' first input (old_order composed).  3D: an incremental pass (the
+ * markers that stay in their tile of 8 x 8 x 16 cells or reach a neighbouring one) that
+ * synchronizes the stream once to read a counter, and sorts from scratch when a marker went
+ * further; 1D and 2D: mpf_bin_markers.  No counterpart in the reference, whose banded CPU
+ * buffers assume markers kept sorted by the caller (scatter_linear_3d_pyccel.py:11-16). */
+int mpf_rebin_markers(int kind, int degree, int ndim, const void *positions, int64_t pos_stride, const void *values,
+                      int64_t val_stride, int dtype, int64_t n, int64_t ncomp, const int64_t *old_order,
+                      const uint32_t *old_start, const double *lower, const double *cell_size,
+                      const int64_t *num_nodes, const int64_t *code, void *sorted_positions, void *sorted_values,
+                      int64_t *order, uint32_t *start, double *scalars, int scale_exp, mpf_context *ctx,
+                      mpf_stream stream);
+
 /* Deposit binned markers (scatter_quadratic_3d etc., the tiled form): one workgroup per
  * tile of cells accumulates int64 fixed point in LDS and adds the tile into field once;
  * bit-reproducible.  3D, codes 0/1: linear, quadratic and spline 0..2 with 1 or 3

@@ -73,6 +73,8 @@ SIGNATURES: dict[str, list] = {
     "mpf_scatter_binned": [_INT, _INT, _INT, _P, _P, _INT, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                            _P],
     "mpf_gather_binned": [_INT, _INT, _INT, _P, _INT, _I64, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P],
+    "mpf_rebin_markers": [_INT, _INT, _INT, _P, _I64, _P, _I64, _INT, _I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P,
+                          _P, _P, _P, _INT, _P, _P],
     "mpf_bin_items": [_INT, _P, _P, _P, _P, _P, _P],
     "mpf_gather_derivative_binned": [_INT, _INT, _INT, _P, _INT, _I64, _P, _P, _P, _I64, _P, _I64, _P, _I64, _P, _P,
                                      _P, _P, _P, _P],

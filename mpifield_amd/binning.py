@@ -88,6 +88,43 @@ class MarkerBins:
             return None, None
         return self.tables.data_ptr(), self.table_counts.data_ptr()
 
+    def rebin(self, positions: torch.Tensor | None = None, values=None, scale_exp: int = _lib.SCALE_AUTO
+              ) -> "MarkerBins":
+        """New bins of these markers after they moved (mpf_rebin_markers): ``positions``
+        (default: this object's sorted positions, updated in place by the caller) and
+        ``values`` (default: the sorted values) are in this object's bins order.  The result
+        holds its own arrays; its ``order`` still maps to the first input.  3D bins move only
+        the markers that changed tile (to a neighbouring one); 1D/2D bins sort again."""
+        ndim, lower, cell, num, code = _geometry_arrays(self.geometry)
+        pos, ps = K.marker_rows(self.positions if positions is None else positions, ndim)
+        n = pos.shape[1]
+        if n != self.n:
+            raise ValueError(f"these bins hold {self.n} markers, got {n} positions")
+        if pos.dtype != self.positions.dtype or pos.device != self.positions.device:
+            raise ValueError("positions must have the bins' dtype and device")
+        dev = pos.device
+        values = self.values if values is None else values
+        if values is not None:
+            val, vs = K.marker_rows(values, None, pos.dtype)
+            if val.shape[1] != n:
+                raise ValueError(f"{n} positions for {val.shape[1]} values")
+            ncomp = val.shape[0]
+            sval = torch.empty((ncomp, n), dtype=pos.dtype, device=dev)
+            val_ptr, sval_ptr = val.data_ptr(), sval.data_ptr()
+        else:
+            vs, ncomp, sval, val_ptr, sval_ptr = 0, 0, None, None, None
+        spos = torch.empty((ndim, n), dtype=pos.dtype, device=dev)
+        order = torch.empty(n, dtype=torch.int64, device=dev)
+        start = torch.empty_like(self.start)
+        scalars = torch.empty(3, dtype=torch.float64, device=dev)
+        with torch.cuda.device(dev):
+            _lib.call("mpf_rebin_markers", self.kind, self.degree, ndim, pos.data_ptr(), ps, val_ptr, vs,
+                      K._dtype_code(pos), n, ncomp, self.order.data_ptr(), self.start.data_ptr(), lower.ctypes.data,
+                      cell.ctypes.data, num.ctypes.data, code.ctypes.data, spos.data_ptr(), sval_ptr,
+                      order.data_ptr(), start.data_ptr(), scalars.data_ptr(), int(scale_exp), None,
+                      K.stream_handle(dev))
+        return MarkerBins(self.kind, self.degree, self.geometry, spos, sval, order, start, scalars)
+
     def unsort(self, sorted_rows: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """Rows (k, N) in bins order back to the input order (mpf_unsort_rows)."""
         if out is None:

@@ -186,6 +186,11 @@ template <class T, int KIND, int P, int NDIM>
 __device__ __forceinline__ uint32_t marker_key(const T *__restrict__ pos, int64_t ps, const T *__restrict__ val,
                                                int64_t vs, int ncomp, int64_t m, const Grid &g, uint32_t dead_key,
                                                double &vmax) {
+  // the positions are loaded before the values decide whether they are needed, so that the
+  // loads go out together
+  T x[NDIM];
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) x[d] = pos[d * ps + m];
   bool live = ncomp == 0;
   for (int c = 0; c < ncomp; ++c) {
     const double v = fabs((double)val[c * vs + m]);
@@ -197,7 +202,7 @@ __device__ __forceinline__ uint32_t marker_key(const T *__restrict__ pos, int64_
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
     double w[4];
-    const double rel = ((double)pos[d * ps + m] - g.lower[d]) * g.inverse[d];
+    const double rel = ((double)x[d] - g.lower[d]) * g.inverse[d];
     const int num = (int)g.num[d];
     int a = anchor_axis<KIND, P>(rel, num, g.code[d], w);
     if (g.code[d] == 1 && a >= num) a -= num;
@@ -558,6 +563,403 @@ __global__ void __launch_bounds__(BIN_THREADS) bin_cells_direct(
   __syncthreads();
   group_offsets(records, stride, lo, hi, first_key, keys, cnt, wave_sums, start);
   place_direct<T>(records, stride, lo, hi, first_key, cnt, ndim, ncomp, n, spos, sval, order);
+}
+
+// ---- re-binning markers kept in bins order ----------------------------------------------
+//
+// A particle code that keeps its markers in the bins' order moves them and bins them again:
+// most stay in their 3D tile and most of the others reach a neighbouring tile.  Pass 1 (one
+// workgroup per old tile, and one for the old dead bin) computes every marker's new key
+// (marker_key, so keys and the value maximum are those a fresh binning finds), counts the
+// markers of each destination tile (a global atomic per destination tile and workgroup) and
+// lists the markers that leave for a neighbouring tile, grouped by destination, in the old
+// tile's own range of a mover array.  Markers whose values became all zero go to one dead list; markers that go further
+// are counted as far, and the call then sorts from scratch.  An exclusive scan of the tile
+// counts gives every new tile its first slot.  Pass 2 (one workgroup per new tile, and one
+// for the dead bin) visits the tile's markers -- those that stayed, from its old range, and
+// those that came, from the parts of its neighbours' mover lists sent to it -- counts them
+// per cell in LDS, writes the cells' offsets, ranks each marker within its cell and copies
+// the rows in the new order through that permutation.  The order within a cell is what
+// the LDS atomics give, as in the full sort.
+constexpr int REBIN_THREADS = 256;       // pass 1 and the dead bin
+constexpr int REBIN_PLACE_THREADS = 512;  // pass 2
+
+struct TileGrid {
+  int nt[3];    // tiles per axis
+  int code[3];  // 1 periodic
+  int tiles;    // nt[0] nt[1] nt[2]; the dead bin is tile `tiles`
+};
+
+__device__ __forceinline__ void tile_coords(const TileGrid &tg, int tile, int (&c)[3]) {
+  c[2] = tile % tg.nt[2];
+  c[1] = (tile / tg.nt[2]) % tg.nt[1];
+  c[0] = tile / (tg.nt[2] * tg.nt[1]);
+}
+
+// The offsets along axis d from tile coordinate c that reach distinct tiles, 0 first:
+// a periodic axis of two tiles has +1 only (it is also -1), one of one tile none; a bounded
+// axis stops at its ends.
+__device__ __forceinline__ int axis_offsets(const TileGrid &tg, int d, int c, int (&off)[3]) {
+  int k = 0;
+  off[k++] = 0;
+  if (tg.code[d] == 1) {
+    if (tg.nt[d] >= 2) off[k++] = 1;
+    if (tg.nt[d] >= 3) off[k++] = -1;
+  } else {
+    if (c + 1 < tg.nt[d]) off[k++] = 1;
+    if (c >= 1) off[k++] = -1;
+  }
+  return k;
+}
+
+__device__ __forceinline__ int axis_step(const TileGrid &tg, int d, int c, int o) {
+  int v = c + o;
+  if (tg.code[d] == 1) v = v < 0 ? v + tg.nt[d] : (v >= tg.nt[d] ? v - tg.nt[d] : v);
+  return v;
+}
+
+__device__ __forceinline__ int tile_at(const TileGrid &tg, int x, int y, int z) {
+  return (x * tg.nt[1] + y) * tg.nt[2] + z;
+}
+
+// Tile u's place 0..26 among tile t's neighbours ((o0 + 1) 9 + (o1 + 1) 3 + o2 + 1 for the
+// offsets axis_offsets gives), or -1 when u is further.
+__device__ __forceinline__ int neighbour_slot(const TileGrid &tg, int t, int u) {
+  int a[3], b[3];
+  tile_coords(tg, t, a);
+  tile_coords(tg, u, b);
+  int slot = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int o = b[d] - a[d];
+    if (tg.code[d] == 1) {
+      if (o > 1) o -= tg.nt[d];
+      if (o < -1) o += tg.nt[d];
+      if (tg.nt[d] == 2 && o == -1) o = 1;
+    }
+    if (o < -1 || o > 1) return -1;
+    slot = slot * 3 + o + 1;
+  }
+  return slot;
+}
+
+// The old dead bin is split over REBIN_DEAD_GROUPS workgroups of pass 1, and the new one is
+// filled by as many workgroups of pass 3 (its markers need no order).
+constexpr int REBIN_DEAD_GROUPS = 64;
+
+// Lane-aggregated LDS counter: every lane with `take` gets a distinct slot of *counter,
+// one atomic per wavefront.
+__device__ __forceinline__ uint32_t wave_take(uint32_t *counter, bool take) {
+  const unsigned long long mask = __ballot(take);
+  if (!mask) return 0u;
+  const int lane = threadIdx.x & 63, first = __ffsll((long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == first) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = __shfl(base, first, 64);
+  return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// Pass 1 also writes each tile's mover list grouped by destination: moff[tile * 28 + s]
+// is where the markers for its neighbour s (0..26, neighbour_slot) start within the list,
+// moff[tile * 28 + 27] its length.
+constexpr int REBIN_MOFF = 28;
+constexpr int REBIN_UNROLL = 4;  // markers per work-item and trip, their loads issued together
+constexpr int REBIN_MSTAGE = 1024;  // movers of a tile that pass 1 keeps in LDS
+
+template <class T, int KIND, int P>
+__global__ void __launch_bounds__(REBIN_THREADS) rebin_count_kernel(
+    const T *__restrict__ pos, int64_t ps, const T *__restrict__ val, int64_t vs, int ncomp, Grid g, TileGrid tg,
+    const uint32_t *__restrict__ ostart, uint32_t *__restrict__ keys, uint2 *__restrict__ movers,
+    uint32_t *__restrict__ moff, uint32_t *__restrict__ total, uint32_t *__restrict__ dead_list,
+    uint32_t *__restrict__ counters, unsigned long long *__restrict__ maxbits) {
+  __shared__ uint32_t dest[27], doff[28];
+  __shared__ uint32_t stayed, died, moved;
+  __shared__ uint2 staged[REBIN_MSTAGE];  // the movers (index, key) as found
+  __shared__ uint8_t staged_slot[REBIN_MSTAGE];
+  const int t = threadIdx.x;
+  const uint32_t dead_key = (uint32_t)tg.tiles * TILE_CELLS;
+  // workgroups past the tiles take one piece each of the old dead bin
+  const int item = (int)blockIdx.x;
+  const bool dead_part = item >= tg.tiles;
+  const int tile = dead_part ? tg.tiles : item;
+  if (t < 27) dest[t] = 0;
+  if (t == 0) stayed = died = moved = 0;
+  __syncthreads();
+  uint32_t lo = ostart[(int64_t)tile * TILE_CELLS];
+  uint32_t hi = ostart[dead_part ? (int64_t)dead_key + 1 : (int64_t)(tile + 1) * TILE_CELLS];
+  if (dead_part) {
+    const uint32_t piece = (hi - lo + REBIN_DEAD_GROUPS - 1) / REBIN_DEAD_GROUPS, k = item - tg.tiles;
+    const uint32_t a = lo + k * piece;
+    lo = a < hi ? a : hi;
+    hi = lo + piece < hi ? lo + piece : hi;
+  }
+  double vmax = 0.0;
+  // whole wavefronts go round the loop together (the lane-aggregated counters need them)
+  for (uint32_t base = lo; base < hi; base += REBIN_THREADS * REBIN_UNROLL) {
+    uint32_t key[REBIN_UNROLL];
+#pragma unroll
+    for (int q = 0; q < REBIN_UNROLL; ++q) {
+      const uint32_t m = base + q * REBIN_THREADS + t;
+      key[q] = m < hi ? marker_key<T, KIND, P, 3>(pos, ps, val, vs, ncomp, m, g, dead_key, vmax) : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int q = 0; q < REBIN_UNROLL; ++q) {
+      const uint32_t m = base + q * REBIN_THREADS + t;
+      const bool in = key[q] != 0xFFFFFFFFu;
+      if (in) keys[m] = key[q];
+      const int u = in ? (int)(key[q] / TILE_CELLS) : -1;
+      const bool stay = in && u == tile;  // also a dead marker that stays dead
+      const bool die = in && !stay && u == tg.tiles;
+      const int s = in && !stay && !die && !dead_part ? neighbour_slot(tg, tile, u) : -1;
+      if (in && !stay && !die && s < 0) atomicAdd(&counters[1], 1u);  // further than a neighbouring tile
+      if (die) dead_list[atomicAdd(&counters[0], 1u)] = m;
+      (void)wave_take(&stayed, stay);
+      (void)wave_take(&died, die);
+      const uint32_t r = wave_take(&moved, s >= 0);
+      if (s >= 0) {
+        atomicAdd(&dest[s], 1u);
+        if (r < (uint32_t)REBIN_MSTAGE) {
+          staged[r] = make_uint2(m, key[q]);
+          staged_slot[r] = (uint8_t)s;
+        }
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(vmax, off, 64);
+    vmax = o > vmax ? o : vmax;
+  }
+  if ((t & 63) == 0 && vmax > 0.0) atomicMax(maxbits, (unsigned long long)__double_as_longlong(vmax));
+  __syncthreads();
+  if (t == 0) {
+    if (stayed) atomicAdd(&total[tile], stayed);
+    if (died) atomicAdd(&total[tg.tiles], died);
+    if (!dead_part) {
+      doff[0] = 0;
+      for (int i = 0; i < 27; ++i) doff[i + 1] = doff[i] + dest[i];
+    }
+  }
+  __syncthreads();
+  if (dead_part) return;
+  if (t < REBIN_MOFF) moff[(int64_t)tile * REBIN_MOFF + t] = doff[t];
+  if (t < 27 && dest[t]) {
+    int a[3];
+    tile_coords(tg, tile, a);
+    const int u = tile_at(tg, axis_step(tg, 0, a[0], t / 9 - 1), axis_step(tg, 1, a[1], (t / 3) % 3 - 1),
+                          axis_step(tg, 2, a[2], t % 3 - 1));
+    atomicAdd(&total[u], dest[t]);
+  }
+  if (doff[27] == 0) return;
+  // the movers, grouped by destination: from LDS, or (more than REBIN_MSTAGE of them) from
+  // the keys again (just written, in cache)
+  if (moved <= (uint32_t)REBIN_MSTAGE) {
+    for (uint32_t r = t; r < moved; r += REBIN_THREADS)
+      movers[lo + atomicAdd(&doff[staged_slot[r]], 1u)] = staged[r];
+    return;
+  }
+  for (uint32_t m = lo + t; m < hi; m += REBIN_THREADS) {
+    const uint32_t key = keys[m];
+    const int u = (int)(key / TILE_CELLS);
+    if (u == tile || u == tg.tiles) continue;
+    const int s = neighbour_slot(tg, tile, u);
+    if (s >= 0) movers[lo + atomicAdd(&doff[s], 1u)] = make_uint2(m, key);
+  }
+}
+
+// The sources of new tile u's markers as one index range: its old range (the markers that
+// stayed, among others), then the lists of markers its distinct neighbours sent to it, in LDS.
+struct RebinSources {
+  uint32_t lo, own;             // the old range of tile u
+  int count;                    // neighbours
+  uint32_t first[26], off[27];  // list start; prefix of the list lengths
+};
+
+__device__ __forceinline__ void rebin_sources(const TileGrid &tg, int u, const uint32_t *__restrict__ ostart,
+                                              const uint32_t *__restrict__ moff, RebinSources &src) {
+  const int t = threadIdx.x;
+  int a[3], ox[3], oy[3], oz[3];
+  tile_coords(tg, u, a);
+  const int nx = axis_offsets(tg, 0, a[0], ox), ny = axis_offsets(tg, 1, a[1], oy), nz = axis_offsets(tg, 2, a[2], oz);
+  const int k = nx * ny * nz - 1;  // the neighbours (offset 0 0 0 is the first combination)
+  if (t == 0) {
+    src.lo = ostart[(int64_t)u * TILE_CELLS];
+    src.own = ostart[(int64_t)(u + 1) * TILE_CELLS] - src.lo;
+    src.count = k;
+  }
+  if (t < k) {
+    const int c = t + 1, i = c / (ny * nz), j = (c / nz) % ny, l = c % nz;
+    const int nb = tile_at(tg, axis_step(tg, 0, a[0], ox[i]), axis_step(tg, 1, a[1], oy[j]), axis_step(tg, 2, a[2], oz[l]));
+    const int slot = neighbour_slot(tg, nb, u);  // u as neighbour nb sees it
+    const uint32_t *mo = moff + (int64_t)nb * REBIN_MOFF + slot;
+    src.first[t] = ostart[(int64_t)nb * TILE_CELLS] + mo[0];
+    src.off[t + 1] = mo[1] - mo[0];
+  }
+  __syncthreads();
+  if (t == 0) {
+    src.off[0] = 0;
+    for (int i = 1; i <= k; ++i) src.off[i] += src.off[i - 1];
+  }
+  __syncthreads();
+}
+
+// Calls f(m, key) for every marker of new tile u among its sources, REBIN_UNROLL candidates
+// per work-item and trip with their loads issued together.
+template <class F>
+__device__ __forceinline__ void rebin_visit(int u, const RebinSources &src, const uint32_t *__restrict__ keys,
+                                            const uint2 *__restrict__ movers, F &&f) {
+  const uint32_t total = src.own + src.off[src.count];
+  for (uint32_t base = 0; base < total; base += REBIN_PLACE_THREADS * REBIN_UNROLL) {
+    uint32_t m[REBIN_UNROLL], key[REBIN_UNROLL];
+#pragma unroll
+    for (int q = 0; q < REBIN_UNROLL; ++q) {
+      const uint32_t e = base + q * REBIN_PLACE_THREADS + threadIdx.x;
+      m[q] = 0xFFFFFFFFu;
+      key[q] = 0xFFFFFFFFu;
+      if (e < src.own) {
+        m[q] = src.lo + e;
+        key[q] = keys[m[q]];
+      } else if (e < total) {
+        const uint32_t j = e - src.own;
+        int lo = 0, hi = src.count;  // the list holding entry j: off[lo] <= j < off[lo + 1]
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (src.off[mid] <= j) lo = mid;
+          else hi = mid;
+        }
+        const uint2 mv = movers[src.first[lo] + (j - src.off[lo])];
+        m[q] = mv.x;
+        key[q] = mv.y;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < REBIN_UNROLL; ++q)
+      if (key[q] != 0xFFFFFFFFu && (int)(key[q] / TILE_CELLS) == u) f(m[q], key[q]);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void rebin_copy(const T *__restrict__ pos, int64_t ps, const T *__restrict__ val,
+                                           int64_t vs, int ncomp, int64_t n, const int64_t *__restrict__ oorder,
+                                           uint32_t m, uint32_t slot, T *__restrict__ spos, T *__restrict__ sval,
+                                           int64_t *__restrict__ order) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) spos[d * n + slot] = pos[d * ps + m];
+  for (int c = 0; c < ncomp; ++c) sval[c * n + slot] = val[c * vs + m];
+  order[slot] = oorder[m];
+}
+
+// Markers of a new tile that pass 2 stages through LDS (larger tiles are placed directly).
+constexpr int REBIN_STAGE = 8192;
+
+// NC >= 0: the component count fixed, every load of two markers issued before their stores;
+// NC < 0: ncomp at run time.
+template <class T, int NC>
+__global__ void __launch_bounds__(REBIN_PLACE_THREADS) rebin_place_kernel(
+    const T *__restrict__ pos, int64_t ps, const T *__restrict__ val, int64_t vs, int ncomp, int64_t n, TileGrid tg,
+    const uint32_t *__restrict__ ostart, const int64_t *__restrict__ oorder, const uint32_t *__restrict__ keys,
+    const uint2 *__restrict__ movers, const uint32_t *__restrict__ moff, const uint32_t *__restrict__ tstart,
+    T *__restrict__ spos, T *__restrict__ sval, int64_t *__restrict__ order, uint32_t *__restrict__ start) {
+  __shared__ uint32_t cnt[TILE_CELLS];
+  __shared__ uint32_t perm[REBIN_STAGE];  // the old index of the tile's i-th new marker
+  __shared__ uint32_t wave_sums[REBIN_PLACE_THREADS / 64];
+  __shared__ RebinSources src;
+  const int t = threadIdx.x, u = blockIdx.x;
+  for (int i = t; i < TILE_CELLS; i += REBIN_PLACE_THREADS) cnt[i] = 0;
+  rebin_sources(tg, u, ostart, moff, src);  // ends with a barrier
+  rebin_visit(u, src, keys, movers, [&](uint32_t, uint32_t key) { atomicAdd(&cnt[key % TILE_CELLS], 1u); });
+  __syncthreads();
+  block_exclusive_scan<REBIN_PLACE_THREADS>(cnt, TILE_CELLS, wave_sums);
+  const uint32_t base = tstart[u], size = tstart[u + 1] - base;
+  for (int i = t; i < TILE_CELLS; i += REBIN_PLACE_THREADS) start[(int64_t)u * TILE_CELLS + i] = base + cnt[i];
+  __syncthreads();  // every offset read before the placement moves them on
+  if (size > (uint32_t)REBIN_STAGE) {  // a crowded tile: each marker straight to its slot
+    rebin_visit(u, src, keys, movers, [&](uint32_t m, uint32_t key) {
+      rebin_copy<T>(pos, ps, val, vs, ncomp, n, oorder, m, base + atomicAdd(&cnt[key % TILE_CELLS], 1u), spos, sval,
+                    order);
+    });
+    return;
+  }
+  // the permutation in LDS, then the markers copied in order: contiguous writes, and reads
+  // within the tile's old range (runs of a cell's markers that stayed together)
+  rebin_visit(u, src, keys, movers, [&](uint32_t m, uint32_t key) { perm[atomicAdd(&cnt[key % TILE_CELLS], 1u)] = m; });
+  __syncthreads();
+  if constexpr (NC >= 0) {
+    constexpr int K = 2;
+    for (uint32_t i0 = 0; i0 < size; i0 += REBIN_PLACE_THREADS * K) {
+      uint32_t m[K];
+      int64_t o[K];
+      T p[K][3], v[K][NC > 0 ? NC : 1];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const uint32_t i = i0 + q * REBIN_PLACE_THREADS + t;
+        m[q] = perm[i < size ? i : size - 1];
+        o[q] = oorder[m[q]];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) p[q][d] = pos[d * ps + m[q]];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[q][c] = val[c * vs + m[q]];
+      }
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const uint32_t i = i0 + q * REBIN_PLACE_THREADS + t;
+        if (i < size) {
+          order[base + i] = o[q];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) spos[d * n + base + i] = p[q][d];
+#pragma unroll
+          for (int c = 0; c < NC; ++c) sval[c * n + base + i] = v[q][c];
+        }
+      }
+    }
+  } else {
+    for (uint32_t i = t; i < size; i += REBIN_PLACE_THREADS) rebin_copy<T>(pos, ps, val, vs, ncomp, n, oorder, perm[i], base + i,
+                                                                     spos, sval, order);
+  }
+}
+
+// The new dead bin: the old dead bin's markers that stayed dead and the dead list, in any
+// order, placed through one global cursor (counters[2]).
+template <class T>
+__global__ void __launch_bounds__(REBIN_THREADS) rebin_dead_kernel(
+    const T *__restrict__ pos, int64_t ps, const T *__restrict__ val, int64_t vs, int ncomp, int64_t n, TileGrid tg,
+    const uint32_t *__restrict__ ostart, const int64_t *__restrict__ oorder, const uint32_t *__restrict__ keys,
+    const uint32_t *__restrict__ tstart, const uint32_t *__restrict__ dead_list, uint32_t *__restrict__ counters,
+    T *__restrict__ spos, T *__restrict__ sval, int64_t *__restrict__ order, uint32_t *__restrict__ start) {
+  __shared__ uint32_t cursor;
+  const int64_t dead_key = (int64_t)tg.tiles * TILE_CELLS;
+  const uint32_t base = tstart[tg.tiles], lo = ostart[dead_key], hi = ostart[dead_key + 1], ndead = counters[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    start[dead_key] = base;
+    start[dead_key + 1] = (uint32_t)n;
+  }
+  const uint32_t stride = gridDim.x * REBIN_THREADS;
+  for (uint32_t round = blockIdx.x * REBIN_THREADS; round < (hi - lo) + ndead; round += stride) {
+    const uint32_t i = round + threadIdx.x;
+    uint32_t m = 0;
+    bool take = false;
+    if (i < hi - lo) {
+      m = lo + i;
+      take = keys[m] == (uint32_t)dead_key;
+    } else if (i < (hi - lo) + ndead) {
+      m = dead_list[i - (hi - lo)];
+      take = true;
+    }
+    if (threadIdx.x == 0) cursor = 0;
+    __syncthreads();
+    const uint32_t r = wave_take(&cursor, take);
+    __syncthreads();
+    if (threadIdx.x == 0 && cursor) cursor = atomicAdd(&counters[2], cursor);
+    __syncthreads();
+    if (take) rebin_copy<T>(pos, ps, val, vs, ncomp, n, oorder, m, base + cursor + r, spos, sval, order);
+    __syncthreads();
+  }
+}
+
+// order[i] = old_order[order[i]]: a permutation relative to the previous bins, made
+// relative to the markers' first input.
+__global__ void compose_order_kernel(int64_t *__restrict__ order, const int64_t *__restrict__ old_order, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    order[i] = old_order[order[i]];
 }
 
 // ---- exact fixed-point accumulation in float64 registers --------------------------------
@@ -2222,6 +2624,184 @@ extern "C" int mpf_bin_markers(int kind, int degree, int ndim, const void *posit
                                         sorted_values, order, start, stream);
   if (rc) return rc;
   // the scale, from the maximum the count pass found
+  rc = prepare_scale(ctx, nullptr, 0, dtype, 1, n, 1.0, scale_exp, stream);
+  if (rc) return rc;
+  MPF_CHECK(hipMemcpyAsync(scalars, scale_slot(ctx), 24, hipMemcpyDeviceToDevice, stream));
+  return MPF_OK;
+}
+
+template <class T, int KIND, int P>
+static void launch_rebin_count(const void *pos, int64_t ps, const void *val, int64_t vs, int ncomp, const Grid &g,
+                               const TileGrid &tg, const uint32_t *ostart, uint32_t *keys, uint2 *movers,
+                               uint32_t *moff, uint32_t *total, uint32_t *dead_list, uint32_t *counters,
+                               unsigned long long *maxbits, hipStream_t stream) {
+  hipLaunchKernelGGL((rebin_count_kernel<T, KIND, P>), dim3(tg.tiles + REBIN_DEAD_GROUPS), dim3(REBIN_THREADS), 0, stream,
+                     (const T *)pos, ps, (const T *)val, vs, ncomp, g, tg, ostart, keys, movers, moff, total,
+                     dead_list, counters, maxbits);
+}
+
+template <class T>
+static void dispatch_rebin_count(int kind, int degree, const void *pos, int64_t ps, const void *val, int64_t vs,
+                                 int ncomp, const Grid &g, const TileGrid &tg, const uint32_t *ostart, uint32_t *keys,
+                                 uint2 *movers, uint32_t *moff, uint32_t *total, uint32_t *dead_list,
+                                 uint32_t *counters, unsigned long long *maxbits, hipStream_t stream) {
+#define RC(K, P) \
+  launch_rebin_count<T, K, P>(pos, ps, val, vs, ncomp, g, tg, ostart, keys, movers, moff, total, dead_list, \
+                              counters, maxbits, stream)
+  if (kind == LINEAR) RC(LINEAR, 1);
+  else if (kind == QUADRATIC) RC(QUADRATIC, 2);
+  else if (degree == 0) RC(SPLINE, 0);
+  else if (degree == 1) RC(SPLINE, 1);
+  else if (degree == 2) RC(SPLINE, 2);
+  else RC(SPLINE, 3);
+#undef RC
+}
+
+static int compose_order(int64_t *order, const int64_t *old_order, int64_t n, hipStream_t stream) {
+  if (n == 0) return MPF_OK;
+  hipLaunchKernelGGL(compose_order_kernel, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, stream, order, old_order, n);
+  MPF_LAUNCH_CHECK();
+  return MPF_OK;
+}
+
+static bool ranges_meet(const void *a, size_t na, const void *b, size_t nb) {
+  const char *pa = (const char *)a, *pb = (const char *)b;
+  return a && b && pa < pb + nb && pb < pa + na;
+}
+
+/* Bin again markers that are in the order of earlier bins (old_start, old_order from
+ * mpf_bin_markers or this call, same kind/degree/grid), after they moved; the outputs are
+ * those of mpf_bin_markers on the same markers (the same bin offsets and scale, each bin's
+ * markers in some order), with order[] relative to the markers' first input (old_order
+ * composed).  3D: an incremental pass over the tiles (markers that stay in their tile or
+ * reach a neighbouring one); it reads back a counter (one stream synchronization) and sorts
+ * from scratch when a marker went further.  1D/2D: mpf_bin_markers. */
+extern "C" int mpf_rebin_markers(int kind, int degree, int ndim, const void *positions, int64_t pos_stride,
+                                 const void *values, int64_t val_stride, int dtype, int64_t n, int64_t ncomp,
+                                 const int64_t *old_order, const uint32_t *old_start, const double *lower,
+                                 const double *cell_size, const int64_t *num_nodes, const int64_t *code,
+                                 void *sorted_positions, void *sorted_values, int64_t *order, uint32_t *start,
+                                 double *scalars, int scale_exp, mpf_context *ctx, mpf_stream stream_) {
+  hipStream_t stream = (hipStream_t)stream_;
+  const size_t elem = dtype == F64 ? 8 : 4;
+  if (n > 0 && (ranges_meet(positions, (size_t)(ndim > 0 ? ndim : 0) * pos_stride * elem, sorted_positions,
+                            (size_t)(ndim > 0 ? ndim : 0) * n * elem) ||
+                ranges_meet(values, (size_t)ncomp * val_stride * elem, sorted_values, (size_t)ncomp * n * elem) ||
+                ranges_meet(old_order, (size_t)n * 8, order, (size_t)n * 8) || !old_order || !old_start)) {
+    set_error("rebin_markers: the new bins need arrays of their own (they are read from the old ones)");
+    return MPF_ERR_ARGUMENT;
+  }
+  const bool tiles3d = ndim == 3 && n > 0;
+  if (!tiles3d) {
+    int rc = mpf_bin_markers(kind, degree, ndim, positions, pos_stride, values, val_stride, dtype, n, ncomp, lower,
+                             cell_size, num_nodes, code, sorted_positions, sorted_values, order, start, scalars,
+                             scale_exp, ctx, stream_);
+    if (rc) return rc;
+    return compose_order(order, old_order, n, stream);
+  }
+  if (kind < LINEAR || kind > SPLINE || (kind == SPLINE && (degree < 0 || degree > 3))) {
+    set_error("rebin_markers: kind 0..2 and spline degree 0..3");
+    return MPF_ERR_ARGUMENT;
+  }
+  if (dtype != F32 && dtype != F64) {
+    set_error("dtype must be MPF_F32 or MPF_F64, got %d", dtype);
+    return MPF_ERR_ARGUMENT;
+  }
+  if (n > 0x7FFFFFFFll || ncomp < 0 || ncomp > 64 || (ncomp > 0 && !values)) {
+    set_error("rebin_markers: need 0 <= n < 2^31 and 0 <= ncomp <= 64 (0 only without values)");
+    return MPF_ERR_ARGUMENT;
+  }
+  Grid g;
+  int rc = make_grid(g, ndim, kind, kind == LINEAR ? 1 : (kind == QUADRATIC ? 2 : degree), lower, cell_size, num_nodes,
+                     code);
+  if (rc) return rc;
+  for (int d = 0; d < 3; ++d)
+    if (code[d] == 2) {
+      set_error("rebin_markers: clamped axes are not binned (the direct kernels take them)");
+      return MPF_ERR_UNSUPPORTED;
+    }
+  TileGrid tg;
+  tg.nt[0] = (int)((num_nodes[0] + TILE_X - 1) / TILE_X);
+  tg.nt[1] = (int)((num_nodes[1] + TILE_Y - 1) / TILE_Y);
+  tg.nt[2] = (int)((num_nodes[2] + TILE_Z - 1) / TILE_Z);
+  for (int d = 0; d < 3; ++d) tg.code[d] = (int)code[d];
+  const int64_t tiles = (int64_t)tg.nt[0] * tg.nt[1] * tg.nt[2];
+  if ((tiles + 1) * TILE_CELLS + 2 > 0x7FFFFFFFll) {
+    set_error("rebin_markers: too many cells for 32-bit keys");
+    return MPF_ERR_ARGUMENT;
+  }
+  tg.tiles = (int)tiles;
+  if (!ctx) ctx = default_context(stream);
+  if (!ctx) return MPF_ERR_HIP;
+  // scratch: keys (input order), movers (the old tiles' ranges), then the dead list, the
+  // tile totals, their scan, the movers' offsets by destination and three counters
+  const size_t nn = (size_t)n, T1 = (size_t)tiles + 1;
+  rc = ensure(ctx->sort_keys, nn * 4, false, stream);
+  if (rc) return rc;
+  rc = ensure(ctx->staging, nn * 8, false, stream);
+  if (rc) return rc;
+  const size_t dead_bytes = (nn * 4 + 255) & ~(size_t)255, table_bytes = ((T1 + 1) * 4 + 255) & ~(size_t)255,
+               moff_bytes = ((size_t)tiles * REBIN_MOFF * 4 + 255) & ~(size_t)255;
+  rc = ensure(ctx->rebin, dead_bytes + 2 * table_bytes + moff_bytes + 256, false, stream);
+  if (rc) return rc;
+  char *base = (char *)ctx->rebin.ptr;
+  uint32_t *dead_list = (uint32_t *)base;
+  uint32_t *total = (uint32_t *)(base + dead_bytes), *tstart = (uint32_t *)(base + dead_bytes + table_bytes),
+           *moff = (uint32_t *)(base + dead_bytes + 2 * table_bytes),
+           *counters = (uint32_t *)(base + dead_bytes + 2 * table_bytes + moff_bytes);
+  rc = zero_async(total, table_bytes, stream);
+  if (rc) return rc;
+  rc = zero_async(counters, 256, stream);
+  if (rc) return rc;
+  rc = ensure(ctx->scalars, 64, true, stream);
+  if (rc) return rc;
+  unsigned long long *maxbits = (unsigned long long *)ctx->scalars.ptr;
+  rc = zero_async(maxbits, 8, stream);
+  if (rc) return rc;
+  uint32_t *keys = (uint32_t *)ctx->sort_keys.ptr;
+  uint2 *movers = (uint2 *)ctx->staging.ptr;
+  if (dtype == F32)
+    dispatch_rebin_count<float>(kind, degree, positions, pos_stride, values, val_stride, (int)ncomp, g, tg, old_start,
+                                keys, movers, moff, total, dead_list, counters, maxbits, stream);
+  else
+    dispatch_rebin_count<double>(kind, degree, positions, pos_stride, values, val_stride, (int)ncomp, g, tg,
+                                 old_start, keys, movers, moff, total, dead_list, counters, maxbits, stream);
+  MPF_LAUNCH_CHECK();
+  size_t temp = 0;
+  MPF_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, temp, total, tstart, (int)T1, stream));
+  rc = ensure(ctx->sort_tmp, temp, false, stream);
+  if (rc) return rc;
+  MPF_CHECK(hipcub::DeviceScan::ExclusiveSum(ctx->sort_tmp.ptr, temp, total, tstart, (int)T1, stream));
+  uint32_t far = 0;
+  MPF_CHECK(hipMemcpyAsync(&far, counters + 1, 4, hipMemcpyDeviceToHost, stream));
+  MPF_CHECK(hipStreamSynchronize(stream));
+  if (far > 0) {  // a marker went further than a neighbouring tile: sort from scratch
+    rc = mpf_bin_markers(kind, degree, ndim, positions, pos_stride, values, val_stride, dtype, n, ncomp, lower,
+                         cell_size, num_nodes, code, sorted_positions, sorted_values, order, start, scalars,
+                         scale_exp, ctx, stream_);
+    if (rc) return rc;
+    return compose_order(order, old_order, n, stream);
+  }
+#define PLACE_NC(T, NC)                                                                                               \
+  hipLaunchKernelGGL((rebin_place_kernel<T, NC>), dim3(tg.tiles), dim3(REBIN_PLACE_THREADS), 0, stream,                 \
+                     (const T *)positions,                                                                            \
+                     pos_stride, (const T *)values, val_stride, (int)ncomp, n, tg, old_start, old_order, keys, movers,  \
+                     moff, tstart, (T *)sorted_positions, (T *)sorted_values, order, start)
+#define PLACE(T)                                                                                                      \
+  {                                                                                                                   \
+    if (ncomp == 0) PLACE_NC(T, 0);                                                                                   \
+    else if (ncomp == 1) PLACE_NC(T, 1);                                                                              \
+    else if (ncomp == 3) PLACE_NC(T, 3);                                                                              \
+    else PLACE_NC(T, -1);                                                                                             \
+    hipLaunchKernelGGL((rebin_dead_kernel<T>), dim3(REBIN_DEAD_GROUPS), dim3(REBIN_THREADS), 0, stream,               \
+                       (const T *)positions, pos_stride, (const T *)values, val_stride, (int)ncomp, n, tg, old_start, \
+                       old_order, keys, tstart, dead_list, counters, (T *)sorted_positions, (T *)sorted_values,      \
+                       order, start);                                                                                 \
+  }
+  if (dtype == F32) PLACE(float) else PLACE(double)
+#undef PLACE
+#undef PLACE_NC
+  MPF_LAUNCH_CHECK();
   rc = prepare_scale(ctx, nullptr, 0, dtype, 1, n, 1.0, scale_exp, stream);
   if (rc) return rc;
   MPF_CHECK(hipMemcpyAsync(scalars, scale_slot(ctx), 24, hipMemcpyDeviceToDevice, stream));

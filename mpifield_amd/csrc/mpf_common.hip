@@ -323,7 +323,7 @@ void mpf_context_destroy(mpf_context *ctx) {
   if (!ctx) return;
   (void)hipDeviceSynchronize();
   mpf::Buffer *all[] = {&ctx->accum,   &ctx->scalars, &ctx->sort_keys, &ctx->sort_tmp,    &ctx->offsets,
-                        &ctx->staging, &ctx->path_in, &ctx->bins,      &ctx->path_sorted};
+                        &ctx->staging, &ctx->path_in, &ctx->bins,      &ctx->path_sorted, &ctx->rebin};
   for (mpf::Buffer *b : all)
     if (b->ptr) (void)hipFree(b->ptr);
   delete ctx;

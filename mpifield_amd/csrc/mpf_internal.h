@@ -35,6 +35,7 @@ struct mpf_context {
   mpf::Buffer path_sorted;  // the binned paths: start, rows, order, bin offsets, scalars
   mpf::Buffer bins;      // node deposits and gathers binned inside the call: sorted rows, order,
                          // bin offsets, scale, results in bins order
+  mpf::Buffer rebin;     // re-binning: markers leaving their tile, per-tile counts and starts
 };
 
 namespace mpf {
