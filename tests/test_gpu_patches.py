@@ -21,13 +21,15 @@ if torch.cuda.is_available():
 NAME = {"linear": "linear", "quadratic": "quadratic"}
 
 
-def make_batch(seed=0, n_patches=40, ndim=2, dtype=np.float64):
+def make_batch(seed=0, n_patches=40, ndim=2, dtype=np.float64, sides=None):
     rng = np.random.default_rng(seed)
     batch = P.PatchBatch(ndim=ndim, ncomp=1)
     spec, pos_parts, val_parts = [], [], []
     choices = [m.LINEAR, m.QUADRATIC, m.CUBIC, spline_nodes("spline0", 0)]
+    if sides is None:
+        sides = (16, 64) if ndim == 2 else (8, 16)
     for i in range(n_patches):
-        side = rng.integers(16, 65, ndim) if ndim == 2 else rng.integers(8, 17, ndim)
+        side = rng.integers(sides[0], sides[1] + 1, ndim)
         cell = rng.uniform(0.01, 0.05, ndim)
         lower = rng.uniform(-1, 1, ndim)
         weighting = choices[i % len(choices)]
@@ -97,3 +99,29 @@ def test_autotune_and_graph_replay_keep_the_result():
         torch.cuda.synchronize()
         assert torch.equal(fields, eager), k
         _ = (fields * 2).sum().item()  # eager work and a sync between replays
+
+
+def test_float32_patches_of_16_to_128_nodes_match_the_oracle():
+    """The batched-patch workload's shape: 2D patches of 16 to 128 nodes per side, float32
+    markers, spline degrees 0 to 3, periodic and bounded axes; each patch against the oracle
+    fed the same float32 values (deposit in fixed point, gather in float32)."""
+    batch, spec, pos, val = make_batch(seed=21, n_patches=48, ndim=2, dtype=np.float32, sides=(16, 128))
+    sizes = [int(np.prod(sp[2])) for sp in spec]
+    assert min(sizes) < 48 * 48 and max(sizes) > 100 * 100
+    pos_d, val_d = torch.as_tensor(pos, device="cuda"), torch.as_tensor(val, device="cuda")
+    batch.build(chunk=4096)
+    fields = torch.zeros(batch.field_values, dtype=torch.float64, device="cuda")
+    batch.scatter(pos_d, val_d, fields)
+    probe = torch.as_tensor(np.random.default_rng(22).normal(size=batch.field_values), device="cuda")
+    result = torch.empty((1, pos.shape[1]), dtype=torch.float32, device="cuda")
+    batch.gather(pos_d, probe, result)
+    got, res, probe_h = fields.cpu().numpy(), result.cpu().numpy().astype(np.float64), probe.cpu().numpy()
+    pos64, val64 = pos.astype(np.float64), val.astype(np.float64)
+    for i, (lower, cell, side, weighting, periodic, offset, sl) in enumerate(spec):
+        want, g, kind = oracle_patch(weighting, pos64[:, sl], val64[:, sl], lower, cell, side, periodic)
+        mine = got[offset : offset + want.size].reshape(want.shape)
+        assert np.abs(mine - want).max() <= 1e-10 * max(1.0, np.abs(want).max()), i
+        f = probe_h[offset : offset + want.size].reshape(want.shape)
+        wg = np.zeros((1, sl.stop - sl.start))
+        orc.gather(kind, pos64[:, sl], f, wg, *g, degree=weighting.degree)
+        np.testing.assert_allclose(res[:, sl], wg, rtol=2e-6, atol=2e-6 * max(1.0, np.abs(wg).max()))
