@@ -369,7 +369,8 @@ int mpf_scatter_binned(int kind, int degree, int ndim, const void *sorted_positi
                        mpf_stream stream);
 
 /* Gather at binned markers (the tiled form of gather_quadratic_3d etc.); result (ncomp, n)
- * in the sorted order, contiguous rows.  Same cases and tables as mpf_scatter_binned. */
+ * in the sorted order, contiguous rows.  Same cases as mpf_scatter_binned.  item_tables and
+ * item_counts are accepted and not read: the gather takes one marker per work-item. */
 int mpf_gather_binned(int kind, int degree, int ndim, const void *sorted_positions, int dtype, int64_t n,
                       const uint32_t *start, const uint16_t *item_tables, const uint32_t *item_counts,
                       const double *field, int64_t ncomp, void *result, const double *lower, const double *cell_size,

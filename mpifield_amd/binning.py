@@ -227,8 +227,7 @@ def gather_binned(bins: MarkerBins, field: torch.Tensor, result: torch.Tensor) -
     ncomp = field.shape[-1]
     _check_result(bins, result, ncomp)
     with torch.cuda.device(field.device):
-        # the gather builds its tiles' work lists itself: the build hides behind its footprint
-        # load, and loading a list costs it registers (measured slower with the lists)
+        # the gather takes one marker per work-item and reads no work lists
         _lib.call("mpf_gather_binned", bins.kind, bins.degree, ndim, bins.positions.data_ptr(),
                   K._dtype_code(bins.positions), bins.n, bins.start.data_ptr(), None, None,
                   field.data_ptr(), ncomp, result.data_ptr(), lower.ctypes.data, cell.ctypes.data, num.ctypes.data,

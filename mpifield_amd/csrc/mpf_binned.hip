@@ -1493,7 +1493,8 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 2 : 1) scatter_tiled(con
   // start in the row table).  Same-colour footprints never overlap, so every node receives
   // at most one addition per launch and the colours run in a fixed order: the addition is
   // a float64 atomic add performed in L2 (no wait for the old value), and the result is the
-  // same as a read-modify-write, bit for bit, whatever the timing.
+  // same as a read-modify-write, bit for bit, whatever the timing (a read-modify-write
+  // measured slower: the workgroup then waits for its reads before it ends).
   const double inv = slot[1];
   const bool plain_z = g.code[2] != 2 && z0 - OFF >= 0 && z0 - OFF + LZ <= num2;  // rows neither wrap nor clamp
   using Walk = RowWalk<BLOCK, LZ * C, LX * LY>;
@@ -1749,50 +1750,36 @@ __device__ __forceinline__ void interpolate_marker(const typename NodeSlot<C>::t
   for (int q = 0; q < C; ++q) out[q] = s[q];
 }
 
-struct PosAhead {
-  float x[ITEM], y[ITEM], z[ITEM];
-};
-
-template <bool SMALL>
-__device__ __forceinline__ void load_pos_ahead(const float *__restrict__ px, const float *__restrict__ py,
-                                               const float *__restrict__ pz, const Item &I, PosAhead &a) {
-#pragma unroll
-  for (int r = 0; r < ITEM; ++r) {
-    const int64_t m = I.begin + (r < (int)I.size ? r : 0);
-    a.x[r] = *row_at<SMALL>(px, m);
-    a.y[r] = *row_at<SMALL>(py, m);
-    a.z[r] = *row_at<SMALL>(pz, m);
-  }
-}
-
-// Tiled gather of float32 markers.  The tile's node footprint is staged in LDS in float32
-// (each position holding its node after the periodic wrap or the wall clamp), the weights
-// are float32 from the float64 cell selection, and the sums are float32: the result is
-// float32 either way.  A work-item runs over items (build_items): it reads its cell's S^3
-// node slots from LDS into registers once and interpolates the item's markers, whose
-// positions were loaded while the previous item was computed.  Tiles are numbered so that
-// each XCD (workgroups go round-robin over the eight) takes a contiguous range of them.
+// Tiled gather of float32 markers, one marker per work-item.  The tile's node footprint is
+// staged in LDS in float32 (each position holding its node after the periodic wrap or the
+// wall clamp), the weights are float32 from the float64 cell selection, and the sums are
+// float32: the result is float32 either way.  The tile's markers are one contiguous run of
+// the binned arrays; consecutive work-items take consecutive markers, so position loads and
+// result stores are coalesced, and the lanes of a cell read the same node slots (one LDS
+// access serves them).  Each marker's cell comes from a table of cells by marker, filled
+// from the cell offsets for chunks of CH markers (a tile holds about 4096 at 4 markers per
+// cell).  The next marker's position is loaded while the current one is interpolated.
+// Tiles are numbered so that each XCD (workgroups go round-robin over the eight) takes a
+// contiguous range of them.
 template <int KIND, int P, int C, int BLOCK, bool SMALL, int ZT = 1>
-__global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 3 : 1) gather_items(const float *__restrict__ pos, int64_t n,
+__global__ void __launch_bounds__(BLOCK) gather_lanes(const float *__restrict__ pos, int64_t n,
                                                       const uint32_t *__restrict__ start, Grid g, int ntx, int nty,
                                                       int ntz, const double *__restrict__ field, int fc,
-                                                      float *__restrict__ res, const uint16_t *__restrict__ tables,
-                                                      const uint32_t *__restrict__ table_counts) {
+                                                      float *__restrict__ res) {
   using Sh = StencilShape<KIND, P>;
   using Slot = typename NodeSlot<C>::type;
   // a workgroup tile of ZT bin tiles along z (consecutive in the bins' key order)
   constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z * ZT, TC = TILE_CELLS * ZT;
   constexpr int S = Sh::S, OFF = Sh::OFF;
   constexpr int LX = TX + S - 1, LY = TY + S - 1, LZ = TZ + S - 1, L = LX * LY * LZ;
-  constexpr int PARTS = TC / BLOCK;
+  constexpr int CH = 4096;
   static_assert(C == 1 || C == 3, "float32 gather tiles hold 1 or 3 components");
-  static_assert(TC <= 4096 && TC % BLOCK == 0, "an item holds its cell in 12 bits");
+  static_assert(TC <= 65536, "a cell index fits 16 bits");
   // the footprint: component 0 (C = 1), or the (x, y) pairs and the z values (C = 3)
   __shared__ float2 tile_xy[C == 3 ? L : 1];
   __shared__ float tile_z[L];
   __shared__ uint32_t cfirst[TC + 1];
-  __shared__ __align__(16) uint16_t items[SWEEPS * TC];
-  __shared__ unsigned long long wave_sums[(BLOCK / 64) * PARTS];
+  __shared__ uint16_t mcell[CH];
   __shared__ long long rows[LX * LY];
 
   const int nb = gridDim.x, b = blockIdx.x;
@@ -1814,7 +1801,6 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 3 : 1) gather_items(cons
     // before the first one is used: one memory latency for both
     constexpr int EO = (TC + 1 + BLOCK - 1) / BLOCK;
     const bool plain_z = g.code[2] != 2 && z0 - OFF >= 0 && z0 - OFF + LZ <= num2;
-    constexpr int Q16 = SWEEPS * TC * 2 / 16, Q = (Q16 + BLOCK - 1) / BLOCK;
     uint32_t o[EO];
 #pragma unroll
     for (int u = 0; u < EO; ++u) {
@@ -1828,14 +1814,6 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 3 : 1) gather_items(cons
     double v[Walk::STEPS];
 #pragma unroll
     for (int u = 0; u < Walk::STEPS; ++u) v[u] = field[rows[walk.row_in(u)] + zq];  // unconditional loads
-    // the item table goes straight into LDS: its loads, issued after the footprint's, return
-    // after them, so waiting for them costs no second latency
-    if (ZT == 1 && tables) {
-      const uint4 *tq = reinterpret_cast<const uint4 *>(tables + (int64_t)id * SWEEPS * TC);
-#pragma unroll
-      for (int u = 0; u < Q; ++u)
-        if (t + u * BLOCK < Q16) reinterpret_cast<uint4 *>(items)[t + u * BLOCK] = tq[t + u * BLOCK];
-    }
 #pragma unroll
     for (int u = 0; u < EO; ++u) {
       const int i = t + u * BLOCK;
@@ -1852,68 +1830,68 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 3 : 1) gather_items(cons
     }
   }
   __syncthreads();
-  const uint32_t total =
-      (ZT == 1 && tables) ? table_counts[id] : build_items<BLOCK, TC>(cfirst, items, wave_sums);
 
-  // lookahead as in scatter_tiled: unconditional loads, the first ones drained before the loop
   const float *px = pos, *py = pos + n, *pz = pos + 2 * n;
   float *rq[C];
 #pragma unroll
   for (int q = 0; q < C; ++q) rq[q] = res + q * n;
   const AxisRange ar = axis_ranges(g);
-  uint32_t it = t;
-  Item cur{0, 0, 0};
-  PosAhead now;
-  if (total > 0) {
-    cur = item_at(items, cfirst, it < total ? it : total - 1);
-    load_pos_ahead<SMALL>(px, py, pz, cur, now);
-    wait_loads();
-  }
-  while (it < total) {
-    const uint32_t nit = it + BLOCK;
-    const Item nxt = item_at(items, cfirst, nit < total ? nit : total - 1);
-    PosAhead next;
-    load_pos_ahead<SMALL>(px, py, pz, nxt, next);
-    const int c = cur.cell, sub = c / TILE_CELLS, loc = c % TILE_CELLS;
-    const int cx = loc / (TY * TILE_Z), cy = (loc / TILE_Z) % TY, cz = sub * TILE_Z + loc % TILE_Z;
-    const int base = (cx * LY + cy) * LZ + cz;
-    const double cell[3] = {(double)(x0 + cx), (double)(y0 + cy), (double)(z0 + cz)};
-    Slot nd[S][S][S];
-#pragma unroll
-    for (int a = 0; a < S; ++a)
-#pragma unroll
-      for (int b2 = 0; b2 < S; ++b2)
-#pragma unroll
-        for (int e = 0; e < S; ++e) {
-          const int k = base + (a * LY + b2) * LZ + e;
-          if constexpr (C == 1) {
-            nd[a][b2][e] = tile_z[k];
-          } else {
-            const float2 xy = tile_xy[k];
-            nd[a][b2][e] = Node3{xy.x, xy.y, tile_z[k]};
-          }
-        }
-#pragma unroll
-    for (int r = 0; r < ITEM; ++r) {
-      if ((uint32_t)r < cur.size) {
-        float out[C];
-        interpolate_marker<KIND, P, C, S>(nd, now.x[r], now.y[r], now.z[r], cell, ar, g, num0, num1, num2, out);
-        const int64_t m = cur.begin + r;
-#pragma unroll
-        for (int q = 0; q < C; ++q) *row_at<SMALL>(rq[q], m) = out[q];
+  const uint32_t first = cfirst[0], count = cfirst[TC] - first;
+  for (uint32_t lo = 0; lo < count; lo += CH) {
+    const uint32_t hi = count - lo < (uint32_t)CH ? count : lo + CH;
+    // the cell of every marker of the chunk [lo, hi)
+    for (int c = t; c < TC; c += BLOCK) {
+      uint32_t f = cfirst[c] - first, e = cfirst[c + 1] - first;
+      f = f > lo ? f : lo;
+      e = e < hi ? e : hi;
+      for (uint32_t m = f; m < e; ++m) mcell[m - lo] = (uint16_t)c;
+    }
+    __syncthreads();
+    // one marker per work-item; the loads past the chunk repeat its last marker
+    const uint32_t mlast = hi - 1;
+    uint32_t m = lo + t;
+    float ax, ay, az;
+    {
+      const int64_t gm = (int64_t)first + (m < hi ? m : mlast);
+      ax = *row_at<SMALL>(px, gm);
+      ay = *row_at<SMALL>(py, gm);
+      az = *row_at<SMALL>(pz, gm);
+    }
+    for (; m < hi; m += BLOCK) {
+      const float x = ax, y = ay, z = az;
+      {
+        const uint32_t mn = m + BLOCK;
+        const int64_t gm = (int64_t)first + (mn < hi ? mn : mlast);
+        ax = *row_at<SMALL>(px, gm);
+        ay = *row_at<SMALL>(py, gm);
+        az = *row_at<SMALL>(pz, gm);
       }
-    }
-    for (uint32_t r = ITEM; r < cur.size; ++r) {
-      const int64_t m = cur.begin + r;
-      float out[C];
-      interpolate_marker<KIND, P, C, S>(nd, *row_at<SMALL>(px, m), *row_at<SMALL>(py, m), *row_at<SMALL>(pz, m), cell,
-                                        ar, g, num0, num1, num2, out);
+      const int c = mcell[m - lo], sub = c / TILE_CELLS, loc = c % TILE_CELLS;
+      const int cx = loc / (TY * TILE_Z), cy = (loc / TILE_Z) % TY, cz = sub * TILE_Z + loc % TILE_Z;
+      const int base = (cx * LY + cy) * LZ + cz;
+      const double cell[3] = {(double)(x0 + cx), (double)(y0 + cy), (double)(z0 + cz)};
+      Slot nd[S][S][S];
 #pragma unroll
-      for (int q = 0; q < C; ++q) *row_at<SMALL>(rq[q], m) = out[q];
+      for (int a = 0; a < S; ++a)
+#pragma unroll
+        for (int b2 = 0; b2 < S; ++b2)
+#pragma unroll
+          for (int e = 0; e < S; ++e) {
+            const int k = base + (a * LY + b2) * LZ + e;
+            if constexpr (C == 1) {
+              nd[a][b2][e] = tile_z[k];
+            } else {
+              const float2 xy = tile_xy[k];
+              nd[a][b2][e] = Node3{xy.x, xy.y, tile_z[k]};
+            }
+          }
+      float out[C];
+      interpolate_marker<KIND, P, C, S>(nd, x, y, z, cell, ar, g, num0, num1, num2, out);
+      const int64_t gm = (int64_t)first + m;
+#pragma unroll
+      for (int q = 0; q < C; ++q) *row_at<SMALL>(rq[q], gm) = out[q];
     }
-    cur = nxt;
-    now = next;
-    it = nit;
+    __syncthreads();  // the next chunk rewrites the cell table
   }
 }
 
@@ -2907,19 +2885,17 @@ static int run_scatter_tiled(const void *pos, const void *val, int64_t n, const 
 }
 
 template <int KIND, int P, int C, int BLOCK, bool SMALL, int ZT>
-static void launch_gather_items(const void *pos, int64_t n, const uint32_t *start, const Grid &g, const double *field,
-                                int fc, void *res, const uint16_t *tables, const uint32_t *table_counts,
-                                hipStream_t stream) {
+static void launch_gather_lanes(const void *pos, int64_t n, const uint32_t *start, const Grid &g, const double *field,
+                                int fc, void *res, hipStream_t stream) {
   const int ntx = (int)((g.num[0] + TILE_X - 1) / TILE_X), nty = (int)((g.num[1] + TILE_Y - 1) / TILE_Y),
             ntz = (int)((g.num[2] + TILE_Z * ZT - 1) / (TILE_Z * ZT));
-  hipLaunchKernelGGL((gather_items<KIND, P, C, BLOCK, SMALL, ZT>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
-                     (const float *)pos, n, start, g, ntx, nty, ntz, field, fc, (float *)res, tables, table_counts);
+  hipLaunchKernelGGL((gather_lanes<KIND, P, C, BLOCK, SMALL, ZT>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
+                     (const float *)pos, n, start, g, ntx, nty, ntz, field, fc, (float *)res);
 }
 
 template <class T, int KIND, int P, int C>
 static int run_gather_tiled(const void *pos, int64_t n, const uint32_t *start, const Grid &g, const double *field,
-                            int fc, void *res, const uint16_t *tables, const uint32_t *table_counts,
-                            hipStream_t stream) {
+                            int fc, void *res, hipStream_t stream) {
   constexpr int TX = TILE_X, TY = TILE_Y, TZ = TILE_Z, BLOCK = 256;
   constexpr int SPAN = KIND == LINEAR ? 2 : P + 1;
   const int ntx = (int)((g.num[0] + TX - 1) / TX), nty = (int)((g.num[1] + TY - 1) / TY),
@@ -2932,15 +2908,14 @@ static int run_gather_tiled(const void *pos, int64_t n, const uint32_t *start, c
       if (small) {
         int v = variant_for(1, KIND, P, C);
         if (!variant_fits(VARIANTS[v].zt, g, SPAN)) v = 0;
-#define GV(B, Z) launch_gather_items<KIND, P, C, B, true, Z>(pos, n, start, g, field, fc, res, tables, table_counts, stream)
+#define GV(B, Z) launch_gather_lanes<KIND, P, C, B, true, Z>(pos, n, start, g, field, fc, res, stream)
         if (v == 1) GV(256, 2);
         else if (v == 2) GV(512, 1);
         else if (v == 3) GV(512, 2);
         else GV(256, 1);
 #undef GV
       } else {
-        launch_gather_items<KIND, P, C, BLOCK, false, 1>(pos, n, start, g, field, fc, res, tables, table_counts,
-                                                         stream);
+        launch_gather_lanes<KIND, P, C, BLOCK, false, 1>(pos, n, start, g, field, fc, res, stream);
       }
     } else {
       hipLaunchKernelGGL((gather_tiled<T, KIND, P, C, BLOCK>), dim3(ntx * nty * ntz), dim3(BLOCK), 0, stream,
@@ -2976,9 +2951,8 @@ static int scatter_tiled_group(int kind, int degree, const void *pos, const void
 
 template <class T, int C>
 static int gather_tiled_group(int kind, int degree, const void *pos, int64_t n, const uint32_t *start, const Grid &g,
-                              const double *field, int fc, void *res, const uint16_t *tables, const uint32_t *counts,
-                              hipStream_t stream) {
-#define GA(K, P) run_gather_tiled<T, K, P, C>(pos, n, start, g, field, fc, res, tables, counts, stream)
+                              const double *field, int fc, void *res, hipStream_t stream) {
+#define GA(K, P) run_gather_tiled<T, K, P, C>(pos, n, start, g, field, fc, res, stream)
   if (kind == LINEAR) return GA(LINEAR, 1);
   if (kind == QUADRATIC) return GA(QUADRATIC, 2);
   if (degree == 0) return GA(SPLINE, 0);
@@ -3022,15 +2996,12 @@ static int scatter_tiled_3d(int kind, int degree, const void *pos, const void *v
 
 template <class T>
 static int gather_tiled_3d(int kind, int degree, const void *pos, int64_t n, int64_t ncomp, const uint32_t *start,
-                           const Grid &g, const double *field, void *res, const uint16_t *tables,
-                           const uint32_t *counts, hipStream_t stream) {
+                           const Grid &g, const double *field, void *res, hipStream_t stream) {
   return component_groups(kind, degree, ncomp, [&](int64_t c0, int size) {
     T *r = (T *)res + c0 * n;
     if (size == 3)
-      return gather_tiled_group<T, 3>(kind, degree, pos, n, start, g, field + c0, (int)ncomp, r, tables, counts,
-                                      stream);
-    return gather_tiled_group<T, 1>(kind, degree, pos, n, start, g, field + c0, (int)ncomp, r, tables, counts,
-                                    stream);
+      return gather_tiled_group<T, 3>(kind, degree, pos, n, start, g, field + c0, (int)ncomp, r, stream);
+    return gather_tiled_group<T, 1>(kind, degree, pos, n, start, g, field + c0, (int)ncomp, r, stream);
   });
 }
 
@@ -3082,6 +3053,8 @@ extern "C" int mpf_gather_binned(int kind, int degree, int ndim, const void *sor
                                  const double *lower, const double *cell_size, const int64_t *num_nodes,
                                  const int64_t *code, mpf_context *ctx, mpf_stream stream_) {
   (void)ctx;
+  (void)item_tables;  // the gather takes one marker per work-item: no work lists
+  (void)item_counts;
   Grid g;
   int rc = make_grid(g, ndim, kind, kind == LINEAR ? 1 : (kind == QUADRATIC ? 2 : degree), lower, cell_size,
                      num_nodes, code);
@@ -3104,10 +3077,8 @@ extern "C" int mpf_gather_binned(int kind, int degree, int ndim, const void *sor
 #undef GA2
   }
   if (dtype == F32)
-    return gather_tiled_3d<float>(kind, degree, sorted_positions, n, ncomp, start, g, field, result, item_tables,
-                                  item_counts, stream);
-  return gather_tiled_3d<double>(kind, degree, sorted_positions, n, ncomp, start, g, field, result, item_tables,
-                                 item_counts, stream);
+    return gather_tiled_3d<float>(kind, degree, sorted_positions, n, ncomp, start, g, field, result, stream);
+  return gather_tiled_3d<double>(kind, degree, sorted_positions, n, ncomp, start, g, field, result, stream);
 }
 
 template <class T, int KIND, int P, int C>
