@@ -1464,6 +1464,35 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 2 : 1) scatter_tiled(con
     now = next;
     it += BLOCK;
   }
+
+  // The tile into the field, one (node, component) per work-item so that neighbouring lanes
+  // touch neighbouring words (a footprint row is LZ * C consecutive words of the field, its
+  // start in the row table).  Same-colour footprints never overlap, so every node receives
+  // at most one addition per launch and the colours run in a fixed order.  Where the
+  // footprint's positions are distinct nodes (no periodic axis shorter than the footprint)
+  // the addition is a plain read-modify-write whose reads are issued before the barrier
+  // that ends the deposit, so they overlap the slowest wavefront's last items; otherwise it
+  // is a float64 atomic add performed in L2.  Either way each word becomes old + v * inv
+  // rounded once: the same bits whatever the timing.
+  const double inv = slot[1];
+  const bool plain_z = g.code[2] != 2 && z0 - OFF >= 0 && z0 - OFF + LZ <= num2;  // rows neither wrap nor clamp
+  using Walk = RowWalk<BLOCK, LZ * C, LX * LY>;
+  const Walk walk(t);
+  const int iz = walk.k / C, q = walk.k - iz * C;
+  const int gz = plain_z ? z0 - OFF + iz : footprint_z(g, z0 - OFF + iz, false);
+  const long long zq = (long long)gz * fc + q;
+  const bool distinct = (g.code[0] != 1 || num0 >= LX) && (g.code[1] != 1 || num1 >= LY) &&
+                        (g.code[2] != 1 || num2 >= LZ);
+  double old[Walk::STEPS];
+  long long at[Walk::STEPS];
+  if (distinct) {
+#pragma unroll
+    for (int u = 0; u < Walk::STEPS; ++u) {
+      const long long rb = rows[walk.row_in(u)];
+      at[u] = (walk.has(u) && rb >= 0 && gz >= 0) ? rb + zq : -1ll;
+      old[u] = field[at[u] >= 0 ? at[u] : 0];
+    }
+  }
   __syncthreads();
 
   // bounded axes: fold the footprint beyond a wall onto the wall node, axis by axis, so
@@ -1488,20 +1517,14 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 2 : 1) scatter_tiled(con
     __syncthreads();
   }
 
-  // the tile into the field, one (node, component) per work-item so that neighbouring lanes
-  // touch neighbouring words (a footprint row is LZ * C consecutive words of the field, its
-  // start in the row table).  Same-colour footprints never overlap, so every node receives
-  // at most one addition per launch and the colours run in a fixed order: the addition is
-  // a float64 atomic add performed in L2 (no wait for the old value), and the result is the
-  // same as a read-modify-write, bit for bit, whatever the timing (a read-modify-write
-  // measured slower: the workgroup then waits for its reads before it ends).
-  const double inv = slot[1];
-  const bool plain_z = g.code[2] != 2 && z0 - OFF >= 0 && z0 - OFF + LZ <= num2;  // rows neither wrap nor clamp
-  using Walk = RowWalk<BLOCK, LZ * C, LX * LY>;
-  const Walk walk(t);
-  const int iz = walk.k / C, q = walk.k - iz * C;
-  const int gz = plain_z ? z0 - OFF + iz : footprint_z(g, z0 - OFF + iz, false);
-  const long long zq = (long long)gz * fc + q;
+  if (distinct) {
+#pragma unroll
+    for (int u = 0; u < Walk::STEPS; ++u) {
+      const long long v = (long long)acc[q * L + walk.row_in(u) * LZ + iz];
+      if (at[u] >= 0 && v != 0) field[at[u]] = old[u] + (double)v * inv;
+    }
+    return;
+  }
   for (int u = 0; u < Walk::STEPS && walk.has(u); ++u) {
     const int r = walk.row(u);
     const long long rb = rows[r];
