@@ -248,6 +248,20 @@ int mpf_deposit_paths(int kind, int degree, int ndim, const void *start, int64_t
                       const double *lower, const double *cell_size, const int64_t *num_nodes, const int64_t *code,
                       int mode, int scale_exp, mpf_context *ctx, mpf_stream stream);
 
+/* The path deposit of paths the caller keeps binned by their start (2D/3D, bounded or
+ * periodic axes): sorted_start (ndim rows of n) and start (the bin offsets) from
+ * mpf_bin_markers of the start points -- kind 0 (Whitney) reads bins of the linear cell
+ * (kind 0, degree 1), kind 1 bins of the spline nodes of its degree (kind 2) -- with end
+ * (ndim rows, row stride end_stride) and charges in the same order.  The fixed-point
+ * deposit of mpf_deposit_paths with the automatic scale, its tiled kernel without the sort
+ * (the same edges bit for bit).  No counterpart in the reference, whose deposit_paths_block
+ * (edges.py:187-215) takes paths in any order: for callers that keep their markers sorted
+ * by cell between steps, as the banded CPU buffers assume (scatter_linear_3d_pyccel.py:11-16). */
+int mpf_deposit_paths_binned(int kind, int degree, int ndim, const void *sorted_start, const void *end,
+                             int64_t end_stride, const void *charges, int dtype, int64_t n, const uint32_t *start,
+                             double *edges, const double *lower, const double *cell_size, const int64_t *num_nodes,
+                             const int64_t *code, mpf_context *ctx, mpf_stream stream);
+
 /* ---- forms of the spline sequence (splines.py:180-201 faces, cells) ---------------- */
 
 /* kernels/scatter_forms_spline_{1,2,3}d and gather_forms_spline_{1,2,3}d

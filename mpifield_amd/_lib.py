@@ -58,6 +58,7 @@ SIGNATURES: dict[str, list] = {
     "mpf_gather_derivative": [_INT, _INT, _INT, _P, _I64, _INT, _I64, _P, _I64, _P, _I64, _P, _P, _P, _P, _P, _P],
     "mpf_deposit_paths": [_INT, _INT, _INT, _P, _I64, _P, _I64, _P, _INT, _I64, _P, _P, _P, _P, _P, _INT, _INT, _P,
                           _P],
+    "mpf_deposit_paths_binned": [_INT, _INT, _INT, _P, _P, _I64, _P, _INT, _I64, _P, _P, _P, _P, _P, _P, _P, _P],
     "mpf_halo_fill": [_INT, _P, _I64, _I64, _P, _P, _P],
     "mpf_halo_fold": [_INT, _P, _I64, _I64, _P, _P, _P],
     "mpf_divergence": [_INT, _P, _P, _P, _P, _P],

@@ -15,6 +15,13 @@ The forms deposit (faces, cells) reads bins of the spline nodes of its degree::
     scatter_forms_binned(bins, faces, form=2)
     gather_forms_binned(bins, faces, result, form=2)   # result (ncomp, N) in bins order
 
+Paths whose starts the caller keeps binned (the markers' positions at the start of a step)
+deposit their current without a sort: Whitney paths read bins of the linear cell, the spline
+edges of degree p bins of the spline nodes of p::
+
+    bins = bin_markers(start, None, (KIND_SPLINE, 2), grid.geometry)
+    deposit_paths_binned(bins, end_in_bins_order, charges_in_bins_order, edges, kind=1, degree=2)
+
 ``bins.order[i]`` is the input index of binned marker ``i``. `Weighting.scatter_block` /
 `gather_block` take this path by themselves for large batches (binning inside the call).
 """
@@ -310,3 +317,32 @@ def gather_forms_binned(bins: MarkerBins, field: torch.Tensor, result: torch.Ten
                   K._dtype_code(bins.positions), bins.n, bins.start.data_ptr(), field.data_ptr(), ncomp,
                   result.data_ptr(), lower.ctypes.data, cell.ctypes.data, num.ctypes.data, code.ctypes.data, None,
                   K.stream_handle(field.device))
+
+
+def deposit_paths_binned(bins: MarkerBins, end, charges, edges: torch.Tensor, kind: int, degree: int) -> None:
+    """Add the current of paths from the binned start points ``bins.positions`` to ``end``
+    (ndim, N) with ``charges`` (N,), both in the bins' order, into ``edges`` (mpf_deposit_paths_binned:
+    mpf_deposit_paths's fixed-point deposit, bit for bit, without its sort).  ``kind`` 0 is
+    WHITNEY (bins of the linear cell), 1 the spline edges of ``degree`` (bins of the spline
+    nodes of that degree)."""
+    want = (_lib.KIND_LINEAR, 1) if kind == 0 else (_lib.KIND_SPLINE, int(degree))
+    if kind not in (0, 1):
+        raise ValueError(f"path kind must be 0 (WHITNEY) or 1 (spline edges), got {kind}")
+    if (bins.kind, bins.degree) != want:
+        raise ValueError(f"these paths read bins of (kind, degree) {want}, got {(bins.kind, bins.degree)}")
+    ndim, lower, cell, num, code = _geometry_arrays(bins.geometry)
+    edges = K._field(edges, "edges")
+    e, es = K.marker_rows(end, ndim, bins.positions.dtype)
+    q = K.as_device(charges, bins.positions.dtype).reshape(-1).contiguous()
+    if e.shape[1] != bins.n or q.numel() != bins.n:
+        raise ValueError(f"{bins.n} binned starts for ends {tuple(e.shape)} and {q.numel()} charges")
+    extent = K._expected_extent(num, code, want[1], _lib.KIND_LINEAR if kind == 0 else _lib.KIND_SPLINE)
+    if tuple(edges.shape) != (*extent, ndim):
+        raise ValueError(f"an edge field on this grid is {(*extent, ndim)}, got {tuple(edges.shape)}")
+    if edges.device != bins.positions.device or e.device != edges.device or q.device != edges.device:
+        raise ValueError("bins, ends, charges and edges must be on one device")
+    with torch.cuda.device(edges.device):
+        _lib.call("mpf_deposit_paths_binned", kind, want[1] if kind == 1 else 1, ndim, bins.positions.data_ptr(),
+                  e.data_ptr(), es, q.data_ptr(), K._dtype_code(bins.positions), bins.n, bins.start.data_ptr(),
+                  edges.data_ptr(), lower.ctypes.data, cell.ctypes.data, num.ctypes.data, code.ctypes.data, None,
+                  K.stream_handle(edges.device))

@@ -527,135 +527,244 @@ __device__ __forceinline__ void edge_axis(double rel, const AxisTile &ax, int *s
   }
 }
 
-// One path's spline-edge contributions into the tile (rounded with to_fixed_small, as the
-// Whitney sink's: the automatic scale keeps them below 2^49 quanta).
+// ---- spline-edge paths into a tile, piece by piece ---------------------------------------
+//
+// The walk of spline_path (stretches of at most four cells, cut at the knot planes and the
+// walls), with the work of one piece at a time: between two consecutive cuts no knot plane
+// is crossed, so every Gauss point of a piece has the same node and edge stencil.  A piece
+// takes its stencil once (the nodes and edges around its midpoint, unwrapped, folded into
+// the footprint once), sums the Gauss points' contributions per edge in float64 registers
+// and adds each sum into the tile as one fixed-point word (to_fixed_small: the automatic
+// scale keeps a path's whole charge times its length below 2^49 quanta).  The offsets of a
+// point from the piece's middle node and edge are r - middle in unwrapped coordinates,
+// equal to the reference's wrapped r - middle.
+
+// The state of one work-item's current path: its stretch, the stretch's cut times and the
+// start of the next piece.
+template <int NDIM>
+struct PathWalk {
+  double r0[NDIM], r1[NDIM], a0[NDIM], a1[NDIM], cut[NDIM * 6];
+  double charge, t0;
+  int st, stretches;
+};
+
+// The cut times of stretch w.st (a slot per candidate plane or wall, divided out only when
+// it exists; 2.0 when absent) and its first piece.
 template <int NDIM, int DEG>
-__device__ __forceinline__ void path_into_tile(const double *r0, const double *r1, double charge, const Grid &g,
-                                               const AxisTile *ax, lds_u64 *lds,
-                                               unsigned long long *__restrict__ accum, double scale) {
-  using PT = PathTile<NDIM>;
+__device__ __forceinline__ void walk_stretch(PathWalk<NDIM> &w, const Grid &g) {
   constexpr double SHIFT = DEG % 2 == 0 ? 0.5 : 0.0;
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    double f0 = (w.r1[d] - w.r0[d]) * (double)w.st, f1 = (w.r1[d] - w.r0[d]) * (double)(w.st + 1);
+    if (w.stretches != 1) {  // a division by 1.0 is exact
+      f0 = f0 / (double)w.stretches;
+      f1 = f1 / (double)w.stretches;
+    }
+    w.a0[d] = w.r0[d] + f0;
+    w.a1[d] = w.r0[d] + f1;
+    const double delta = w.a1[d] - w.a0[d];
+    const double low = delta > 0.0 ? w.a0[d] : w.a1[d];
+    const double high = delta > 0.0 ? w.a1[d] : w.a0[d];
+    const double offset = g.code[d] == 2 ? 0.0 : SHIFT;
+    double plane = (double)floor_int(low - offset) + 1.0 + offset;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double t = 2.0;
+      if (delta != 0.0 && plane < high) {
+        t = (plane - w.a0[d]) / delta;
+        t = t > 0.0 && t < 1.0 ? t : 2.0;
+      }
+      w.cut[d * 6 + k] = t;
+      plane += 1.0;
+    }
+    double tw0 = 2.0, tw1 = 2.0;
+    if (g.code[d] != 1 && delta != 0.0 && (low < 0.0 || high > (double)g.num[d] - 1.0)) {
+      tw0 = (0.0 - w.a0[d]) / delta;
+      tw1 = ((double)g.num[d] - 1.0 - w.a0[d]) / delta;
+      tw0 = tw0 > 0.0 && tw0 < 1.0 ? tw0 : 2.0;
+      tw1 = tw1 > 0.0 && tw1 < 1.0 ? tw1 : 2.0;
+    }
+    w.cut[d * 6 + 4] = tw0;
+    w.cut[d * 6 + 5] = tw1;
+  }
+  w.t0 = 0.0;
+}
+
+// The node and edge stencil of a piece on one axis from its midpoint rm (clamped to the grid
+// on a bounded axis; on a periodic axis taken modulo num, so that the middle node is in
+// [0, num] and one fold reaches every index): the offsets of the midpoint from the middle
+// node and the middle edge (a Gauss point's are these plus its distance from the midpoint),
+// and the footprint slots and indices of the stencil (node_axis, edge_axis).
+template <int DEG>
+struct PieceAxis {
+  double dn, de;
+  int ns[DEG + 1], nn[DEG + 1], es[DEG], en[DEG];
+};
+
+template <int DEG>
+__device__ __forceinline__ void piece_axis(double rm, const AxisTile &ax, PieceAxis<DEG> &pa) {
+  constexpr int LOWER = DEG - 1;
+  const double r = ax.code == 1 ? wrap_rel(rm, ax.num) : rm;
+  const int nmid = (int)((DEG % 2 == 0) ? floor_int(r + 0.5) : floor_int(r));
+  pa.dn = r - (double)nmid;
+#pragma unroll
+  for (int a = 0; a <= DEG; ++a) fold_node(nmid - DEG / 2 + a, ax, pa.ns[a], pa.nn[a]);
+  const double shifted = r - 0.5;
+  int emid;
+  if (LOWER % 2 == 0) {
+    emid = (int)floor_int(shifted + 0.5);
+    if (ax.code == 0 && emid > ax.num - 2) emid = ax.num - 2;
+  } else {
+    emid = (int)floor_int(shifted);
+  }
+  pa.de = shifted - (double)emid;
+#pragma unroll
+  for (int a = 0; a < DEG; ++a) {
+    const int k = emid - LOWER / 2 + a;
+    if (ax.code == 1) {
+      fold_node(k, ax, pa.es[a], pa.en[a]);
+    } else if (k < 0 || k > ax.num - 2) {
+      pa.es[a] = -2;
+      pa.en[a] = 0;
+    } else {
+      pa.en[a] = k;
+      const int l = k - ax.base;
+      pa.es[a] = l >= 0 && l < ax.len ? l : -1;
+    }
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void bspline_weights(double o, double *w) {
+  if (P == 0) {
+    w[0] = 1.0;
+  } else if (P == 1) {
+    w[0] = 1.0 - o;
+    w[1] = o;
+  } else if (P == 2) {
+    w[0] = 0.5 * (0.5 - o) * (0.5 - o);
+    w[1] = 0.75 - o * o;
+    w[2] = 0.5 * (0.5 + o) * (0.5 + o);
+  } else {
+    const double cm = 1.0 - o, sq = o * o, cu = sq * o;
+    w[0] = cm * cm * cm / 6.0;
+    w[1] = (3.0 * cu - 6.0 * sq + 4.0) / 6.0;
+    w[2] = (-3.0 * cu + 3.0 * sq + 3.0 * o + 1.0) / 6.0;
+    w[3] = cu / 6.0;
+  }
+}
+
+// One contribution of a piece into the tile (or the global accumulator outside it).
+template <int NDIM>
+__device__ __forceinline__ void piece_add(double v, int axis, const int (&slot)[3], const int (&idx)[3],
+                                          const Grid &g, lds_u64 *lds, unsigned long long *__restrict__ accum,
+                                          double scale) {
+  using PT = PathTile<NDIM>;
+  if (slot[0] == -2 || slot[1] == -2 || slot[2] == -2) return;  // an edge past a bounded wall
+  const unsigned long long f = to_fixed_small(v * scale);
+  if (f == 0ull) return;
+  if (slot[0] >= 0 && slot[1] >= 0 && slot[2] >= 0) {
+    __hip_atomic_fetch_add(lds + ((slot[0] * PT::LY + slot[1]) * PT::LZ + slot[2]) * NDIM + axis, f,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  } else {
+    const int64_t at = ((int64_t)idx[0] * g.stride[0] + (NDIM > 1 ? (int64_t)idx[1] * g.stride[1] : 0) +
+                        (NDIM > 2 ? (int64_t)idx[2] * g.stride[2] : 0)) * NDIM + axis;
+    atomicAdd(accum + at, f);
+  }
+}
+
+// The piece [w.t0, t1) of the current stretch into the tile.
+template <int NDIM, int DEG>
+__device__ __forceinline__ void piece_into_tile(const PathWalk<NDIM> &w, double t1, const Grid &g,
+                                                const AxisTile *ax, lds_u64 *lds,
+                                                unsigned long long *__restrict__ accum, double scale) {
   constexpr int POINTS = (DEG - 1 + (NDIM - 1) * DEG) / 2 + 1;
   constexpr int S = DEG + 1;
   double point[5], gw[5];
   gauss_rule(POINTS, point, gw);
-  double extent = 0.0;
+  double p0[NDIM], stp[NDIM];
+  PieceAxis<DEG> pa[NDIM];
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
-    const double span = r1[d] > r0[d] ? r1[d] - r0[d] : r0[d] - r1[d];
-    if (span > extent) extent = span;
+    double begin = w.a0[d] + w.t0 * (w.a1[d] - w.a0[d]);
+    double finish = w.a0[d] + t1 * (w.a1[d] - w.a0[d]);
+    if (g.code[d] != 1) {
+      const double top = (double)(g.num[d] - 1);
+      if (begin < 0.0) begin = 0.0;
+      if (begin > top) begin = (double)g.num[d] - 1.0;
+      if (finish < 0.0) finish = 0.0;
+      if (finish > top) finish = (double)g.num[d] - 1.0;
+    }
+    p0[d] = begin;
+    stp[d] = finish - begin;
+    piece_axis<DEG>(begin + 0.5 * stp[d], ax[d], pa[d]);
   }
-  const int64_t stretches = (int64_t)(extent / 4.0) + 1;
-  for (int64_t st = 0; st < stretches; ++st) {
-    double a0[NDIM], a1[NDIM];
+  // per axis the sums over the Gauss points of charge * gw * step * (edge weight along the
+  // axis) * (node weights across), indexed [a][b][e] as the stencils
+  constexpr int NB = NDIM > 1 ? S : 1, NE = NDIM > 2 ? S : 1;
+  double sum[NDIM][S][NB][NE];
+#pragma unroll
+  for (int axis = 0; axis < NDIM; ++axis)
+#pragma unroll
+    for (int a = 0; a < S; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int e = 0; e < NE; ++e) sum[axis][a][b][e] = 0.0;
+#pragma unroll
+  for (int gp = 0; gp < POINTS; ++gp) {
+    double wn[NDIM][S], we[NDIM][DEG];
 #pragma unroll
     for (int d = 0; d < NDIM; ++d) {
-      double f0 = (r1[d] - r0[d]) * (double)st, f1 = (r1[d] - r0[d]) * (double)(st + 1);
-      if (stretches != 1) {  // a division by 1.0 is exact
-        f0 = f0 / (double)stretches;
-        f1 = f1 / (double)stretches;
-      }
-      a0[d] = r0[d] + f0;
-      a1[d] = r0[d] + f1;
+      const double off = (point[gp] - 0.5) * stp[d];  // from the midpoint
+      bspline_weights<DEG>(pa[d].dn + off, wn[d]);
+      bspline_weights<DEG - 1>(pa[d].de + off, we[d]);
     }
-    // the cut times: a slot per candidate plane or wall, divided out only when it exists
-    double cut[NDIM * 6];
 #pragma unroll
-    for (int d = 0; d < NDIM; ++d) {
-      const double delta = a1[d] - a0[d];
-      const double low = delta > 0.0 ? a0[d] : a1[d];
-      const double high = delta > 0.0 ? a1[d] : a0[d];
-      const double offset = g.code[d] == 2 ? 0.0 : SHIFT;
-      double plane = (double)floor_int(low - offset) + 1.0 + offset;
+    for (int axis = 0; axis < NDIM; ++axis) {
+      const double amount = w.charge * gw[gp] * stp[axis];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        double t = 2.0;
-        if (delta != 0.0 && plane < high) {
-          t = (plane - a0[d]) / delta;
-          t = t > 0.0 && t < 1.0 ? t : 2.0;
-        }
-        cut[d * 6 + k] = t;
-        plane += 1.0;
-      }
-      double tw0 = 2.0, tw1 = 2.0;
-      if (g.code[d] != 1 && delta != 0.0 && (low < 0.0 || high > (double)g.num[d] - 1.0)) {
-        tw0 = (0.0 - a0[d]) / delta;
-        tw1 = ((double)g.num[d] - 1.0 - a0[d]) / delta;
-        tw0 = tw0 > 0.0 && tw0 < 1.0 ? tw0 : 2.0;
-        tw1 = tw1 > 0.0 && tw1 < 1.0 ? tw1 : 2.0;
-      }
-      cut[d * 6 + 4] = tw0;
-      cut[d * 6 + 5] = tw1;
-    }
-    double t0 = 0.0;
-    while (t0 < 1.0) {
-      double t1 = 1.0;
+      for (int a = 0; a < (axis == 0 ? DEG : S); ++a) {
+        const double va = amount * (axis == 0 ? we[0][a] : wn[0][a]);
 #pragma unroll
-      for (int i = 0; i < NDIM * 6; ++i) t1 = (cut[i] > t0 && cut[i] < t1) ? cut[i] : t1;
-      double p0[NDIM], stp[NDIM];
+        for (int b = 0; b < (NDIM < 2 ? 1 : (axis == 1 ? DEG : S)); ++b) {
+          const double vb = NDIM < 2 ? va : va * (axis == 1 ? we[NDIM > 1 ? 1 : 0][b] : wn[NDIM > 1 ? 1 : 0][b]);
 #pragma unroll
-      for (int d = 0; d < NDIM; ++d) {
-        double begin = a0[d] + t0 * (a1[d] - a0[d]);
-        double finish = a0[d] + t1 * (a1[d] - a0[d]);
-        if (g.code[d] != 1) {
-          const double top = (double)(g.num[d] - 1);
-          if (begin < 0.0) begin = 0.0;
-          if (begin > top) begin = (double)g.num[d] - 1.0;
-          if (finish < 0.0) finish = 0.0;
-          if (finish > top) finish = (double)g.num[d] - 1.0;
-        }
-        p0[d] = begin;
-        stp[d] = finish - begin;
-      }
-#pragma unroll
-      for (int gp = 0; gp < POINTS; ++gp) {
-        int ns[NDIM][S], nn[NDIM][S], es[NDIM][DEG], en[NDIM][DEG];
-        double wn[NDIM][S], we[NDIM][DEG];
-#pragma unroll
-        for (int d = 0; d < NDIM; ++d) {
-          const double rel = p0[d] + point[gp] * stp[d];
-          node_axis<DEG>(rel, ax[d], ns[d], nn[d], wn[d]);
-          edge_axis<DEG>(rel, ax[d], es[d], en[d], we[d]);
-        }
-#pragma unroll
-        for (int axis = 0; axis < NDIM; ++axis) {
-          const double amount = charge * gw[gp] * stp[axis];
-          if (amount == 0.0) continue;
-#pragma unroll
-          for (int a = 0; a < (axis == 0 ? DEG : S); ++a) {
-            const int sa = axis == 0 ? es[0][a] : ns[0][a];
-            if (sa == -2) continue;
-            const double va = amount * (axis == 0 ? we[0][a] : wn[0][a]);
-            const int ia = axis == 0 ? en[0][a] : nn[0][a];
-#pragma unroll
-            for (int b = 0; b < (NDIM < 2 ? 1 : (axis == 1 ? DEG : S)); ++b) {
-              const int sb = NDIM < 2 ? 0 : (axis == 1 ? es[NDIM - 1 > 0 ? 1 : 0][b] : ns[NDIM - 1 > 0 ? 1 : 0][b]);
-              if (sb == -2) continue;
-              const double vb = NDIM < 2 ? va : va * (axis == 1 ? we[NDIM - 1 > 0 ? 1 : 0][b] : wn[NDIM - 1 > 0 ? 1 : 0][b]);
-              const int ib = NDIM < 2 ? 0 : (axis == 1 ? en[NDIM - 1 > 0 ? 1 : 0][b] : nn[NDIM - 1 > 0 ? 1 : 0][b]);
-#pragma unroll
-              for (int e = 0; e < (NDIM < 3 ? 1 : (axis == 2 ? DEG : S)); ++e) {
-                const int se = NDIM < 3 ? 0 : (axis == 2 ? es[NDIM - 1][e] : ns[NDIM - 1][e]);
-                if (se == -2) continue;
-                const double v = NDIM < 3 ? vb : vb * (axis == 2 ? we[NDIM - 1][e] : wn[NDIM - 1][e]);
-                const int ie = NDIM < 3 ? 0 : (axis == 2 ? en[NDIM - 1][e] : nn[NDIM - 1][e]);
-                const unsigned long long f = to_fixed_small(v * scale);
-                if (sa >= 0 && sb >= 0 && se >= 0) {
-                  __hip_atomic_fetch_add(lds + ((sa * PT::LY + sb) * PT::LZ + se) * NDIM + axis, f,
-                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                } else {
-                  const int64_t at = ((int64_t)ia * g.stride[0] + (NDIM > 1 ? (int64_t)ib * g.stride[1] : 0) +
-                                      (NDIM > 2 ? (int64_t)ie * g.stride[2] : 0)) * NDIM + axis;
-                  atomicAdd(accum + at, f);
-                }
-              }
-            }
+          for (int e = 0; e < (NDIM < 3 ? 1 : (axis == 2 ? DEG : S)); ++e) {
+            const double v = NDIM < 3 ? vb : vb * (axis == 2 ? we[NDIM - 1][e] : wn[NDIM - 1][e]);
+            sum[axis][a][b][e] += v;
           }
         }
       }
-      t0 = t1;
     }
   }
+#pragma unroll
+  for (int axis = 0; axis < NDIM; ++axis) {
+    if (stp[axis] == 0.0) continue;  // no motion along this axis in this piece
+#pragma unroll
+    for (int a = 0; a < (axis == 0 ? DEG : S); ++a)
+#pragma unroll
+      for (int b = 0; b < (NDIM < 2 ? 1 : (axis == 1 ? DEG : S)); ++b)
+#pragma unroll
+        for (int e = 0; e < (NDIM < 3 ? 1 : (axis == 2 ? DEG : S)); ++e) {
+          const int slot[3] = {axis == 0 ? pa[0].es[a] : pa[0].ns[a],
+                               NDIM < 2 ? 0 : (axis == 1 ? pa[NDIM > 1 ? 1 : 0].es[b] : pa[NDIM > 1 ? 1 : 0].ns[b]),
+                               NDIM < 3 ? 0 : (axis == 2 ? pa[NDIM - 1].es[e] : pa[NDIM - 1].ns[e])};
+          const int idx[3] = {axis == 0 ? pa[0].en[a] : pa[0].nn[a],
+                              NDIM < 2 ? 0 : (axis == 1 ? pa[NDIM > 1 ? 1 : 0].en[b] : pa[NDIM > 1 ? 1 : 0].nn[b]),
+                              NDIM < 3 ? 0 : (axis == 2 ? pa[NDIM - 1].en[e] : pa[NDIM - 1].nn[e])};
+          piece_add<NDIM>(sum[axis][a][b][e], axis, slot, idx, g, lds, accum, scale);
+        }
+  }
+}
+
+// The next piece's end: the smallest cut strictly after t0, 1.0 when none.
+template <int NDIM>
+__device__ __forceinline__ double piece_end(const PathWalk<NDIM> &w) {
+  double t1 = 1.0;
+#pragma unroll
+  for (int i = 0; i < NDIM * 6; ++i) t1 = (w.cut[i] > w.t0 && w.cut[i] < t1) ? w.cut[i] : t1;
+  return t1;
 }
 
 // Whitney contributions into a tile's LDS accumulator: the wrapped node index of each axis
@@ -693,14 +802,16 @@ struct WhitneyTileSink {
   }
 };
 
-// DEG 1..3: spline edges; DEG 0: Whitney.  At most 128 VGPRs (four waves per SIMD): the
-// work-items walk long chains of dependent float64 arithmetic, and more waves hide more of it.
-// A 3D tile's LDS (103 KB) leaves room for one workgroup per CU, so 3D tiles take 1024
-// lanes (measured for 2^23 paths on 128^3: Whitney 2.7 -> 1.9 ms, degree 2 10.3 -> 5.1,
-// degree 3 33 -> 15 against 256 lanes); 2D tiles four workgroups of 256.
+// DEG 1..3: spline edges (piece by piece, one piece per trip); DEG 0: Whitney.  2D tiles:
+// four workgroups of 256 per CU, at most 128 VGPRs (four waves per SIMD: the work-items walk
+// long chains of dependent float64 arithmetic, and more waves hide more of it).  A 3D
+// tile's LDS (103 KB) leaves room for one workgroup per CU: 1024 work-items for the
+// Whitney walk (2^23 paths on 128^3: 2.7 -> 1.9 ms against 256), 512 for the spline edges,
+// whose per-piece sums hold up to 3 x 18 float64 registers.
 template <class T, int NDIM, int DEG, int BLOCK>
-__global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 4 : 1) spline_paths_tiled(const T *__restrict__ st, const T *__restrict__ rows,
-                                                            int64_t n, const uint32_t *__restrict__ start, Grid g,
+__global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 4 : 1) spline_paths_tiled(const T *__restrict__ st,
+                                                            const T *__restrict__ q, const T *__restrict__ en,
+                                                            int64_t es, int64_t n, const uint32_t *__restrict__ start, Grid g,
                                                             int nty, int ntz, unsigned long long *__restrict__ accum,
                                                             const double *__restrict__ slot) {
   using PT = PathTile<NDIM>;
@@ -722,27 +833,62 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 4 : 1) spline_paths_tile
   // one run made every load touch 64 lines: Whitney 5.5 -> 3.4 ms for 2^25 paths on 1024^2).
   // 3D tiles hold a few thousand for 1024 lanes: an even split, one run per lane.
   const uint32_t count = last - first;
-  const uint32_t run = NDIM == 2 ? 8u : (count + BLOCK - 1) / BLOCK;
-  WhitneyTileSink<NDIM> wsink{(lds_u64 *)lds, accum, scale, &g, ax};
-  for (uint32_t lo = first + t * run; lo < last; lo += BLOCK * run)
-  for (uint32_t p = lo; p < (lo + run < last ? lo + run : last); ++p) {
-    const double charge = (double)rows[p];
-    if (charge == 0.0) continue;
-    double r0[NDIM], r1[NDIM];
-    bool finite = true;
+  const uint32_t run = NDIM == 2 ? 8u : (count > 0 ? (count + BLOCK - 1) / BLOCK : 1u);
+  if constexpr (DEG == 0) {
+    WhitneyTileSink<NDIM> wsink{(lds_u64 *)lds, accum, scale, &g, ax};
+    for (uint32_t lo = first + t * run; lo < last; lo += BLOCK * run)
+      for (uint32_t p = lo; p < (lo + run < last ? lo + run : last); ++p) {
+        const double charge = (double)q[p];
+        if (charge == 0.0) continue;
+        double r0[NDIM], r1[NDIM];
 #pragma unroll
-    for (int d = 0; d < NDIM; ++d) {
-      const double s0 = (double)st[d * n + p], e0 = (double)rows[(1 + d) * n + p];
-      finite = finite && (s0 - s0 == 0.0) && (e0 - e0 == 0.0);
-      r0[d] = (s0 - g.lower[d]) * g.inverse[d];
-      r1[d] = (e0 - g.lower[d]) * g.inverse[d];
-    }
-    if constexpr (DEG == 0) {
-      if (!whitney_finite<NDIM>(r0, r1)) continue;
-      whitney_path<NDIM>(r0, r1, charge, g, wsink);
-    } else {
-      if (!finite) continue;
-      path_into_tile<NDIM, DEG>(r0, r1, charge, g, ax, (lds_u64 *)lds, accum, scale);
+        for (int d = 0; d < NDIM; ++d) {
+          r0[d] = ((double)st[d * n + p] - g.lower[d]) * g.inverse[d];
+          r1[d] = ((double)en[d * es + p] - g.lower[d]) * g.inverse[d];
+        }
+        if (!whitney_finite<NDIM>(r0, r1)) continue;
+        whitney_path<NDIM>(r0, r1, charge, g, wsink);
+      }
+  } else {
+    // One piece per trip: a work-item whose path is done takes its next live path (and its
+    // first stretch) in the same trip, so the lanes of a wavefront stay busy while their
+    // paths have different numbers of pieces.  The k-th path of work-item t is path
+    // first + (k / run) * BLOCK * run + t * run + k % run.
+    uint32_t k = 0;
+    PathWalk<NDIM> w;
+    bool live = false;
+    for (;;) {
+      while (!live) {
+        const uint32_t p = first + (k / run) * (BLOCK * run) + t * run + k % run;
+        if (p >= last) break;  // no paths left (later ones lie further on)
+        ++k;
+        const double charge = (double)q[p];
+        bool finite = true;
+        double extent = 0.0;
+#pragma unroll
+        for (int d = 0; d < NDIM; ++d) {
+          const double s0 = (double)st[d * n + p], e0 = (double)en[d * es + p];
+          finite = finite && (s0 - s0 == 0.0) && (e0 - e0 == 0.0);
+          w.r0[d] = (s0 - g.lower[d]) * g.inverse[d];
+          w.r1[d] = (e0 - g.lower[d]) * g.inverse[d];
+          const double span = w.r1[d] > w.r0[d] ? w.r1[d] - w.r0[d] : w.r0[d] - w.r1[d];
+          extent = span > extent ? span : extent;
+        }
+        if (charge == 0.0 || !finite) continue;
+        w.charge = charge;
+        w.stretches = (int)(extent / 4.0) + 1;
+        w.st = 0;
+        walk_stretch<NDIM, DEG>(w, g);
+        live = true;
+      }
+      if (!live) break;
+      const double t1 = piece_end<NDIM>(w);
+      piece_into_tile<NDIM, DEG>(w, t1, g, ax, (lds_u64 *)lds, accum, scale);
+      w.t0 = t1;
+      if (t1 >= 1.0) {
+        if (++w.st < w.stretches) walk_stretch<NDIM, DEG>(w, g);
+        else live = false;
+      }
     }
   }
   __syncthreads();
@@ -828,17 +974,21 @@ static void dispatch_paths(int kind, int ndim, int mode, const void *st, int64_t
 }
 
 template <class T, int NDIM>
-static void launch_paths_tiled(int degree, const void *spos, const void *srows, int64_t n, const uint32_t *bins,
+static void launch_paths_tiled(int degree, const void *spos, const void *q, const void *en, int64_t es, int64_t n,
+                               const uint32_t *bins,
                                const Grid &g, unsigned long long *accum, const double *slot, hipStream_t stream) {
   using PT = PathTile<NDIM>;
   const int ntx = (int)((g.num[0] + PT::TX - 1) / PT::TX), nty = (int)((g.num[1] + PT::TY - 1) / PT::TY);
   const int ntz = NDIM == 3 ? (int)((g.num[2] + PT::TZ - 1) / PT::TZ) : 1;
-  const dim3 grid(ntx * nty * ntz), block(NDIM == 3 ? 1024 : 256);
+  const dim3 grid(ntx * nty * ntz);
   ProfileScope prof("spline_paths_tiled", stream);
+  // 3D: 1024 work-items for the Whitney walk (125 VGPRs), 512 for the spline edges, whose
+  // per-piece sums need more than the 128 VGPRs of 1024 (2^23 paths on 128^3, degree 2:
+  // 512 work-items without spills 2.8 ms, 1024 with 120 VGPRs spilled 4.7 ms)
 #define TILED(D)                                                                                                   \
-  hipLaunchKernelGGL((spline_paths_tiled<T, NDIM, D, NDIM == 3 ? 1024 : 256>), grid, block, 0, stream,           \
-                     (const T *)spos,                                                                              \
-                     (const T *)srows, n, bins, g, nty, ntz, accum, slot)
+  hipLaunchKernelGGL((spline_paths_tiled<T, NDIM, D, NDIM == 3 ? (D == 0 ? 1024 : 512) : 256>), grid,             \
+                     dim3(NDIM == 3 ? (D == 0 ? 1024 : 512) : 256), 0, stream,                                     \
+                     (const T *)spos, (const T *)q, (const T *)en, es, n, bins, g, nty, ntz, accum, slot)
   if (degree == 0) TILED(0);  // Whitney
   else if (degree == 1) TILED(1);
   else if (degree == 2) TILED(2);
@@ -1070,10 +1220,13 @@ extern "C" int mpf_deposit_paths(int kind, int degree, int ndim, const void *sta
   }
   if (tiled) {
     const int tdeg = kind == 0 ? 0 : deg;  // degree 0 selects the Whitney tile kernel
-    if (ndim == 2 && dtype == F32) launch_paths_tiled<float, 2>(tdeg, spos, srows, n, bins, g, accum, slot, stream);
-    if (ndim == 2 && dtype == F64) launch_paths_tiled<double, 2>(tdeg, spos, srows, n, bins, g, accum, slot, stream);
-    if (ndim == 3 && dtype == F32) launch_paths_tiled<float, 3>(tdeg, spos, srows, n, bins, g, accum, slot, stream);
-    if (ndim == 3 && dtype == F64) launch_paths_tiled<double, 3>(tdeg, spos, srows, n, bins, g, accum, slot, stream);
+    // the sorted rows: the charge, then the end point
+    const size_t elem = dtype == F32 ? 4 : 8;
+    const void *sq = srows, *sen = (const char *)srows + (size_t)n * elem;
+    if (ndim == 2 && dtype == F32) launch_paths_tiled<float, 2>(tdeg, spos, sq, sen, n, n, bins, g, accum, slot, stream);
+    if (ndim == 2 && dtype == F64) launch_paths_tiled<double, 2>(tdeg, spos, sq, sen, n, n, bins, g, accum, slot, stream);
+    if (ndim == 3 && dtype == F32) launch_paths_tiled<float, 3>(tdeg, spos, sq, sen, n, n, bins, g, accum, slot, stream);
+    if (ndim == 3 && dtype == F64) launch_paths_tiled<double, 3>(tdeg, spos, sq, sen, n, n, bins, g, accum, slot, stream);
   } else if (dtype == F32) {
     dispatch_paths<float>(kind, ndim, mode, start, start_stride, end, end_stride, charges, n, g, deg, edges, accum,
                           slot, stream);
@@ -1084,6 +1237,81 @@ extern "C" int mpf_deposit_paths(int kind, int degree, int ndim, const void *sta
   MPF_LAUNCH_CHECK();
   if (mode == MODE_FIXED) return finalize_fixed(ctx, edges, count, stream);
   return MPF_OK;
+}
+
+// Paths kept binned by the caller (sorted_start in the bins' order, with end and charges in
+// the same order): the tiled fixed-point deposit without the sort.  Whitney paths read bins
+// of the linear cell, spline-edge paths bins of the spline nodes of their degree.
+extern "C" int mpf_deposit_paths_binned(int kind, int degree, int ndim, const void *sorted_start, const void *end,
+                                        int64_t end_stride, const void *charges, int dtype, int64_t n,
+                                        const uint32_t *start, double *edges, const double *lower,
+                                        const double *cell_size, const int64_t *num_nodes, const int64_t *code,
+                                        mpf_context *ctx, mpf_stream stream_) {
+  if (kind != 0 && kind != 1) {
+    set_error("path kind must be 0 (Whitney) or 1 (spline edges), got %d", kind);
+    return MPF_ERR_ARGUMENT;
+  }
+  if (kind == 1 && (degree < 1 || degree > 3)) {
+    set_error("spline edge degrees are 1 to 3, got %d", degree);
+    return MPF_ERR_ARGUMENT;
+  }
+  if (dtype != F32 && dtype != F64) {
+    set_error("dtype must be MPF_F32 or MPF_F64, got %d", dtype);
+    return MPF_ERR_ARGUMENT;
+  }
+  if (ndim != 2 && ndim != 3) {
+    set_error("binned path deposits are 2D or 3D, got %dD", ndim);
+    return MPF_ERR_UNSUPPORTED;
+  }
+  if (n < 0 || (n > 0 && end_stride < n)) {
+    set_error("need n >= 0 and end_stride >= n");
+    return MPF_ERR_ARGUMENT;
+  }
+  for (int d = 0; d < ndim; ++d)
+    if (code[d] != 0 && code[d] != 1) {
+      set_error("binned path deposits take bounded or periodic axes (clamped axes: mpf_deposit_paths)");
+      return MPF_ERR_UNSUPPORTED;
+    }
+  Grid g;
+  const int deg = kind == 0 ? 1 : degree;
+  int rc = make_grid(g, ndim, kind == 0 ? LINEAR : SPLINE, deg, lower, cell_size, num_nodes, code);
+  if (rc) return rc;
+  if (n == 0) return MPF_OK;
+  hipStream_t stream = (hipStream_t)stream_;
+  if (!ctx) ctx = default_context(stream);
+  if (!ctx) return MPF_ERR_HIP;
+  const int64_t count = g.extent[0] * g.extent[1] * (ndim > 2 ? g.extent[2] : 1) * ndim;
+  rc = ensure(ctx->accum, (size_t)count * 8, true, stream);
+  if (rc) return rc;
+  rc = ensure(ctx->scalars, 64, true, stream);
+  if (rc) return rc;
+  unsigned long long *maxbits = (unsigned long long *)ctx->scalars.ptr;
+  rc = zero_async(maxbits, 8, stream);
+  if (rc) return rc;
+  const int threads = 256, blocks = blocks_for(n, threads, 4096);
+  if (dtype == F32)
+    hipLaunchKernelGGL(path_bound_kernel<float>, dim3(blocks), dim3(threads), 0, stream, (const float *)sorted_start,
+                       n, (const float *)end, end_stride, (const float *)charges, n, g, maxbits);
+  else
+    hipLaunchKernelGGL(path_bound_kernel<double>, dim3(blocks), dim3(threads), 0, stream,
+                       (const double *)sorted_start, n, (const double *)end, end_stride, (const double *)charges, n,
+                       g, maxbits);
+  MPF_LAUNCH_CHECK();
+  rc = prepare_scale(ctx, nullptr, 0, dtype, 1, n, 1.0, MPF_SCALE_AUTO, stream);
+  if (rc) return rc;
+  unsigned long long *accum = (unsigned long long *)ctx->accum.ptr;
+  const double *slot = scale_slot(ctx);
+  const int tdeg = kind == 0 ? 0 : deg;
+  if (ndim == 2 && dtype == F32)
+    launch_paths_tiled<float, 2>(tdeg, sorted_start, charges, end, end_stride, n, start, g, accum, slot, stream);
+  if (ndim == 2 && dtype == F64)
+    launch_paths_tiled<double, 2>(tdeg, sorted_start, charges, end, end_stride, n, start, g, accum, slot, stream);
+  if (ndim == 3 && dtype == F32)
+    launch_paths_tiled<float, 3>(tdeg, sorted_start, charges, end, end_stride, n, start, g, accum, slot, stream);
+  if (ndim == 3 && dtype == F64)
+    launch_paths_tiled<double, 3>(tdeg, sorted_start, charges, end, end_stride, n, start, g, accum, slot, stream);
+  MPF_LAUNCH_CHECK();
+  return finalize_fixed(ctx, edges, count, stream);
 }
 
 #define GEOM1 MPF_GEOM_1D

@@ -177,3 +177,28 @@ def test_float32_tiled_paths_match_oracle_and_conserve_charge(degree, ndim, peri
     div = K.divergence(got, [periodic] * ndim)
     change = (rho1 - rho0)[..., 0]
     assert float((div - change).abs().max()) < 1e-9 * max(float(change.abs().max()), 1.0)
+
+
+@pytest.mark.parametrize("periodic", [1, 0])
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("degree", [0, 1, 2, 3])
+def test_binned_path_deposit_equals_the_sorting_deposit(degree, ndim, periodic):
+    """Paths kept binned by their start (binning.deposit_paths_binned) deposit the same edges,
+    bit for bit, as mpf_deposit_paths, which bins them inside the call; dead paths (zero
+    charge, NaN end) included; wrong bins are refused."""
+    from mpifield_amd import binning
+
+    print(f"binned paths p={degree} {ndim}D periodic={periodic}", flush=True)
+    nodes = 256 if ndim == 2 else 48
+    n = 1 << 17
+    g, shape, start, end, q = short_paths(ndim, periodic, n, seed=70 + degree, nodes=nodes)
+    kind, deg = (0, 1) if degree == 0 else (1, degree)
+    want = torch.zeros((*shape, ndim), dtype=torch.float64, device="cuda")
+    K.deposit_paths(kind, deg, cuda(start), cuda(end), cuda(q), want, g, mode=_lib.MODE_FIXED)
+    bins = binning.bin_markers(cuda(start), None, (0, 1) if degree == 0 else (2, degree), g)
+    order = bins.order
+    got = torch.zeros_like(want)
+    binning.deposit_paths_binned(bins, cuda(end)[:, order], cuda(q)[order], got, kind, deg)
+    assert torch.equal(got, want)
+    with pytest.raises(ValueError):
+        binning.deposit_paths_binned(bins, cuda(end)[:, order], cuda(q)[order], got, 1, deg + 1 if kind else 1)
