@@ -285,4 +285,24 @@ __device__ __forceinline__ unsigned long long to_fixed_small(double x) {
   return (unsigned long long)__double_as_longlong(x + bias) - (unsigned long long)__double_as_longlong(bias);
 }
 
+// ---- exact fixed-point accumulation in float64 registers --------------------------------
+//
+// acc holds 1.5 * 2^52 + s for an integer s with |s| < 2^51, so its last place is 1.
+// exact_fma(u, w, acc) = fma(u, w, acc) rounds acc + u*w to the nearest integer once.  With
+// u's lowest significand bit forced to one and |u| < 2^51, u = odd * 2^e with e <= -2, and
+// w in [0, 1] is odd * 2^f with f <= 0: the exact product is an odd multiple of a power of
+// two at most 1/4, so it is never halfway between two integers and the rounding is the
+// same whatever acc holds.  The accumulated integer is therefore the sum of the
+// contributions each rounded to the nearest quantum, in any order.
+__device__ __forceinline__ double odd_significand(double u) {
+  long long b = __double_as_longlong(u);
+  return __longlong_as_double(b | 1ll);
+}
+
+__device__ __forceinline__ long long biased_to_int(double acc) {
+  return __double_as_longlong(acc) - 0x4338000000000000ll;  // acc - 1.5 * 2^52, as an integer
+}
+
+constexpr double BIAS = 6755399441055744.0;  // 1.5 * 2^52
+
 }  // namespace mpf

@@ -40,7 +40,7 @@ struct PatchStencil {
 };
 
 template <int KIND, int P>
-__device__ __forceinline__ void patch_stencil(double rel, int num, int code, int *node, double *w) {
+__device__ __forceinline__ int patch_stencil(double rel, int num, int code, int *node, double *w) {
   double r = rel;
   if (KIND == LINEAR) {
     int l;
@@ -57,7 +57,7 @@ __device__ __forceinline__ void patch_stencil(double rel, int num, int code, int
     const double f = d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d);
     w[0] = 1.0 - f;
     w[1] = f;
-    return;
+    return l;
   }
   if (code == 1) {
     if (!(r >= 0.0 && r < (double)num)) r = r - (double)num * floor(r / (double)num);
@@ -93,18 +93,22 @@ __device__ __forceinline__ void patch_stencil(double rel, int num, int code, int
     else
       node[a] = k < 0 ? 0 : (k > num - 1 ? num - 1 : k);
   }
+  return middle;
 }
 
 // The node offsets of one marker of patch p (each axis's stencil, row-major strides), and
 // its weights; false for a dead marker.
 template <int NDIM, int KIND, int P, class T>
-__device__ __forceinline__ void patch_marker(const mpf_patch &p, const T *__restrict__ pos, int64_t ps, int64_t m,
-                                             int (&node)[3][4], double (&w)[3][4]) {
+__device__ __forceinline__ int patch_marker(const mpf_patch &p, const T *__restrict__ pos, int64_t ps, int64_t m,
+                                            int (&node)[3][4], double (&w)[3][4]) {
+  int key = 0;
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
     const double rel = ((double)pos[d * ps + m] - p.lower[d]) * (1.0 / p.cell_size[d]);
-    patch_stencil<KIND, P>(rel, (int)p.num_nodes[d], (int)p.code[d], node[d], w[d]);
+    const int anchor = patch_stencil<KIND, P>(rel, (int)p.num_nodes[d], (int)p.code[d], node[d], w[d]);
+    key = key * ((int)p.num_nodes[d] + 1) + anchor;  // anchors lie in [0, num]
   }
+  return key;
 }
 
 template <int NDIM, int KIND, int P, class T>
@@ -131,11 +135,90 @@ __device__ __forceinline__ void patch_scatter_item(const mpf_patch &p, const T *
           const double weight = NDIM == 3 ? wab * w[NDIM - 1][e] : wab;
           const int at = (node[0][a] * s0 + (NDIM >= 2 ? node[NDIM >= 2 ? 1 : 0][b] * s1 : 0) +
                           (NDIM == 3 ? node[NDIM - 1][e] : 0)) * ncomp;
+          // the automatic scale keeps one marker's value below 2^49 quanta: to_fixed_small
           for (int c = 0; c < ncomp; ++c)
-            __hip_atomic_fetch_add(lds + at + c, to_fixed(weight * (double)val[c * vs + m] * scale), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(lds + at + c, to_fixed_small(weight * (double)val[c * vs + m] * scale),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
       }
+  }
+}
+
+// The deposit of an item's markers of one component in runs: work-item t takes runs of
+// RUN consecutive markers, run t of each pass of blockDim runs (a wavefront's loads stay
+// near each other), and sums the contributions of consecutive markers with the same stencil
+// (the same anchor cell) in float64 registers, exactly (the biased fma of
+// odd_significand), adding them into the LDS copy when the anchor changes, after
+// `per_pass` markers (the scale's bound on one register sum) and at the end of the run.  On
+// markers kept sorted by cell within their patch, a run is mostly one cell: one LDS add
+// per node and cell instead of one per marker.  Each contribution is rounded on its own, so
+// the sums are the same integers whatever the runs.
+template <int NDIM, int KIND, int P, class T>
+__device__ __forceinline__ void patch_scatter_runs(const mpf_patch &p, const T *__restrict__ pos, int64_t ps,
+                                                   const T *__restrict__ val, int64_t first, int64_t count,
+                                                   lds_u64 *lds, double scale, int per_pass) {
+  constexpr int S = PatchStencil<KIND, P>::S, RUN = 8;
+  constexpr int SB = NDIM >= 2 ? S : 1, SE = NDIM == 3 ? S : 1;
+  const int s1 = NDIM == 3 ? (int)p.num_nodes[2] : 1;
+  const int s0 = NDIM == 3 ? (int)(p.num_nodes[1] * p.num_nodes[2]) : (NDIM == 2 ? (int)p.num_nodes[1] : 1);
+  const int64_t end = first + count;
+  for (int64_t lo = first + (int64_t)threadIdx.x * RUN; lo < end; lo += (int64_t)blockDim.x * RUN) {
+    const int64_t hi = lo + RUN < end ? lo + RUN : end;
+    double acc[S][SB][SE];
+    int at[S][SB][SE];
+    int key = -1, in_pass = 0;
+    for (int64_t m = lo; m < hi; ++m) {
+      const double v = (double)val[m];
+      if (v == 0.0) continue;  // a dead marker deposits nothing
+      int node[3][4];
+      double w[3][4];
+      const int k = patch_marker<NDIM, KIND, P, T>(p, pos, ps, m, node, w);
+      if (k != key || in_pass >= per_pass) {
+        if (key >= 0) {
+#pragma unroll
+          for (int a = 0; a < S; ++a)
+#pragma unroll
+            for (int b = 0; b < SB; ++b)
+#pragma unroll
+              for (int e = 0; e < SE; ++e)
+                __hip_atomic_fetch_add(lds + at[a][b][e], (unsigned long long)biased_to_int(acc[a][b][e]),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        key = k;
+        in_pass = 0;
+#pragma unroll
+        for (int a = 0; a < S; ++a)
+#pragma unroll
+          for (int b = 0; b < SB; ++b)
+#pragma unroll
+            for (int e = 0; e < SE; ++e) {
+              acc[a][b][e] = BIAS;
+              at[a][b][e] = node[0][a] * s0 + (NDIM >= 2 ? node[NDIM >= 2 ? 1 : 0][b] * s1 : 0) +
+                            (NDIM == 3 ? node[NDIM - 1][e] : 0);
+            }
+      }
+      const double u = odd_significand(v * scale);
+#pragma unroll
+      for (int a = 0; a < S; ++a)
+#pragma unroll
+        for (int b = 0; b < SB; ++b) {
+          const double wab = NDIM >= 2 ? w[0][a] * w[NDIM >= 2 ? 1 : 0][b] : w[0][a];
+#pragma unroll
+          for (int e = 0; e < SE; ++e)
+            acc[a][b][e] = __builtin_fma(u, NDIM == 3 ? wab * w[NDIM - 1][e] : wab, acc[a][b][e]);
+        }
+      ++in_pass;
+    }
+    if (key >= 0) {
+#pragma unroll
+      for (int a = 0; a < S; ++a)
+#pragma unroll
+        for (int b = 0; b < SB; ++b)
+#pragma unroll
+          for (int e = 0; e < SE; ++e)
+            __hip_atomic_fetch_add(lds + at[a][b][e], (unsigned long long)biased_to_int(acc[a][b][e]),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
   }
 }
 
@@ -189,33 +272,46 @@ __global__ void __launch_bounds__(512) patches_scatter_kernel(const T *__restric
                                                               const Item *__restrict__ items, int64_t n_items,
                                                               unsigned int *__restrict__ counter,
                                                               unsigned long long *__restrict__ accum,
-                                                              const double *__restrict__ slot) {
+                                                              const double *__restrict__ slot, int lds_words) {
   extern __shared__ unsigned long long lds[];
-  __shared__ int64_t current;
+  __shared__ int64_t taken[2];
   const double scale = slot[0];
+  // markers per exact register sum: the scale keeps every sum of this many below 2^51 quanta
+  const int per_pass = slot[2] < 8.0 ? (int)slot[2] : 8;
+  // the LDS copy starts zero and each item leaves its words zero again (its flush clears
+  // them), so no item clears the copy first
+  for (int i = threadIdx.x; i < lds_words; i += blockDim.x) lds[i] = 0ull;
+  if (threadIdx.x == 0) taken[0] = (int64_t)atomicAdd(counter, 1u);
+  __syncthreads();
+  int buf = 0;
   while (true) {
-    if (threadIdx.x == 0) current = (int64_t)atomicAdd(counter, 1u);
-    __syncthreads();
-    const int64_t it = current;
+    const int64_t it = taken[buf];
     if (it >= n_items) break;  // every wave of the workgroup leaves together
+    // the next item's index, fetched while this one is deposited
+    if (threadIdx.x == 0) taken[buf ^ 1] = (int64_t)atomicAdd(counter, 1u);
     const Item item = items[it];
     const mpf_patch p = patches[item.patch];
     int nodes = 1;
 #pragma unroll
     for (int d = 0; d < NDIM; ++d) nodes *= (int)p.num_nodes[d];
     const int values = nodes * ncomp;
-    for (int i = threadIdx.x; i < values; i += blockDim.x) lds[i] = 0ull;
-    __syncthreads();
-#define SCATTER(D, K, P) \
-  patch_scatter_item<D, K, P, T>(p, pos, ps, val, vs, ncomp, item.first, item.count, (lds_u64 *)lds, scale)
+#define SCATTER(D, K, P)                                                                                        \
+  if (ncomp == 1 && (D < 3 || PatchStencil<K, P>::S <= 3))                                                        \
+    patch_scatter_runs<D, K, P, T>(p, pos, ps, val, item.first, item.count, (lds_u64 *)lds, scale, per_pass);     \
+  else                                                                                                           \
+    patch_scatter_item<D, K, P, T>(p, pos, ps, val, vs, ncomp, item.first, item.count, (lds_u64 *)lds, scale)
     PATCH_DISPATCH(NDIM, SCATTER)
 #undef SCATTER
     __syncthreads();
     for (int i = threadIdx.x; i < values; i += blockDim.x) {
       const unsigned long long q = lds[i];
-      if (q != 0ull) atomicAdd(accum + p.field_offset + i, q);
+      if (q != 0ull) {
+        atomicAdd(accum + p.field_offset + i, q);
+        lds[i] = 0ull;
+      }
     }
-    __syncthreads();  // the LDS copy and `current` are reused by the next item
+    __syncthreads();  // the copy is zero again and the next index is in `taken`
+    buf ^= 1;
   }
 }
 
@@ -347,7 +443,7 @@ extern "C" int mpf_patches_scatter(const mpf_patch_plan *plan, const void *posit
   hipLaunchKernelGGL((patches_scatter_kernel<T, D>), dim3(workgroups), dim3(block), lds, stream,                  \
                      (const T *)positions, pos_stride, (const T *)values, val_stride, (int)ncomp,                 \
                      (const mpf_patch *)plan->patches, (const Item *)plan->items, plan->n_items, plan->counter, \
-                     (unsigned long long *)ctx->accum.ptr, scale_slot(ctx))
+                     (unsigned long long *)ctx->accum.ptr, scale_slot(ctx), (int)(lds / 8))
     if (dtype == F32) {
       if (plan->ndim == 1) LAUNCH(float, 1);
       if (plan->ndim == 2) LAUNCH(float, 2);

@@ -133,6 +133,38 @@ class PatchBatch:
             _lib.call("mpf_patches_gather", self.plan, pos.data_ptr(), ps, K._dtype_code(pos), fields.data_ptr(),
                       self.ncomp, result.data_ptr(), rs, 0, None, K.stream_handle(fields.device))
 
+    def sort(self, positions: torch.Tensor, values: torch.Tensor | None = None):
+        """The batch's markers ordered by cell within each patch, the order a particle code
+        keeps between steps: returns (positions, values, order) with column i of the result
+        column order[i] of the input.  Each patch's markers stay within its range, so the
+        plan serves the sorted arrays as they are; the deposit gives the same integers in
+        any order, and on sorted markers a work-item sums runs of one cell in registers."""
+        pos, _ = K.marker_rows(positions, self.ndim)
+        dev = pos.device
+        counts = torch.as_tensor([int(p.num_markers) for p in self.patches], device=dev)
+        patch_of = torch.repeat_interleave(torch.arange(len(self.patches), device=dev), counts)
+        lower = torch.as_tensor([[p.lower[d] for d in range(self.ndim)] for p in self.patches],
+                                dtype=torch.float64, device=dev)[patch_of]
+        cell = torch.as_tensor([[p.cell_size[d] for d in range(self.ndim)] for p in self.patches],
+                               dtype=torch.float64, device=dev)[patch_of]
+        nodes = torch.as_tensor([[p.num_nodes[d] for d in range(self.ndim)] for p in self.patches],
+                                dtype=torch.int64, device=dev)
+        sizes = nodes.prod(dim=1)
+        offsets = torch.cumsum(sizes, 0) - sizes
+        num = nodes[patch_of]
+        key = offsets[patch_of]
+        stride = torch.ones(len(patch_of), dtype=torch.int64, device=dev)
+        for d in reversed(range(self.ndim)):
+            rel = (pos[d, : self.markers].double() - lower[:, d]) / cell[:, d]
+            c = torch.nan_to_num(torch.floor(rel), nan=0.0).clamp(0, None).to(torch.int64)
+            c = torch.minimum(c, num[:, d] - 1)
+            key = key + c * stride
+            stride = stride * num[:, d]
+        order = torch.argsort(key, stable=True)
+        out_pos = positions[:, : self.markers][:, order].contiguous()
+        out_val = None if values is None else values[:, : self.markers][:, order].contiguous()
+        return out_pos, out_val, order
+
     def capture(self, positions, values, fields) -> torch.cuda.CUDAGraph:
         """Record one deposit of the batch in a HIP graph; replay with ``graph.replay()``.
 
