@@ -83,7 +83,7 @@ def measured_traffic(kernel: str, n: int, nodes: int, comps: int) -> tuple[float
         return None, None
     summary = json.loads(TRAFFIC_SUMMARY.read_text())
     for name, value in summary.get("traffic_bytes_per_launch", {}).items():
-        if name.startswith(kernel) or (kernel == "gather_tiled" and name.startswith("gather_items")):
+        if name.startswith(kernel) or (kernel == "gather_tiled" and name.startswith("gather_lanes")):
             return float(value), str(TRAFFIC_SUMMARY.relative_to(ROOT))
     return None, None
 

@@ -917,11 +917,18 @@ template <class T>
 __global__ void path_bound_kernel(const T *__restrict__ st, int64_t ss, const T *__restrict__ en, int64_t es,
                                   const T *__restrict__ q, int64_t n, Grid g, unsigned long long *__restrict__ out) {
   double m = 0.0;
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-    double len = 0.0;
-    for (int d = 0; d < g.ndim; ++d) len += fabs(((double)en[d * es + p] - (double)st[d * ss + p]) * g.inverse[d]);
-    const double b = fabs((double)q[p]) * (len + 2.0);
+  // two paths of the grid-stride walk per trip, their loads issued together
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += 2 * stride) {
+    const int64_t p2 = p + stride < n ? p + stride : p;
+    double len = 0.0, len2 = 0.0;
+    for (int d = 0; d < g.ndim; ++d) {
+      len += fabs(((double)en[d * es + p] - (double)st[d * ss + p]) * g.inverse[d]);
+      len2 += fabs(((double)en[d * es + p2] - (double)st[d * ss + p2]) * g.inverse[d]);
+    }
+    const double b = fabs((double)q[p]) * (len + 2.0), b2 = fabs((double)q[p2]) * (len2 + 2.0);
     if (b > m && b < 1.0e300) m = b;
+    if (b2 > m && b2 < 1.0e300) m = b2;
   }
   for (int off = 32; off > 0; off >>= 1) {
     double o = __shfl_xor(m, off, 64);
