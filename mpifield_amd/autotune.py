@@ -68,7 +68,7 @@ def _time(torch, fn, repeats: int) -> float:
 
 
 def tune_tiled(kind: int, degree: int, ncomp_group: int, nodes=(256, 256, 256), n: int = 1 << 24,
-               repeats: int = 5, seed: int = 0) -> list[dict]:
+               repeats: int = 5, seed: int = 0, rounds: int = 3) -> list[dict]:
     """Time every launch variant of the deposit and the gather of ``ncomp_group`` (1 or 3)
     components of weighting (kind, degree) on a periodic grid of ``nodes`` with ``n``
     uniform float32 markers; set the fastest of each in the library and return one entry per
@@ -91,10 +91,18 @@ def tune_tiled(kind: int, degree: int, ncomp_group: int, nodes=(256, 256, 256), 
     entries = []
     for op, fn in runs.items():
         code = OPS[op]
-        times = {}
+        # every variant warmed up first, then timed in interleaved rounds (the median of
+        # each): timing the variants one after the other in a single pass favoured the later
+        # ones while the clocks ramped up
         for v in range(len(shapes)):
             lib.mpf_tiled_variant(code, kind, degree, ncomp_group, v)
-            times[v] = _time(torch, fn, repeats)
+            _time(torch, fn, 2)
+        samples = {v: [] for v in range(len(shapes))}
+        for _ in range(rounds):
+            for v in range(len(shapes)):
+                lib.mpf_tiled_variant(code, kind, degree, ncomp_group, v)
+                samples[v].append(_time(torch, fn, repeats))
+        times = {v: sorted(t)[len(t) // 2] for v, t in samples.items()}
         best = min(times, key=times.get)
         lib.mpf_tiled_variant(code, kind, degree, ncomp_group, best)
         entries.append({"op": op, "kind": kind, "degree": degree, "ncomp_group": ncomp_group, "variant": best,
