@@ -337,7 +337,9 @@ int mpf_bin_sizes(int ndim, const int64_t *num_nodes, int64_t *cells);
 /* On-device counting sort of markers by stencil cell (the cell whose nodes a marker's
  * stencil covers, found with the reference's float64 arithmetic): in 3D the cells of each
  * tile of 8 x 8 x 16 cells are consecutive bins (x-major within the tile, tiles x-major),
- * so a tile's markers are one contiguous run; in 1D and 2D, row-major cell order.  Markers
+ * so a tile's markers are one contiguous run; in 1D and 2D, row-major cell order.  A
+ * clamped axis (code 2) is binned in 3D for the spline kernels (kind 2) only, by the cell
+ * 0 .. num - 2 whose degree + 1 knots' basis functions hold the position.  Markers
  * whose values are all zero go to a trailing bin.  values may be NULL with
  * ncomp 0 (a gather's markers: all live).  Writes, into caller-owned device memory:
  * sorted_positions (ndim, n) and sorted_values (ncomp, n) with contiguous rows, order[n]
@@ -361,7 +363,7 @@ int mpf_bin_markers(int kind, int degree, int ndim, const void *positions, int64
  * relative to the markers' first input (old_order composed).  3D: an incremental pass (the
  * markers that stay in their tile of 8 x 8 x 16 cells or reach a neighbouring one) that
  * synchronizes the stream once to read a counter, and sorts from scratch when a marker went
- * further; 1D and 2D: mpf_bin_markers.  No counterpart in the reference, whose banded CPU
+ * further; 1D and 2D, and bins with a clamped axis: mpf_bin_markers.  No counterpart in the reference, whose banded CPU
  * buffers assume markers kept sorted by the caller (scatter_linear_3d_pyccel.py:11-16). */
 int mpf_rebin_markers(int kind, int degree, int ndim, const void *positions, int64_t pos_stride, const void *values,
                       int64_t val_stride, int dtype, int64_t n, int64_t ncomp, const int64_t *old_order,
@@ -372,9 +374,10 @@ int mpf_rebin_markers(int kind, int degree, int ndim, const void *positions, int
 
 /* Deposit binned markers (scatter_quadratic_3d etc., the tiled form): one workgroup per
  * tile of cells accumulates int64 fixed point in LDS and adds the tile into field once;
- * bit-reproducible.  3D, codes 0/1: linear, quadratic and spline 0..2 with 1 or 3
- * components, cubic with 1; 2D, codes 0/1: every node weighting with 1 to 3 components;
- * MPF_ERR_UNSUPPORTED otherwise (use mpf_scatter).  item_tables / item_counts: the 3D bins'
+ * bit-reproducible.  3D: every node weighting with 1 to 64 components on bounded axes and
+ * the splines' clamped axes of at least a tile (8, 8, 16 cells), periodic axes of at least
+ * two tiles and a stencil footprint; 2D, codes 0/1: every node weighting with 1 to 3
+ * components; MPF_ERR_UNSUPPORTED otherwise (use mpf_scatter).  item_tables / item_counts: the 3D bins'
  * tables from mpf_bin_items, or NULL (each launch builds them). */
 int mpf_scatter_binned(int kind, int degree, int ndim, const void *sorted_positions, const void *sorted_values,
                        int dtype, int64_t n, int64_t ncomp, const uint32_t *start, const uint16_t *item_tables,

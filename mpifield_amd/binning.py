@@ -206,7 +206,8 @@ def _bins_field(bins: MarkerBins, field, ncomp: int | None = None, form: int | N
     if form is not None:
         field = K._form_field(field, bins.degree, form, num, code)
     else:
-        want = (*(int(v) for v in num), field.shape[-1] if ncomp is None else ncomp)
+        extent = K._expected_extent(num, code, bins.degree, bins.kind)  # num + degree - 1 on a clamped axis
+        want = (*extent, field.shape[-1] if ncomp is None else ncomp)
         if tuple(field.shape) != want:
             raise ValueError(f"a field on this grid is {want}, got {tuple(field.shape)}")
     if field.device != bins.positions.device:

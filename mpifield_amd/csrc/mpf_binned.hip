@@ -38,6 +38,37 @@ namespace mpf {
 
 // ---- per-axis stencil cell and weights -----------------------------------------------
 
+// The cell of rel on a clamped axis (clamped_cell's: the position clamped to [0, num - 1],
+// the cell to [0, num - 2]); its stencil nodes are cell .. cell + P of num + P - 1.
+__device__ __forceinline__ int clamped_axis_cell(double rel, int num) {
+  double r = rel;
+  if (r < 0.0) r = 0.0;
+  if (r > (double)(num - 1)) r = (double)num - 1.0;  // NaN stays NaN: the cast below gives 0
+  const int cell = (int)fmax(r, 0.0);
+  return cell > num - 2 ? num - 2 : cell;
+}
+
+// The first stencil node of a tile cell's stencil relative to the cell: -OFF for the
+// uniform stencils (anchor - OFF + a), 0 on a clamped axis (cell + a).
+template <int OFF>
+__device__ __forceinline__ int axis_off(int code) { return code == 2 ? 0 : OFF; }
+
+// The anchor and weights of rel on one axis as the tiles see them: a tile's footprint
+// position of stencil node a is anchor - (tile start) + a, for the uniform stencils
+// (anchor_axis, nodes anchor - OFF + a) and on a clamped axis (clamped_cell, nodes cell + a).
+template <int KIND, int P>
+__device__ __forceinline__ int anchor_axis(double rel, int num, int code, double *w);
+
+template <int KIND, int P>
+__device__ __forceinline__ int tile_anchor(double rel, int num, int code, double *w) {
+  if (KIND == SPLINE && code == 2) {
+    int64_t node[4];
+    clamped_cell(rel, num, P, 0, node, w);
+    return (int)node[0];
+  }
+  return anchor_axis<KIND, P>(rel, num, code, w);
+}
+
 // The anchor (unwrapped) and the weights of rel on one axis; nodes are anchor - OFF + a,
 // a = 0..P, OFF = P / 2 (linear: 0), clamped to [0, num - 1] on a bounded axis and taken
 // modulo num on a periodic one.  Same arithmetic as mpf_device.h's stencils.
@@ -124,6 +155,11 @@ __device__ __forceinline__ bool offset_in_cell(double rel, double o, double hi, 
 // from the full selection elsewhere.
 template <int KIND, int P>
 __device__ __forceinline__ void cell_weights_f64(double rel, double cell, double hi, int num, int code, double *w) {
+  if (KIND == SPLINE && code == 2) {  // clamped knots: the direct kernels' Cox-de Boor weights
+    int64_t node[4];
+    clamped_cell(rel, num, P, 0, node, w);
+    return;
+  }
   const double o = rel - cell;
   if (!offset_in_cell<KIND, P>(rel, o, hi, code)) {
     anchor_axis<KIND, P>(rel, num, code, w);
@@ -204,6 +240,10 @@ __device__ __forceinline__ uint32_t marker_key(const T *__restrict__ pos, int64_
     double w[4];
     const double rel = ((double)x[d] - g.lower[d]) * g.inverse[d];
     const int num = (int)g.num[d];
+    if (KIND == SPLINE && g.code[d] == 2) {
+      cell[d] = clamped_axis_cell(rel, num);  // in [0, num - 2]
+      continue;
+    }
     int a = anchor_axis<KIND, P>(rel, num, g.code[d], w);
     if (g.code[d] == 1 && a >= num) a -= num;
     cell[d] = a < 0 ? 0 : (a > num - 1 ? num - 1 : a);  // a key in range whatever the position
@@ -1283,31 +1323,35 @@ __device__ __forceinline__ void deposit_markers(unsigned long long *acc, int bas
   }
 }
 
-// The field index (times C) of node (gx, gy, z0 - OFF) for each footprint row (ix, iy) of a
+// The field index (times C) of node (gx, gy, z0 - off_z) for each footprint row (ix, iy) of a
 // tile, wrapped on periodic axes; -1 for a row beyond a bounded wall (clamp = false) or the
-// clamped row (clamp = true, the gather's view).  A row is LZ nodes along z.
+// clamped row (clamp = true, the gather's view).  A row is LZ nodes along z.  The footprint
+// of an axis starts OFF nodes before the tile (the uniform stencils' first node), or at the
+// tile on a clamped axis, whose extent is num + P - 1 nodes and which no stencil leaves.
 template <int LX, int LY, int OFF, int BLOCK>
 __device__ __forceinline__ void footprint_rows(const Grid &g, int x0, int y0, int z0, bool clamp, int fc,
                                                long long *rows) {
-  const int num0 = (int)g.num[0], num1 = (int)g.num[1], num2 = (int)g.num[2];
+  const int num0 = (int)g.num[0], num1 = (int)g.num[1];
+  const int ext0 = (int)g.extent[0], ext1 = (int)g.extent[1];
+  const int o0 = axis_off<OFF>(g.code[0]), o1 = axis_off<OFF>(g.code[1]);
   for (int r = threadIdx.x; r < LX * LY; r += BLOCK) {
-    int gx = x0 - OFF + r / LY, gy = y0 - OFF + r % LY;
+    int gx = x0 - o0 + r / LY, gy = y0 - o1 + r % LY;
     bool ok = true;
     if (g.code[0] == 1) gx = gx < 0 ? gx + num0 : (gx >= num0 ? gx - num0 : gx);
-    else if (gx < 0 || gx >= num0) { ok = clamp; gx = gx < 0 ? 0 : num0 - 1; }
+    else if (gx < 0 || gx >= ext0) { ok = clamp; gx = gx < 0 ? 0 : ext0 - 1; }
     if (g.code[1] == 1) gy = gy < 0 ? gy + num1 : (gy >= num1 ? gy - num1 : gy);
-    else if (gy < 0 || gy >= num1) { ok = ok && clamp; gy = gy < 0 ? 0 : num1 - 1; }
-    rows[r] = ok ? ((long long)gx * num1 + gy) * num2 * fc : -1ll;
+    else if (gy < 0 || gy >= ext1) { ok = ok && clamp; gy = gy < 0 ? 0 : ext1 - 1; }
+    rows[r] = ok ? ((long long)gx * ext1 + gy) * g.extent[2] * fc : -1ll;
   }
 }
 
 // The z node of footprint position iz of a tile, wrapped on a periodic axis; -1 beyond a
-// bounded wall (clamp = false) or the clamped node (clamp = true).
+// bounded wall or a clamped axis's extent (clamp = false) or the clamped node (clamp = true).
 __device__ __forceinline__ int footprint_z(const Grid &g, int gz, bool clamp) {
-  const int num2 = (int)g.num[2];
+  const int num2 = (int)g.num[2], ext2 = (int)g.extent[2];
   if (g.code[2] == 1) return gz < 0 ? gz + num2 : (gz >= num2 ? gz - num2 : gz);
-  if (gz >= 0 && gz < num2) return gz;
-  return clamp ? (gz < 0 ? 0 : num2 - 1) : -1;
+  if (gz >= 0 && gz < ext2) return gz;
+  return clamp ? (gz < 0 ? 0 : ext2 - 1) : -1;
 }
 
 // A walk over a tile's footprint, NROWS rows of ROW words (a row: LZ nodes along z times C
@@ -1459,7 +1503,7 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 2 : 1) scatter_tiled(con
   using Walk = RowWalk<BLOCK, LZ * C, LX * LY>;
   const Walk walk(t);
   const int iz = walk.k / C, q = walk.k - iz * C;
-  const int gz = plain_z ? z0 - OFF + iz : footprint_z(g, z0 - OFF + iz, false);
+  const int gz = plain_z ? z0 - OFF + iz : footprint_z(g, z0 - axis_off<OFF>(g.code[2]) + iz, false);
   const long long zq = (long long)gz * fc + q;
   const bool distinct = (g.code[0] != 1 || num0 >= LX) && (g.code[1] != 1 || num1 >= LY) &&
                         (g.code[2] != 1 || num2 >= LZ);
@@ -1482,7 +1526,7 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 2 : 1) scatter_tiled(con
   const int stride[3] = {LY * LZ, LZ, 1};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    if (g.code[d] == 1) continue;
+    if (g.code[d] != 0) continue;  // periodic: wrapped; clamped: every stencil node exists
     const int num = (int)g.num[d];
     if (t0[d] - OFF >= 0 && t0[d] - OFF + ext[d] - 1 <= num - 1) continue;  // no wall in reach
     for (int i = t; i < L; i += BLOCK) {
@@ -1541,13 +1585,15 @@ __global__ void __launch_bounds__(BLOCK) gather_tiled(const T *__restrict__ pos,
   const int tz = id % ntz, ty = (id / ntz) % nty, tx = id / (ntz * nty);
   const int num0 = (int)g.num[0], num1 = (int)g.num[1], num2 = (int)g.num[2];
   const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
+  // a clamped axis's footprint starts at the tile and runs over num + P - 1 nodes
+  const int o0 = axis_off<OFF>(g.code[0]), o1 = axis_off<OFF>(g.code[1]), o2 = axis_off<OFF>(g.code[2]);
 #pragma unroll
   for (int i = threadIdx.x; i < L; i += BLOCK) {
     const int iz = i % LZ, iy = (i / LZ) % LY, ix = i / (LZ * LY);
-    const int64_t gx = wrap_index(x0 - OFF + ix, num0, g.code[0]);
-    const int64_t gy = wrap_index(y0 - OFF + iy, num1, g.code[1]);
-    const int64_t gz = wrap_index(z0 - OFF + iz, num2, g.code[2]);
-    const int64_t at = ((gx * num1 + gy) * num2 + gz) * fc;
+    const int64_t gx = wrap_index(x0 - o0 + ix, g.code[0] == 2 ? g.extent[0] : num0, g.code[0]);
+    const int64_t gy = wrap_index(y0 - o1 + iy, g.code[1] == 2 ? g.extent[1] : num1, g.code[1]);
+    const int64_t gz = wrap_index(z0 - o2 + iz, g.code[2] == 2 ? g.extent[2] : num2, g.code[2]);
+    const int64_t at = ((gx * g.extent[1] + gy) * g.extent[2] + gz) * fc;
 #pragma unroll
     for (int c = 0; c < C; ++c) tile[i * C + c] = field[at + c];
   }
@@ -1576,9 +1622,9 @@ __global__ void __launch_bounds__(BLOCK) gather_tiled(const T *__restrict__ pos,
       if (i >= total) continue;
       const int64_t m = tile_first + i;
       double wx[4], wy[4], wz[4];
-      int bx = anchor_axis<KIND, P>(((double)px[u] - g.lower[0]) * g.inverse[0], num0, g.code[0], wx);
-      int by = anchor_axis<KIND, P>(((double)py[u] - g.lower[1]) * g.inverse[1], num1, g.code[1], wy);
-      int bz = anchor_axis<KIND, P>(((double)pz[u] - g.lower[2]) * g.inverse[2], num2, g.code[2], wz);
+      int bx = tile_anchor<KIND, P>(((double)px[u] - g.lower[0]) * g.inverse[0], num0, g.code[0], wx);
+      int by = tile_anchor<KIND, P>(((double)py[u] - g.lower[1]) * g.inverse[1], num1, g.code[1], wy);
+      int bz = tile_anchor<KIND, P>(((double)pz[u] - g.lower[2]) * g.inverse[2], num2, g.code[2], wz);
       // the anchor cell within the tile (binning put it there; the clamp keeps LDS reads inside)
       bx = (g.code[0] == 1 && bx >= num0 ? bx - num0 : bx) - x0;
       by = (g.code[1] == 1 && by >= num1 ? by - num1 : by) - y0;
@@ -1682,6 +1728,14 @@ struct NodeSlot<1> {
 // is the bin's cell for any rel and the fraction is clamped.
 template <int KIND, int P>
 __device__ __forceinline__ void cell_weights_f32(double rel, double cell, double hi, int num, int code, float *w) {
+  if (KIND == SPLINE && code == 2) {  // clamped knots: the float64 Cox-de Boor weights, rounded
+    int64_t node[4];
+    double wd[4];
+    clamped_cell(rel, num, P, 0, node, wd);
+#pragma unroll
+    for (int a = 0; a <= P; ++a) w[a] = (float)wd[a];
+    return;
+  }
   const double o = rel - cell;
   if (!offset_in_cell<KIND, P>(rel, o, hi, code)) {
     weights_f32<KIND, P>(rel, num, code, w);
@@ -1813,7 +1867,8 @@ __global__ void __launch_bounds__(BLOCK) gather_lanes(const float *__restrict__ 
     using Walk = RowWalk<BLOCK, LZ * C, LX * LY>;
     const Walk walk(t);
     const int iz = walk.k / C, q = walk.k - iz * C;
-    const long long zq = (long long)(plain_z ? z0 - OFF + iz : footprint_z(g, z0 - OFF + iz, true)) * fc + q;
+    const long long zq =
+        (long long)(plain_z ? z0 - OFF + iz : footprint_z(g, z0 - axis_off<OFF>(g.code[2]) + iz, true)) * fc + q;
     double v[Walk::STEPS];
 #pragma unroll
     for (int u = 0; u < Walk::STEPS; ++u) v[u] = field[rows[walk.row_in(u)] + zq];  // unconditional loads
@@ -2300,10 +2355,11 @@ static int colors_for(int nt, int code, int64_t num, int tile, int span) {
 
 // Which (kind, degree, ncomp, dims, codes) the 3D tiled deposit and gather take; others use
 // the direct path.  Every degree and component count (groups of three components where a
-// kernel holds three, p <= 2; one at a time otherwise); bounded axes of at least a tile,
-// periodic axes of at least two tiles, whole or with a partial last tile, and at least a
-// footprint long (a tile's footprint never meets itself across the seam); colors_for keeps
-// same-colour footprints apart across the seam, a partial last tile included.
+// kernel holds three, p <= 2; one at a time otherwise); bounded axes and the splines'
+// clamped axes of at least a tile, periodic axes of at least two tiles, whole or with a
+// partial last tile, and at least a footprint long (a tile's footprint never meets itself
+// across the seam); colors_for keeps same-colour footprints apart across the seam, a
+// partial last tile included.
 static bool tiled_supported(int kind, int degree, int ndim, int64_t ncomp, const Grid &g, const TileShape &ts) {
   if (ndim != 3 || ncomp < 1 || ncomp > 64) return false;
   if (kind == SPLINE && (degree < 0 || degree > 3)) return false;
@@ -2311,7 +2367,7 @@ static bool tiled_supported(int kind, int degree, int ndim, int64_t ncomp, const
   const int span = kind == LINEAR ? 2 : p + 1;  // stencil nodes per axis
   const int t[3] = {ts.tx, ts.ty, ts.tz};
   for (int d = 0; d < 3; ++d) {
-    if (g.code[d] == 2) return false;
+    if (g.code[d] == 2 && kind != SPLINE) return false;  // clamped knots belong to the splines
     if (g.num[d] < t[d]) return false;
     if (g.code[d] == 1 && ((g.num[d] + t[d] - 1) / t[d] < 2 || g.num[d] < t[d] + span - 1)) return false;
   }
@@ -2327,7 +2383,7 @@ static bool derivative_tiled_supported(int kind, int degree, int ndim, int64_t n
   if (!((ncomp == 1 || ncomp == 3) && (p <= 2 || (p == 3 && ncomp == 1)))) return false;
   const int t[3] = {ts.tx, ts.ty, ts.tz};
   for (int d = 0; d < 3; ++d)
-    if (g.code[d] == 1 && g.num[d] % t[d] != 0) return false;
+    if (g.code[d] == 2 || (g.code[d] == 1 && g.num[d] % t[d] != 0)) return false;
   return true;
 }
 
@@ -2531,8 +2587,9 @@ extern "C" int mpf_bin_markers(int kind, int degree, int ndim, const void *posit
                      code);
   if (rc) return rc;
   for (int d = 0; d < ndim; ++d)
-    if (code[d] == 2) {
-      set_error("bin_markers: clamped axes are not binned (the direct kernels take them)");
+    if (code[d] == 2 && (ndim != 3 || kind != SPLINE)) {
+      set_error("bin_markers: clamped axes are binned in 3D for the spline kernels only (the direct kernels take "
+                "the others)");
       return MPF_ERR_UNSUPPORTED;
     }
   const int64_t cells = bins_of(ndim, num_nodes);  // bins before the dead one
@@ -2672,7 +2729,9 @@ extern "C" int mpf_rebin_markers(int kind, int degree, int ndim, const void *pos
     set_error("rebin_markers: the new bins need arrays of their own (they are read from the old ones)");
     return MPF_ERR_ARGUMENT;
   }
-  const bool tiles3d = ndim == 3 && n > 0;
+  // 3D bins move markers between neighbouring tiles; other bins, and those on clamped axes
+  // (cells of their own, clamped_axis_cell), are sorted again
+  const bool tiles3d = ndim == 3 && n > 0 && code && code[0] != 2 && code[1] != 2 && code[2] != 2;
   if (!tiles3d) {
     int rc = mpf_bin_markers(kind, degree, ndim, positions, pos_stride, values, val_stride, dtype, n, ncomp, lower,
                              cell_size, num_nodes, code, sorted_positions, sorted_values, order, start, scalars,
@@ -2696,11 +2755,6 @@ extern "C" int mpf_rebin_markers(int kind, int degree, int ndim, const void *pos
   int rc = make_grid(g, ndim, kind, kind == LINEAR ? 1 : (kind == QUADRATIC ? 2 : degree), lower, cell_size, num_nodes,
                      code);
   if (rc) return rc;
-  for (int d = 0; d < 3; ++d)
-    if (code[d] == 2) {
-      set_error("rebin_markers: clamped axes are not binned (the direct kernels take them)");
-      return MPF_ERR_UNSUPPORTED;
-    }
   TileGrid tg;
   tg.nt[0] = (int)((num_nodes[0] + TILE_X - 1) / TILE_X);
   tg.nt[1] = (int)((num_nodes[1] + TILE_Y - 1) / TILE_Y);

@@ -178,9 +178,9 @@ def tiled_ok(kind: int, degree: int, ncomp: int, num, code) -> bool:
     """Whether the tiled deposit and gather take this case (the rules of mpf_binned.hip's
     tiled_supported and tiled_2d_supported): 2D with 1-3 components on bounded or periodic
     axes; 3D with any degree and 1-64 components (groups of three where p <= 2), bounded axes
-    of at least a tile, periodic axes of at least two tiles and a footprint (partial last
-    tiles allowed)."""
-    if len(num) not in (2, 3) or any(int(c) == 2 for c in code):
+    and the splines' clamped axes of at least a tile, periodic axes of at least two tiles and
+    a footprint (partial last tiles allowed)."""
+    if len(num) not in (2, 3) or (any(int(c) == 2 for c in code) and (len(num) == 2 or kind != _lib.KIND_SPLINE)):
         return False
     if kind == _lib.KIND_SPLINE and not 0 <= degree <= 3:
         return False
@@ -205,7 +205,7 @@ def derivative_tiled_ok(kind: int, degree: int, ncomp: int, num, code) -> bool:
     p = {0: 1, 1: 2}.get(kind, degree)
     if not ((ncomp in (1, 3) and p <= 2) or (p == 3 and ncomp == 1)):
         return False
-    return all(int(c) != 1 or int(n) % t == 0 for n, c, t in zip(num, code, _TILE))
+    return all(int(c) == 0 or (int(c) == 1 and int(n) % t == 0) for n, c, t in zip(num, code, _TILE))
 
 
 def scatter(kind: int, degree: int, positions, values, field: torch.Tensor, geometry, *, mode: int | None = None,

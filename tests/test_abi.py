@@ -138,8 +138,12 @@ def test_binned_route_stops_below_two_to_the_31_markers():
     assert not K.binned_route(1, 2, 3, num, code, 1 << 31)
     assert not K.binned_route(1, 2, 3, num, code, K.BINNED_MIN_MARKERS - 1)
     assert K.binned_route(1, 2, 3, num, code, K.BINNED_MIN_MARKERS)
-    # clamped axes never bin; any degree and component count does on tiles
-    assert not K.binned_route(2, 3, 3, num, (2, 1, 1), 1 << 20)
+    # clamped axes bin for the 3D splines only; any degree and component count does on tiles
+    assert K.binned_route(2, 3, 3, num, (2, 1, 1), 1 << 20)
+    assert not K.binned_route(1, 2, 3, num, (2, 1, 1), 1 << 20)
+    assert not K.binned_route(2, 2, 1, (256, 256), (2, 1), 1 << 20)
+    assert not K.binned_route(2, 2, 1, (7, 64, 64), (2, 1, 1), 1 << 20)  # a clamped axis shorter than a tile
+    assert not K.derivative_tiled_ok(2, 2, 1, num, (2, 1, 1))
     assert K.binned_route(2, 3, 3, num, code, 1 << 20)
     assert K.binned_route(1, 2, 5, (100, 36, 52), (1, 1, 1), 1 << 20)  # partial periodic tiles
     # a periodic axis needs two tiles and a footprint: 8 x 8 x 16 tiles, quadratic stencils
