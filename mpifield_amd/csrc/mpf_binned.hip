@@ -36,6 +36,16 @@
 
 namespace mpf {
 
+// A uniform value held in scalar registers from this point on: the wait for its scalar load
+// is placed here, once, instead of at its first use inside a loop, where it would be taken
+// on every trip and, since scalar loads and LDS operations share one counter, would also
+// wait for every LDS operation the loop had issued.
+template <class T>
+__device__ __forceinline__ T uniform_here(T v) {
+  asm volatile("" : "+s"(v));
+  return v;
+}
+
 // ---- per-axis stencil cell and weights -----------------------------------------------
 
 // The cell of rel on a clamped axis (clamped_cell's: the position clamped to [0, num - 1],
@@ -154,6 +164,9 @@ __device__ __forceinline__ bool offset_in_cell(double rel, double o, double hi, 
 // rel - cell where offset_in_cell holds (the same float64 operations, so the same bits),
 // from the full selection elsewhere.
 template <int KIND, int P>
+__device__ __forceinline__ void offset_weights_f64(double o, double *w);
+
+template <int KIND, int P>
 __device__ __forceinline__ void cell_weights_f64(double rel, double cell, double hi, int num, int code, double *w) {
   if (KIND == SPLINE && code == 2) {  // clamped knots: the direct kernels' Cox-de Boor weights
     int64_t node[4];
@@ -165,6 +178,26 @@ __device__ __forceinline__ void cell_weights_f64(double rel, double cell, double
     anchor_axis<KIND, P>(rel, num, code, w);
     return;
   }
+  offset_weights_f64<KIND, P>(o, w);
+}
+
+// Whether a marker binned into stencil cell `cell` takes offset_weights_f64 on every axis
+// (no clamped axis, offset_in_cell on each): the test of cell_weights_f64 for all three
+// axes at once, so that a caller computes the offset weights unconditionally and takes the
+// full selection under one branch that the markers of a binned batch almost never enter.
+template <int KIND, int P>
+__device__ __forceinline__ bool offsets_in_cell(const double (&rel)[3], const double (&o)[3], const AxisRange &ar,
+                                                const Grid &g) {
+  bool ok = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    ok &= !(KIND == SPLINE && g.code[d] == 2) & offset_in_cell<KIND, P>(rel[d], o[d], ar.hi[d], g.code[d]);
+  return ok;
+}
+
+// The weights of cell_weights_f64 where offset_in_cell holds: a polynomial in the offset.
+template <int KIND, int P>
+__device__ __forceinline__ void offset_weights_f64(double o, double *w) {
   if (KIND == LINEAR) {
     const double f = o < 0.0 ? 0.0 : (o > 1.0 ? 1.0 : o);
     w[0] = 1.0 - f;
@@ -1285,16 +1318,25 @@ __device__ __forceinline__ void deposit_markers(unsigned long long *acc, int bas
   double wx[ITEM][S], wyz[ITEM][S][S], sc[ITEM];
 #pragma unroll
   for (int r = 0; r < ITEM; ++r) {
-    double w0[4], w1[4], w2[4];
-    cell_weights_f64<KIND, P>(((double)mk.x[r] - g.lower[0]) * g.inverse[0], cell[0], ar.hi[0], num0, g.code[0], w0);
-    cell_weights_f64<KIND, P>(((double)mk.y[r] - g.lower[1]) * g.inverse[1], cell[1], ar.hi[1], num1, g.code[1], w1);
-    cell_weights_f64<KIND, P>(((double)mk.z[r] - g.lower[2]) * g.inverse[2], cell[2], ar.hi[2], num2, g.code[2], w2);
+    // the slot's weights from its offsets in the cell, then the full selection under one
+    // branch if any axis needs it (a wrap, a clamp or a clamped axis)
+    const double rel[3] = {((double)mk.x[r] - g.lower[0]) * g.inverse[0], ((double)mk.y[r] - g.lower[1]) * g.inverse[1],
+                           ((double)mk.z[r] - g.lower[2]) * g.inverse[2]};
+    const double o[3] = {rel[0] - cell[0], rel[1] - cell[1], rel[2] - cell[2]};
+    double w[3][4];
 #pragma unroll
-    for (int i = 0; i < S; ++i) wx[r][i] = w0[i];
+    for (int d = 0; d < 3; ++d) offset_weights_f64<KIND, P>(o[d], w[d]);
+    if (!offsets_in_cell<KIND, P>(rel, o, ar, g)) {
+      cell_weights_f64<KIND, P>(rel[0], cell[0], ar.hi[0], num0, g.code[0], w[0]);
+      cell_weights_f64<KIND, P>(rel[1], cell[1], ar.hi[1], num1, g.code[1], w[1]);
+      cell_weights_f64<KIND, P>(rel[2], cell[2], ar.hi[2], num2, g.code[2], w[2]);
+    }
+#pragma unroll
+    for (int i = 0; i < S; ++i) wx[r][i] = w[0][i];
 #pragma unroll
     for (int j = 0; j < S; ++j)
 #pragma unroll
-      for (int k = 0; k < S; ++k) wyz[r][j][k] = w1[j] * w2[k];
+      for (int k = 0; k < S; ++k) wyz[r][j][k] = w[1][j] * w[2][k];
     sc[r] = (r == 0 || (uint32_t)r < count) ? scale : 0.0;
   }
   // LDS reads the caller wants before this deposit's atomics are queued (they complete in
@@ -1427,10 +1469,17 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 2 : 1) scatter_tiled(con
   const uint32_t total =
       (ZT == 1 && tables) ? table_counts[tile_id] : build_items<BLOCK, TC>(cfirst, items, wave_sums);
 
-  const double scale = slot[0];
+  const double scale = uniform_here(slot[0]);
   // markers per register pass (the exact sums stay below 2^51 quanta): below ITEM only for a
   // scale fixed by the caller, when every marker gets its own pass
   const bool one = slot[2] < (double)ITEM;
+  // the grid's origin and inverse spacing in scalar registers before the item loop
+  Grid gr = g;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    gr.lower[d] = uniform_here(g.lower[d]);
+    gr.inverse[d] = uniform_here(g.inverse[d]);
+  }
 
   // The lookahead: the markers of the next item load while the current one is deposited.
   // The loads are unconditional (past the last item they repeat it) and the first ones are
@@ -1466,10 +1515,10 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 2 : 1) scatter_tiled(con
     const uint32_t first_count = cur.size < (uint32_t)ITEM ? cur.size : ITEM;
     auto read_next = [&]() { nn = item_at(items, cfirst, after < total ? after : last); };
     if (one)
-      deposit_markers<KIND, P, C, S, L, LY, LZ, true>(acc, base, now, first_count, cell, ar, g, num0, num1, num2,
+      deposit_markers<KIND, P, C, S, L, LY, LZ, true>(acc, base, now, first_count, cell, ar, gr, num0, num1, num2,
                                                       scale, read_next);
     else
-      deposit_markers<KIND, P, C, S, L, LY, LZ, false>(acc, base, now, first_count, cell, ar, g, num0, num1, num2,
+      deposit_markers<KIND, P, C, S, L, LY, LZ, false>(acc, base, now, first_count, cell, ar, gr, num0, num1, num2,
                                                        scale, read_next);
     // items of more than ITEM markers (cells of more than ITEM * SWEEPS markers), ITEM at a time
     for (uint32_t r0 = ITEM; r0 < cur.size; r0 += ITEM) {
@@ -1477,10 +1526,10 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 2 : 1) scatter_tiled(con
       load_ahead<SMALL>(mrows, cur.begin + r0, cur.size - r0, more);
       const uint32_t k = cur.size - r0 < (uint32_t)ITEM ? cur.size - r0 : ITEM;
       if (one)
-        deposit_markers<KIND, P, C, S, L, LY, LZ, true>(acc, base, more, k, cell, ar, g, num0, num1, num2, scale,
+        deposit_markers<KIND, P, C, S, L, LY, LZ, true>(acc, base, more, k, cell, ar, gr, num0, num1, num2, scale,
                                                         []() {});
       else
-        deposit_markers<KIND, P, C, S, L, LY, LZ, false>(acc, base, more, k, cell, ar, g, num0, num1, num2, scale,
+        deposit_markers<KIND, P, C, S, L, LY, LZ, false>(acc, base, more, k, cell, ar, gr, num0, num1, num2, scale,
                                                          []() {});
     }
     cur = nxt;
@@ -1727,6 +1776,9 @@ struct NodeSlot<1> {
 // take the full selection of weights_f32.  Linear bounded axes need no test: the left node
 // is the bin's cell for any rel and the fraction is clamped.
 template <int KIND, int P>
+__device__ __forceinline__ void offset_weights_f32(double o, float *w);
+
+template <int KIND, int P>
 __device__ __forceinline__ void cell_weights_f32(double rel, double cell, double hi, int num, int code, float *w) {
   if (KIND == SPLINE && code == 2) {  // clamped knots: the float64 Cox-de Boor weights, rounded
     int64_t node[4];
@@ -1741,6 +1793,13 @@ __device__ __forceinline__ void cell_weights_f32(double rel, double cell, double
     weights_f32<KIND, P>(rel, num, code, w);
     return;
   }
+  offset_weights_f32<KIND, P>(o, w);
+}
+
+// The weights of cell_weights_f32 where offset_in_cell holds: float32 polynomials in the
+// offset rounded to float32.
+template <int KIND, int P>
+__device__ __forceinline__ void offset_weights_f32(double o, float *w) {
   if (KIND == LINEAR) {
     const float f = (float)(o < 0.0 ? 0.0 : (o > 1.0 ? 1.0 : o));
     w[0] = 1.0f - f;
@@ -1773,10 +1832,20 @@ template <int KIND, int P, int C, int S>
 __device__ __forceinline__ void interpolate_marker(const typename NodeSlot<C>::type (&nd)[S][S][S], float x, float y,
                                                    float z, const double (&cell)[3], const AxisRange &ar,
                                                    const Grid &g, int num0, int num1, int num2, float *out) {
+  // the weights from the offsets in the cell, then the full selection under one branch if
+  // any axis needs it (a wrap, a clamp or a clamped axis)
   float wx[4], wy[4], wz[4];
-  cell_weights_f32<KIND, P>(((double)x - g.lower[0]) * g.inverse[0], cell[0], ar.hi[0], num0, g.code[0], wx);
-  cell_weights_f32<KIND, P>(((double)y - g.lower[1]) * g.inverse[1], cell[1], ar.hi[1], num1, g.code[1], wy);
-  cell_weights_f32<KIND, P>(((double)z - g.lower[2]) * g.inverse[2], cell[2], ar.hi[2], num2, g.code[2], wz);
+  const double rel[3] = {((double)x - g.lower[0]) * g.inverse[0], ((double)y - g.lower[1]) * g.inverse[1],
+                         ((double)z - g.lower[2]) * g.inverse[2]};
+  const double o[3] = {rel[0] - cell[0], rel[1] - cell[1], rel[2] - cell[2]};
+  offset_weights_f32<KIND, P>(o[0], wx);
+  offset_weights_f32<KIND, P>(o[1], wy);
+  offset_weights_f32<KIND, P>(o[2], wz);
+  if (!offsets_in_cell<KIND, P>(rel, o, ar, g)) {
+    cell_weights_f32<KIND, P>(rel[0], cell[0], ar.hi[0], num0, g.code[0], wx);
+    cell_weights_f32<KIND, P>(rel[1], cell[1], ar.hi[1], num1, g.code[1], wy);
+    cell_weights_f32<KIND, P>(rel[2], cell[2], ar.hi[2], num2, g.code[2], wz);
+  }
   float wyz[S][S];
 #pragma unroll
   for (int b = 0; b < S; ++b)
@@ -1819,7 +1888,7 @@ __device__ __forceinline__ void interpolate_marker(const typename NodeSlot<C>::t
 // Tiles are numbered so that each XCD (workgroups go round-robin over the eight) takes a
 // contiguous range of them.
 template <int KIND, int P, int C, int BLOCK, bool SMALL, int ZT = 1>
-__global__ void __launch_bounds__(BLOCK) gather_lanes(const float *__restrict__ pos, int64_t n,
+__global__ void __launch_bounds__(BLOCK, 1024 / BLOCK) gather_lanes(const float *__restrict__ pos, int64_t n,
                                                       const uint32_t *__restrict__ start, Grid g, int ntx, int nty,
                                                       int ntz, const double *__restrict__ field, int fc,
                                                       float *__restrict__ res) {
@@ -1909,22 +1978,27 @@ __global__ void __launch_bounds__(BLOCK) gather_lanes(const float *__restrict__ 
     const uint32_t mlast = hi - 1;
     uint32_t m = lo + t;
     float ax, ay, az;
+    int ac;
     {
-      const int64_t gm = (int64_t)first + (m < hi ? m : mlast);
+      const uint32_t mm = m < hi ? m : mlast;
+      const int64_t gm = (int64_t)first + mm;
       ax = *row_at<SMALL>(px, gm);
       ay = *row_at<SMALL>(py, gm);
       az = *row_at<SMALL>(pz, gm);
+      ac = mcell[mm - lo];
     }
     for (; m < hi; m += BLOCK) {
       const float x = ax, y = ay, z = az;
+      const int c = ac, sub = c / TILE_CELLS, loc = c % TILE_CELLS;
       {
-        const uint32_t mn = m + BLOCK;
-        const int64_t gm = (int64_t)first + (mn < hi ? mn : mlast);
+        // the next marker's position and cell, read while this one is interpolated
+        const uint32_t mn = m + BLOCK < hi ? m + BLOCK : mlast;
+        const int64_t gm = (int64_t)first + mn;
         ax = *row_at<SMALL>(px, gm);
         ay = *row_at<SMALL>(py, gm);
         az = *row_at<SMALL>(pz, gm);
+        ac = mcell[mn - lo];
       }
-      const int c = mcell[m - lo], sub = c / TILE_CELLS, loc = c % TILE_CELLS;
       const int cx = loc / (TY * TILE_Z), cy = (loc / TILE_Z) % TY, cz = sub * TILE_Z + loc % TILE_Z;
       const int base = (cx * LY + cy) * LZ + cz;
       const double cell[3] = {(double)(x0 + cx), (double)(y0 + cy), (double)(z0 + cz)};
