@@ -421,10 +421,32 @@ int mpf_fixed_scale(const void *values, int64_t val_stride, int dtype, int64_t n
 
 /* Rows (rows, n) of binned markers back to the input order: out[c * n + order[m]] =
  * sorted[c * n + m] with order the permutation mpf_bin_markers wrote; both arrays
- * contiguous rows of dtype.  No counterpart in the reference (its markers are never
- * reordered). */
+ * contiguous rows of dtype.  From 2^20 markers through a plan made in the stream's default
+ * context (mpf_perm_rows), below with scattered writes.  No counterpart in the reference
+ * (its markers are never reordered). */
 int mpf_unsort_rows(const void *sorted, void *out, const int64_t *order, int64_t rows, int64_t n, int dtype,
                     mpf_stream stream);
+
+/* Marker rows between the input order and the bins' order through a plan of the
+ * permutation, in two passes that read and write whole blocks (chunks of 2^15 bins-order
+ * indices, buckets of 2^15 input indices, staged through LDS) instead of one pass of
+ * scattered accesses.  No counterpart in the reference (its
+ * markers are never reordered); for bins kept across steps, whose values arrive and whose
+ * results leave in the input order (Field.scatter / Field.gather with bins,
+ * Weighting.scatter_block / gather_block with bins).
+ *
+ * mpf_perm_plan_bytes: the size of the plan of n markers (0 outside 1 .. 2^28).
+ * mpf_perm_plan_build: the plan of order (mpf_bin_markers' permutation: bins-order index ->
+ *   input index) into device memory of that size (scratch from ctx).
+ * mpf_perm_rows: rows of n markers, src rows src_stride elements apart, dst rows dst_stride
+ *   apart; direction 0: dst[c * dst_stride + order[m]] = src[c * src_stride + m] (bins order
+ *   to input order, as mpf_unsort_rows); direction 1: dst[c * dst_stride + m] =
+ *   src[c * src_stride + order[m]] (input order to bins order).  dtype MPF_F32 or MPF_F64;
+ *   the staging rows live in ctx. */
+int mpf_perm_plan_bytes(int64_t n, int64_t *bytes);
+int mpf_perm_plan_build(const int64_t *order, int64_t n, void *plan, mpf_context *ctx, mpf_stream stream);
+int mpf_perm_rows(const void *plan, int64_t n, int direction, const void *src, int64_t src_stride, void *dst,
+                  int64_t dst_stride, int64_t rows, int dtype, mpf_context *ctx, mpf_stream stream);
 
 /* ---- many patches in one launch ---------------------------------------------------- */
 

@@ -81,6 +81,9 @@ SIGNATURES: dict[str, list] = {
                                      _P, _P, _P, _P],
     "mpf_fixed_scale": [_P, _I64, _INT, _I64, _I64, _INT, _P, _P, _P],
     "mpf_unsort_rows": [_P, _P, _P, _I64, _I64, _INT, _P],
+    "mpf_perm_plan_bytes": [_I64, _P],
+    "mpf_perm_plan_build": [_P, _I64, _P, _P, _P],
+    "mpf_perm_rows": [_P, _I64, _INT, _P, _I64, _P, _I64, _I64, _INT, _P, _P],
     "mpf_gather_edges": [_INT, _INT, _INT, _P, _I64, _INT, _I64, _P, _P, _I64, _P, _P, _P, _P, _P, _P],
     "mpf_scatter_forms": [_INT, _INT, _INT, _P, _I64, _P, _I64, _INT, _I64, _I64, _P, _P, _P, _P, _P, _INT, _INT, _P,
                           _P],

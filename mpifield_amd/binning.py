@@ -28,6 +28,7 @@ edges of degree p bins of the spline nodes of p::
 
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 
 import numpy as np
@@ -35,6 +36,10 @@ import torch
 
 from mpifield_amd import _lib
 from mpifield_amd import kernels as K
+
+#: bins of this many markers or more move rows between the input order and the bins' order
+#: through a permutation plan (two passes of whole blocks) instead of scattered accesses
+PLAN_MIN = 1 << 18
 
 #: (kind, degree) of the built-in node weightings, by name and degree
 _KINDS = {("linear", 1): (0, 1), ("quadratic", 2): (1, 2), ("cubic", 3): (2, 3)}
@@ -66,6 +71,7 @@ class MarkerBins:
     scalars: torch.Tensor  # (3,) float64: [2^k, 2^-k, markers per register pass]
     tables: torch.Tensor | None = None  # (tiles, 4096) int16: the 3D tiles' work lists (prepare_items)
     table_counts: torch.Tensor | None = None  # (tiles,) int32
+    plan: torch.Tensor | None = None  # uint8: the permutation plan (permutation_plan), made on first use
 
     @property
     def n(self) -> int:
@@ -132,8 +138,26 @@ class MarkerBins:
                       K.stream_handle(dev))
         return MarkerBins(self.kind, self.degree, self.geometry, spos, sval, order, start, scalars)
 
+    def permutation_plan(self) -> torch.Tensor | None:
+        """The plan of this binning's permutation (mpf_perm_plan_build), made on first use and
+        kept with the bins, so that rows move between the input order and the bins' order in
+        two passes of whole blocks; None below PLAN_MIN markers (scattered accesses cost less
+        there) and above 2^28."""
+        if self.plan is None and self.n >= PLAN_MIN:
+            size = ctypes.c_int64(0)
+            _lib.call("mpf_perm_plan_bytes", self.n, ctypes.addressof(size))
+            if size.value:
+                dev = self.order.device
+                plan = torch.empty(size.value, dtype=torch.uint8, device=dev)
+                with torch.cuda.device(dev):
+                    _lib.call("mpf_perm_plan_build", self.order.data_ptr(), self.n, plan.data_ptr(), None,
+                              K.stream_handle(dev))
+                self.plan = plan
+        return self.plan
+
     def unsort(self, sorted_rows: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
-        """Rows (k, N) in bins order back to the input order (mpf_unsort_rows)."""
+        """Rows (k, N) in bins order back to the input order (mpf_perm_rows through the bins'
+        plan, or mpf_unsort_rows)."""
         if out is None:
             out = torch.empty_like(sorted_rows)
         ok = (sorted_rows.dtype in (torch.float32, torch.float64) and out.dtype == sorted_rows.dtype
@@ -142,9 +166,30 @@ class MarkerBins:
         if not ok:  # other dtypes or layouts: torch's own scatter
             out.index_copy_(1, self.order, sorted_rows)
             return out
+        plan = self.permutation_plan()
         with torch.cuda.device(out.device):
-            _lib.call("mpf_unsort_rows", sorted_rows.data_ptr(), out.data_ptr(), self.order.data_ptr(),
-                      sorted_rows.shape[0], self.n, K._dtype_code(sorted_rows), K.stream_handle(out.device))
+            if plan is not None:
+                _lib.call("mpf_perm_rows", plan.data_ptr(), self.n, 0, sorted_rows.data_ptr(), self.n, out.data_ptr(),
+                          self.n, sorted_rows.shape[0], K._dtype_code(sorted_rows), None, K.stream_handle(out.device))
+            else:
+                _lib.call("mpf_unsort_rows", sorted_rows.data_ptr(), out.data_ptr(), self.order.data_ptr(),
+                          sorted_rows.shape[0], self.n, K._dtype_code(sorted_rows), K.stream_handle(out.device))
+        return out
+
+    def to_bins_order(self, rows: torch.Tensor) -> torch.Tensor:
+        """Rows (k, N) of these markers in the input order, in the bins' order (contiguous):
+        mpf_perm_rows through the bins' plan, or torch.index_select."""
+        plan = (self.permutation_plan() if rows.dtype in (torch.float32, torch.float64) and rows.dim() == 2
+                and rows.shape[1] == self.n and rows.is_cuda else None)
+        if plan is None:
+            return torch.index_select(rows, 1, self.order)
+        if rows.shape[0] > 1 and (rows.stride(1) != 1 or rows.stride(0) < self.n):
+            rows = rows.contiguous()
+        stride = rows.stride(0) if rows.shape[0] > 1 else self.n
+        out = torch.empty((rows.shape[0], self.n), dtype=rows.dtype, device=rows.device)
+        with torch.cuda.device(rows.device):
+            _lib.call("mpf_perm_rows", plan.data_ptr(), self.n, 1, rows.data_ptr(), stride, out.data_ptr(), self.n,
+                      rows.shape[0], K._dtype_code(rows), None, K.stream_handle(rows.device))
         return out
 
 
@@ -251,13 +296,13 @@ def with_values(bins: MarkerBins, values, scale_exp: int = _lib.SCALE_AUTO) -> M
     val, vs = K.marker_rows(values, None, bins.positions.dtype)
     if val.shape[1] != bins.n:
         raise ValueError(f"{bins.n} binned markers for {val.shape[1]} values")
-    sval = torch.index_select(val, 1, bins.order)
+    sval = bins.to_bins_order(val)
     scalars = torch.empty(3, dtype=torch.float64, device=val.device)
     with torch.cuda.device(val.device):
         _lib.call("mpf_fixed_scale", val.data_ptr(), vs, K._dtype_code(val), bins.n, val.shape[0], int(scale_exp),
                   scalars.data_ptr(), None, K.stream_handle(val.device))
     return MarkerBins(bins.kind, bins.degree, bins.geometry, bins.positions, sval, bins.order, bins.start, scalars,
-                      bins.tables, bins.table_counts)
+                      bins.tables, bins.table_counts, bins.plan)
 
 
 def _check_result(bins: MarkerBins, result, ncomp: int) -> None:

@@ -336,7 +336,8 @@ void mpf_context_destroy(mpf_context *ctx) {
   if (!ctx) return;
   (void)hipDeviceSynchronize();
   mpf::Buffer *all[] = {&ctx->accum,   &ctx->scalars, &ctx->sort_keys, &ctx->sort_tmp,    &ctx->offsets,
-                        &ctx->staging, &ctx->path_in, &ctx->bins,      &ctx->path_sorted, &ctx->rebin};
+                        &ctx->staging, &ctx->path_in, &ctx->bins,      &ctx->path_sorted, &ctx->rebin,
+                        &ctx->perm_plan, &ctx->perm_build, &ctx->perm_stage};
   for (mpf::Buffer *b : all)
     if (b->ptr) (void)hipFree(b->ptr);
   delete ctx;

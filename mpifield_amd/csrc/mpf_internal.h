@@ -36,6 +36,9 @@ struct mpf_context {
   mpf::Buffer bins;      // node deposits and gathers binned inside the call: sorted rows, order,
                          // bin offsets, scale, results in bins order
   mpf::Buffer rebin;     // re-binning: markers leaving their tile, per-tile counts and starts
+  mpf::Buffer perm_plan;   // the permutation plan of bins made inside a call
+  mpf::Buffer perm_build;  // a plan's counts per (chunk, bucket) and their scan down each bucket
+  mpf::Buffer perm_stage;  // rows in the staging order between a permutation's two passes
 };
 
 namespace mpf {
@@ -75,6 +78,19 @@ int gather_via_bins(int kind, int degree, int ndim, const void *positions, int64
                     const double *field, int64_t ncomp, void *result, int64_t result_stride, const double *lower,
                     const double *cell_size, const int64_t *num_nodes, const int64_t *code, mpf_context *ctx,
                     hipStream_t stream);
+
+// Marker rows between the input order and the bins' order in two passes of whole blocks
+// (mpf_permute.hip).  A plan of the permutation `order` (bins-order index -> input index)
+// for 1 to 2^28 markers (perm_plan_ok), built into caller memory with the context's
+// scratch; direction 0 moves rows from the bins' order to the input order, 1 back.
+constexpr int64_t PERM_MIN = 1 << 20;  // the in-call unsort plans from this many markers on
+bool perm_plan_ok(int64_t n);
+int perm_plan_build(const int64_t *order, int64_t n, void *plan, mpf_context *ctx, hipStream_t stream);
+int perm_apply(const void *plan, int64_t n, int direction, const void *src, int64_t src_stride, void *dst,
+               int64_t dst_stride, int64_t rows, int dtype, mpf_context *ctx, hipStream_t stream);
+// The unsort of rows through a plan made in ctx: out[c * out_stride + order[m]] = sorted[c * n + m].
+int unsort_planned(const void *sorted, void *out, int64_t out_stride, const int64_t *order, int64_t rows, int64_t n,
+                   int dtype, mpf_context *ctx, hipStream_t stream);
 
 inline int blocks_for(int64_t n, int threads, int64_t cap = 1 << 20) {
   int64_t b = (n + threads - 1) / threads;

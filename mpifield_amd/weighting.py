@@ -153,7 +153,7 @@ class Weighting:
                 if K.current_mode() == "fixed":
                     binning.scatter_binned(binning.with_values(positions, val), buffer)
                 else:  # float64 atomics: the direct kernels on the markers in bins order
-                    sval = torch.index_select(val, 1, positions.order)
+                    sval = positions.to_bins_order(val)
                     self.scatter_kernel(len(grid.periodic))(positions.positions, sval, buffer, *grid.geometry)
             return
         kernel = self.scatter_kernel(len(grid.periodic))

@@ -154,3 +154,36 @@ def test_binned_route_stops_below_two_to_the_31_markers():
     assert not K.derivative_tiled_ok(2, 3, 3, num, code)
     assert not K.derivative_tiled_ok(1, 2, 3, (100, 36, 52), (1, 1, 1))
     assert K.derivative_tiled_ok(1, 2, 3, num, code)
+
+
+def test_permutation_plan_sizes_and_argument_errors_need_no_gpu():
+    """mpf_perm_plan_bytes: q (2 B), dst (4 B per slot of whole 2^15 chunks) and lo (2 B) per
+    marker, each part 256-byte aligned; no plan outside 1 .. 2^28 markers; mpf_perm_rows
+    refuses a bad direction, dtype or stride before any device work."""
+    from mpifield_amd import _lib
+
+    lib = _lib.load()
+
+    def size(n):
+        out = ctypes.c_int64(-1)
+        assert lib.mpf_perm_plan_bytes(n, ctypes.addressof(out)) == 0
+        return out.value
+
+    def a256(b):
+        return (b + 255) // 256 * 256
+
+    for n in (1, 1000, 1 << 15, (1 << 15) + 1, 3_000_000, 1 << 28):
+        chunks = -(-n // (1 << 15))
+        assert size(n) == 2 * a256(2 * n) + a256(4 * chunks * (1 << 15)), n
+    assert size(0) == 0 and size((1 << 28) + 1) == 0 and size(-5) == 0
+    assert lib.mpf_perm_plan_bytes(10, None) == 1
+    rc = lib.mpf_perm_rows(None, 10, 2, None, 10, None, 10, 1, _lib.F32, None, None)
+    assert rc == 1 and b"direction" in lib.mpf_last_error()
+    rc = lib.mpf_perm_rows(None, 10, 0, None, 10, None, 10, 1, 7, None, None)
+    assert rc == 1 and b"dtype" in lib.mpf_last_error()
+    rc = lib.mpf_perm_rows(None, 10, 0, None, 9, None, 10, 1, _lib.F32, None, None)
+    assert rc == 1 and b"strides" in lib.mpf_last_error()
+    rc = lib.mpf_perm_rows(None, (1 << 28) + 1, 0, None, 1 << 29, None, 1 << 29, 1, _lib.F32, None, None)
+    assert rc == 1 and b"2^28" in lib.mpf_last_error()
+    # nothing to move is not an error
+    assert lib.mpf_perm_rows(None, 0, 0, None, 0, None, 0, 3, _lib.F32, None, None) == 0
