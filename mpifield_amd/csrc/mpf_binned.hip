@@ -1115,6 +1115,14 @@ __device__ uint32_t build_items(const uint32_t *cfirst, uint16_t *items, unsigne
   return base;
 }
 
+// A loaded value held at this point: its load completes here, and no load before this point
+// is moved past it (the compiler otherwise sinks a load into the guarded store that uses it,
+// so that a tile's start waits on one load after another).
+template <class V>
+__device__ __forceinline__ void loaded_here(const V &v) {
+  asm volatile("" ::"v"(v) : "memory");
+}
+
 // The tile's TC + 1 cell offsets into LDS, every load issued before the first store.
 template <int BLOCK, int TC>
 __device__ __forceinline__ void load_offsets(const uint32_t *__restrict__ start, uint32_t *cfirst) {
@@ -1123,8 +1131,10 @@ __device__ __forceinline__ void load_offsets(const uint32_t *__restrict__ start,
 #pragma unroll
   for (int u = 0; u < E; ++u) {
     const int i = threadIdx.x + u * BLOCK;
-    if (i <= TC) v[u] = start[i];
+    v[u] = start[i <= TC ? i : TC];
   }
+#pragma unroll
+  for (int u = 0; u < E; ++u) loaded_here(v[u]);
 #pragma unroll
   for (int u = 0; u < E; ++u) {
     const int i = threadIdx.x + u * BLOCK;
@@ -1143,6 +1153,8 @@ __device__ __forceinline__ void load_offsets_valid(const uint32_t *__restrict__ 
     const int i = threadIdx.x + u * BLOCK;
     v[u] = start[i <= valid ? i : valid];
   }
+#pragma unroll
+  for (int u = 0; u < E; ++u) loaded_here(v[u]);
 #pragma unroll
   for (int u = 0; u < E; ++u) {
     const int i = threadIdx.x + u * BLOCK;
@@ -1169,6 +1181,15 @@ __device__ __forceinline__ void load_offsets_and_items(const uint32_t *__restric
   for (int u = 0; u < Q; ++u) {
     const int i = threadIdx.x + u * BLOCK;
     q[u] = reinterpret_cast<const uint4 *>(table)[i < Q16 ? i : Q16 - 1];
+  }
+#pragma unroll
+  for (int u = 0; u < E; ++u) loaded_here(v[u]);
+#pragma unroll
+  for (int u = 0; u < Q; ++u) {
+    loaded_here(q[u].x);
+    loaded_here(q[u].y);
+    loaded_here(q[u].z);
+    loaded_here(q[u].w);
   }
 #pragma unroll
   for (int u = 0; u < E; ++u) {
@@ -1459,6 +1480,8 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 2 : 1) scatter_tiled(con
   const int zt_valid = tl.ntz_bins - tz * ZT < ZT ? tl.ntz_bins - tz * ZT : ZT;
 
   const int64_t tile_id = tile_key / TILE_CELLS;
+  // the tile's item count loads with its offsets and items (one memory latency for all)
+  const uint32_t tabled_total = (ZT == 1 && tables) ? table_counts[tile_id] : 0u;
   if (ZT == 1 && tables)
     load_offsets_and_items<BLOCK, TC>(start + tile_key, tables + tile_id * SWEEPS * TC, cfirst, items);
   else
@@ -1467,7 +1490,7 @@ __global__ void __launch_bounds__(BLOCK, BLOCK == 256 ? 2 : 1) scatter_tiled(con
   footprint_rows<LX, LY, OFF, BLOCK>(g, x0, y0, z0, false, fc, rows);
   __syncthreads();
   const uint32_t total =
-      (ZT == 1 && tables) ? table_counts[tile_id] : build_items<BLOCK, TC>(cfirst, items, wave_sums);
+      (ZT == 1 && tables) ? tabled_total : build_items<BLOCK, TC>(cfirst, items, wave_sums);
 
   const double scale = uniform_here(slot[0]);
   // markers per register pass (the exact sums stay below 2^51 quanta): below ITEM only for a
