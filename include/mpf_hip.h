@@ -421,8 +421,8 @@ int mpf_fixed_scale(const void *values, int64_t val_stride, int dtype, int64_t n
 
 /* Rows (rows, n) of binned markers back to the input order: out[c * n + order[m]] =
  * sorted[c * n + m] with order the permutation mpf_bin_markers wrote; both arrays
- * contiguous rows of dtype.  From 2^20 markers through a plan made in the stream's default
- * context (mpf_perm_rows), below with scattered writes.  No counterpart in the reference
+ * contiguous rows of dtype.  Two rows or more of 2^20 markers or more through a plan made in
+ * the stream's default context (mpf_perm_rows), others with scattered writes.  No counterpart in the reference
  * (its markers are never reordered). */
 int mpf_unsort_rows(const void *sorted, void *out, const int64_t *order, int64_t rows, int64_t n, int dtype,
                     mpf_stream stream);

@@ -71,7 +71,8 @@ class MarkerBins:
     scalars: torch.Tensor  # (3,) float64: [2^k, 2^-k, markers per register pass]
     tables: torch.Tensor | None = None  # (tiles, 4096) int16: the 3D tiles' work lists (prepare_items)
     table_counts: torch.Tensor | None = None  # (tiles,) int32
-    plan: torch.Tensor | None = None  # uint8: the permutation plan (permutation_plan), made on first use
+    plan: torch.Tensor | None = None  # uint8: the permutation plan (permutation_plan), made when it pays
+    moves: int = 0  # single-row permutations so far without a plan
 
     @property
     def n(self) -> int:
@@ -155,6 +156,15 @@ class MarkerBins:
                 self.plan = plan
         return self.plan
 
+    def _plan_for(self, rows: int) -> torch.Tensor | None:
+        """The plan for a permutation of ``rows`` rows: the existing one, or a new one when it
+        pays -- several rows at once, or a second single-row permutation of these bins (one
+        row of scattered accesses costs about what making the plan does)."""
+        if self.plan is None and rows < 2 and self.moves < 1:
+            self.moves += 1
+            return None
+        return self.permutation_plan()
+
     def unsort(self, sorted_rows: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """Rows (k, N) in bins order back to the input order (mpf_perm_rows through the bins'
         plan, or mpf_unsort_rows)."""
@@ -166,7 +176,7 @@ class MarkerBins:
         if not ok:  # other dtypes or layouts: torch's own scatter
             out.index_copy_(1, self.order, sorted_rows)
             return out
-        plan = self.permutation_plan()
+        plan = self._plan_for(sorted_rows.shape[0])
         with torch.cuda.device(out.device):
             if plan is not None:
                 _lib.call("mpf_perm_rows", plan.data_ptr(), self.n, 0, sorted_rows.data_ptr(), self.n, out.data_ptr(),
@@ -179,7 +189,7 @@ class MarkerBins:
     def to_bins_order(self, rows: torch.Tensor) -> torch.Tensor:
         """Rows (k, N) of these markers in the input order, in the bins' order (contiguous):
         mpf_perm_rows through the bins' plan, or torch.index_select."""
-        plan = (self.permutation_plan() if rows.dtype in (torch.float32, torch.float64) and rows.dim() == 2
+        plan = (self._plan_for(rows.shape[0]) if rows.dtype in (torch.float32, torch.float64) and rows.dim() == 2
                 and rows.shape[1] == self.n and rows.is_cuda else None)
         if plan is None:
             return torch.index_select(rows, 1, self.order)
@@ -302,7 +312,7 @@ def with_values(bins: MarkerBins, values, scale_exp: int = _lib.SCALE_AUTO) -> M
         _lib.call("mpf_fixed_scale", val.data_ptr(), vs, K._dtype_code(val), bins.n, val.shape[0], int(scale_exp),
                   scalars.data_ptr(), None, K.stream_handle(val.device))
     return MarkerBins(bins.kind, bins.degree, bins.geometry, bins.positions, sval, bins.order, bins.start, scalars,
-                      bins.tables, bins.table_counts, bins.plan)
+                      bins.tables, bins.table_counts, bins.plan, bins.moves)
 
 
 def _check_result(bins: MarkerBins, result, ncomp: int) -> None:

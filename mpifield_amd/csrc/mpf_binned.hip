@@ -3463,7 +3463,7 @@ int gather_via_bins(int kind, int degree, int ndim, const void *positions, int64
   rc = mpf_gather_binned(kind, degree, ndim, b.pos, dtype, n, b.start, nullptr, nullptr, field, ncomp, b.res, lower,
                          cell_size, num_nodes, code, ctx, stream);
   if (rc) return rc;
-  if (n >= PERM_MIN && perm_plan_ok(n))
+  if (ncomp >= 2 && n >= PERM_MIN && perm_plan_ok(n))  // one row: scattered writes cost about the plan
     return unsort_planned(b.res, result, result_stride, b.order, ncomp, n, dtype, ctx, stream);
   return unsort_rows_strided(b.res, result, result_stride, b.order, ncomp, n, dtype, stream);
 }
@@ -3519,7 +3519,7 @@ extern "C" int mpf_unsort_rows(const void *sorted, void *out, const int64_t *ord
     return MPF_ERR_ARGUMENT;
   }
   hipStream_t stream = (hipStream_t)stream_;
-  if (rows > 0 && n >= PERM_MIN && perm_plan_ok(n)) {  // a plan in the stream's default context
+  if (rows >= 2 && n >= PERM_MIN && perm_plan_ok(n)) {  // a plan in the stream's default context
     mpf_context *ctx = default_context(stream);
     if (!ctx) return MPF_ERR_HIP;
     return unsort_planned(sorted, out, n, order, rows, n, dtype, ctx, stream);

@@ -78,16 +78,34 @@ def test_plan_reused_across_calls_and_rows():
 
 
 @pytest.mark.parametrize("n", [(1 << 20) - 1, (1 << 20) + 777])
-def test_unsort_rows_below_and_above_the_planned_size(n):
-    """mpf_unsort_rows: scattered writes below 2^20 markers, a plan made in the default
-    context from 2^20 on; the same values either way."""
+@pytest.mark.parametrize("rows", [1, 3])
+def test_unsort_rows_below_and_above_the_planned_size(n, rows):
+    """mpf_unsort_rows: scattered writes below 2^20 markers or for one row, a plan made in the
+    default context otherwise; the same values either way."""
     order = torch.randperm(n, device="cuda")
-    src = torch.randn((3, n), device="cuda")
+    src = torch.randn((rows, n), device="cuda")
     out = torch.empty_like(src)
-    _lib.call("mpf_unsort_rows", src.data_ptr(), out.data_ptr(), order.data_ptr(), 3, n, _lib.F32, stream())
+    _lib.call("mpf_unsort_rows", src.data_ptr(), out.data_ptr(), order.data_ptr(), rows, n, _lib.F32, stream())
     want = torch.empty_like(src)
     want.index_copy_(1, order, src)
     assert torch.equal(out, want)
+
+
+def test_single_row_moves_plan_on_the_second_use():
+    """One row of fresh bins moves by scattered accesses, the second single-row move makes
+    the plan, and both equal the index ops."""
+    n, nodes = 1 << 19, 64
+    rng = np.random.default_rng(9)
+    pos = torch.as_tensor(rng.random((3, n), dtype=np.float32), device="cuda")
+    geometry = (0.0, 0.0, 0.0, 1 / nodes, 1 / nodes, 1 / nodes, nodes, nodes, nodes, 1, 1, 1)
+    bins = binning.bin_markers(pos, None, (2, 3), geometry)
+    row = torch.randn((1, n), device="cuda")
+    assert torch.equal(bins.to_bins_order(row), torch.index_select(row, 1, bins.order))
+    assert bins.plan is None
+    want = torch.empty_like(row)
+    want.index_copy_(1, bins.order, row)
+    assert torch.equal(bins.unsort(row), want)
+    assert bins.plan is not None
 
 
 def test_bins_unsort_to_bins_order_and_with_values():
