@@ -73,6 +73,38 @@ def algorithmic_bytes(n: int, nodes: int, comps: int, ndim: int = 3, marker_byte
 
 
 TRAFFIC_SUMMARY = ROOT / "profiles" / "quadratic3d_256cubed_64M_traffic.json"
+PATHS_VALU = ROOT / "profiles" / "paths_valu.json"
+
+
+def paths_valu_roofline(kernel: str, paths: int, kernel_ms: float) -> dict | None:
+    """The VALU roofline of a tiled path deposit: the lane operations per path that rocprofv3
+    counted (committed summary) times the paths, over the kernel's HIP-event time."""
+    if not PATHS_VALU.exists() or kernel_ms <= 0.0:
+        return None
+    summary = json.loads(PATHS_VALU.read_text())
+    ops = summary["valu_lane_ops_per_path"].get(kernel)
+    if ops is None:
+        return None
+    peak = summary["peak_lane_ops_per_s"] / 1e9
+    achieved = ops * paths / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "valu", "kernel": kernel, "achieved": achieved, "peak": peak, "unit": "G lane-ops/s",
+            "frac": achieved / peak, "lane_ops_per_path": ops, "kernel_ms": kernel_ms,
+            "source": str(PATHS_VALU.relative_to(ROOT))}
+
+
+def time_binned_paths(torch, fn, steps: int) -> tuple[float, float]:
+    """(ms per call of fn by HIP events, ms per call of the library's spline_paths_tiled)."""
+    from mpifield_amd import _lib
+
+    prof = Profile(_lib.load())
+    prof.reset(False)
+    fn()
+    torch.cuda.synchronize()
+    prof.reset(True)
+    ms = time_events(torch, fn, steps, 0)
+    kms, count = prof.query("spline_paths_tiled")
+    prof.reset(False)
+    return ms, (kms / count if count else 0.0)
 
 
 def measured_traffic(kernel: str, n: int, nodes: int, comps: int) -> tuple[float | None, str | None]:
@@ -231,7 +263,8 @@ def config3(torch, args) -> dict:
         out[label] = ms
     bins = binning.bin_markers(start, None, (_lib.KIND_SPLINE, 2), g)  # before timing
     end_b, q_b = end[:, bins.order].contiguous(), q[bins.order].contiguous()
-    out["binned"] = time_events(torch, lambda: binning.deposit_paths_binned(bins, end_b, q_b, j, 1, 2), args.steps, 1)
+    out["binned"], kernel_ms = time_binned_paths(torch, lambda: binning.deposit_paths_binned(bins, end_b, q_b, j, 1, 2),
+                                                 args.steps)
     j.zero_()
     binning.deposit_paths_binned(bins, end_b, q_b, j, 1, 2)
     rho0 = torch.zeros((nodes, nodes, 1), dtype=torch.float64, device="cuda")
@@ -258,6 +291,7 @@ def config3(torch, args) -> dict:
                    "timing), tiled fixed point"},
         "roofline": {"bound": "hbm", "achieved": alg / (out["binned"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg / (out["binned"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "roofline_valu": paths_valu_roofline("spline_paths_tiled<float, 2, 2, 256>", n, kernel_ms),
         "variants": {"sort_inside_ms": out["fixed"], "sort_inside_paths_per_s": n / (out["fixed"] * 1e-3),
                      "fp64_atomic_ms": out["atomic"], "fp64_atomic_paths_per_s": n / (out["atomic"] * 1e-3)},
         "continuity_max_abs_residual": residual, "total_charge": float(n), "cpu_baseline": cpu,
@@ -286,7 +320,8 @@ def config10(torch, args) -> dict:
         out[label] = time_events(torch, lambda: K.deposit_paths(1, 2, start, end, q, j, g, mode=mode), args.steps, 1)
     bins = binning.bin_markers(start, None, (_lib.KIND_SPLINE, 2), g)  # before timing
     end_b, q_b = end[:, bins.order].contiguous(), q[bins.order].contiguous()
-    out["binned"] = time_events(torch, lambda: binning.deposit_paths_binned(bins, end_b, q_b, j, 1, 2), args.steps, 1)
+    out["binned"], kernel_ms = time_binned_paths(torch, lambda: binning.deposit_paths_binned(bins, end_b, q_b, j, 1, 2),
+                                                 args.steps)
     j.zero_()
     binning.deposit_paths_binned(bins, end_b, q_b, j, 1, 2)
     rho0 = torch.zeros((nodes, nodes, nodes, 1), dtype=torch.float64, device="cuda")
@@ -315,6 +350,7 @@ def config10(torch, args) -> dict:
                    "timing), tiled fixed point"},
         "roofline": {"bound": "hbm", "achieved": alg / (out["binned"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg / (out["binned"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "roofline_valu": paths_valu_roofline("spline_paths_tiled<float, 3, 2, 512>", n, kernel_ms),
         "variants": {"sort_inside_ms": out["fixed"], "sort_inside_paths_per_s": n / (out["fixed"] * 1e-3),
                      "fp64_atomic_ms": out["atomic"]},
         "continuity_max_abs_residual": residual, "cpu_baseline": cpu,
