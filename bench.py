@@ -74,6 +74,7 @@ def algorithmic_bytes(n: int, nodes: int, comps: int, ndim: int = 3, marker_byte
 
 TRAFFIC_SUMMARY = ROOT / "profiles" / "quadratic3d_256cubed_64M_traffic.json"
 PATHS_VALU = ROOT / "profiles" / "paths_valu.json"
+HEADLINE_VALU = ROOT / "profiles" / "quadratic3d_256cubed_64M_valu.json"
 
 
 def paths_valu_roofline(kernel: str, paths: int, kernel_ms: float) -> dict | None:
@@ -762,6 +763,17 @@ def main() -> None:
     achieved = dk["GBs"]
     launches = dk["launches_per_step"]
     traffic, traffic_source = measured_traffic(dominant, n, nodes, comps)
+    roofline_valu = None
+    if (n, nodes, comps) == (1 << 26, 256, 3) and HEADLINE_VALU.exists():
+        # the dominant kernel against the VALU issue rate as well: lane operations per marker
+        # counted by rocprofv3 (committed summary) over its HIP-event time
+        vs = json.loads(HEADLINE_VALU.read_text())
+        ops = vs["valu_lane_ops_per_marker"][dominant]
+        peak = vs["peak_lane_ops_per_s"] / 1e9
+        ach = ops * n / (dk["ms_per_step"] * 1e-3) / 1e9
+        roofline_valu = {"bound": "valu", "kernel": dominant, "achieved": ach, "peak": peak, "unit": "G lane-ops/s",
+                         "frac": ach / peak, "lane_ops_per_marker": ops,
+                         "source": str(HEADLINE_VALU.relative_to(ROOT))}
     line = {
         "metric": METRIC,
         "value": value,
@@ -797,6 +809,7 @@ def main() -> None:
             "algorithmic_bytes_per_launch": dk["algorithmic_bytes"] / launches if launches else None,
             "launches_per_step": launches,
         },
+        "roofline_valu": roofline_valu,
     }
     if not args.only_headline:
         # ---- error against the float64 oracle on a sample of markers ----
