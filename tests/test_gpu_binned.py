@@ -420,3 +420,36 @@ def test_no_markers_leave_the_field_and_raise_nothing():
             K.gather_edges(kind, degree, pos[:2], edges, res[:2], grid2)
         torch.cuda.synchronize()
         assert bool((field == 0.5).all()) and not bool(edges.any())
+
+
+@pytest.mark.parametrize("dtype,ncomp", [(torch.float32, 3), (torch.float32, 1), (torch.float64, 3)])
+def test_binning_of_a_batch_agrees_with_binning_its_halves(dtype, ncomp):
+    """5.2M markers on 128^3 (2049 groups of 1024 keys) and each half binned on its own (1025
+    groups): every marker lands in the same bin either way, the order is a permutation and
+    the sorted rows are the input rows in that order."""
+    print(f"batch vs halves dtype={dtype} ncomp={ncomp}", flush=True)
+    nodes, n = 128, (1 << 22) + (1 << 20) + 4321
+    g = (0.0, 0.0, 0.0, 1 / nodes, 1 / nodes, 1 / nodes, nodes, nodes, nodes, 0, 0, 0)
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    pos = torch.rand((3, n), generator=gen, device="cuda", dtype=torch.float32).to(dtype)
+    val = torch.randn((ncomp, n), generator=gen, device="cuda", dtype=torch.float32).to(dtype)
+
+    def bin_of_input(bins):
+        count = bins.order.numel()
+        start = bins.start.long() & 0xFFFFFFFF
+        idx = torch.arange(count, device="cuda")
+        sorted_bin = torch.searchsorted(start, idx, right=True) - 1
+        per_input = torch.empty_like(sorted_bin)
+        per_input[bins.order] = sorted_bin
+        return per_input
+
+    full = binning.bin_markers(pos, val, (1, 2), g)
+    assert int(full.start[-1].item()) == n
+    assert torch.equal(torch.sort(full.order).values, torch.arange(n, device="cuda"))
+    assert torch.equal(full.positions, pos[:, full.order])
+    assert torch.equal(full.values, val[:, full.order])
+    whole = bin_of_input(full)
+    half = n // 2
+    for lo, hi in ((0, half), (half, n)):
+        part = binning.bin_markers(pos[:, lo:hi].contiguous(), val[:, lo:hi].contiguous(), (1, 2), g)
+        assert torch.equal(bin_of_input(part), whole[lo:hi]), (lo, hi)
